@@ -1,0 +1,378 @@
+// Body coupling: Yang3 regularized delta (structure-ops.jl), interpolation E, spreading Eᵀ,
+// surface-force solvers (dense inverse GEMV and CG, cnab/ops.jl:79-114) and the per-body
+// force coefficients (streamfcn/states.jl:53-84).  Reductions are FP64.
+#include "ibh_internal.h"
+
+namespace ibh {
+
+static constexpr int SUPP = 6, NS = 13;
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// δh, structure-ops.jl:114-171 — Yang et al. (JCP 2009) smoothed delta times h; the
+// coefficients are the reference's literals.
+__device__ double deltah(double rf, double rb, double dr) {
+    double r = fabs(rf - rb);
+    double r1 = r / dr;
+    double r2 = r1 * r1;
+    double r3 = r2 * r1;
+    double r4 = r3 * r1;
+    if (r1 <= 1.0) {
+        double a5 = asin((1.0 / 2.0) * sqrt(3.0) * (2.0 * r1 - 1.0));
+        double a8 = sqrt(1.0 - 12.0 * r2 + 12.0 * r1);
+        return 4.166666667e-2 * r4 + (-0.1388888889 + 3.472222222e-2 * a8) * r3 +
+               (-7.121664902e-2 - 5.208333333e-2 * a8 + 0.2405626122 * a5) * r2 +
+               (-0.2405626122 * a5 - 0.3792313933 + 0.1012731481 * a8) * r1 +
+               8.0187537413e-2 * a5 - 4.195601852e-2 * a8 + 0.6485698427;
+    } else if (r1 <= 2.0) {
+        double a6 = asin((1.0 / 2.0) * sqrt(3.0) * (-3.0 + 2.0 * r1));
+        double a9 = sqrt(-23.0 + 36.0 * r1 - 12.0 * r2);
+        return -6.250000000e-2 * r4 + (0.4861111111 - 1.736111111e-2 * a9) * r3 +
+               (-1.143175026 + 7.812500000e-2 * a9 - 0.1202813061 * a6) * r2 +
+               (0.8751991178 + 0.3608439183 * a6 - 0.1548032407 * a9) * r1 - 0.2806563809 * a6 +
+               8.22848104e-3 + 0.1150173611 * a9;
+    } else if (r1 <= 3.0) {
+        double a1 = asin((1.0 / 2.0 * (2.0 * r1 - 5.0)) * sqrt(3.0));
+        double a7 = sqrt(-71.0 - 12.0 * r2 + 60.0 * r1);
+        return 2.083333333e-2 * r4 + (3.472222222e-3 * a7 - 0.2638888889) * r3 +
+               (1.214391675 - 2.604166667e-2 * a7 + 2.405626122e-2 * a1) * r2 +
+               (-0.1202813061 * a1 - 2.449273192 + 7.262731481e-2 * a7) * r1 +
+               0.1523563211 * a1 + 1.843201677 - 7.306134259e-2 * a7;
+    }
+    return 0.0;
+}
+
+// Reg update!, structure-ops.jl:37-61.  The 13×13 weight of point k is the product of two
+// 1-D factors, stored separately: w[k][0][s] = δh(x_s), w[k][1][t] = δh(y_t + h/2) for the
+// x-flux, w[k][2][s] = δh(x_s + h/2), w[k][3][t] = δh(y_t) for the y-flux.
+__global__ void k_reg_update(Dom d, RegDev r, const double *__restrict__ pts, double h, double x0,
+                             double y0) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r.nb * 4 * NS) return;
+    const int k = t / (4 * NS), c = (t / NS) % 4, s = t % NS;
+    const double xb = pts[k], yb = pts[r.nb + k];
+    const int bx = (int)floor((xb - x0) / h), by = (int)floor((yb - y0) / h);
+    double v;
+    if (c == 0) v = deltah(h * (double)(bx - 1 + (s - SUPP)) + x0, xb, h);
+    else if (c == 1) v = deltah(h * (double)(by - 1 + (s - SUPP)) + y0 + h / 2, yb, h);
+    else if (c == 2) v = deltah(h * (double)(bx - 1 + (s - SUPP)) + x0 + h / 2, xb, h);
+    else v = deltah(h * (double)(by - 1 + (s - SUPP)) + y0, yb, h);
+    r.w[t] = v;
+    if (c == 0 && s == 0) {
+        r.bidx[k] = bx;
+        r.bidx[r.nb + k] = by;
+        if (bx - SUPP < 1 || bx + SUPP > d.nx || by - SUPP < 1 || by + SUPP > d.ny)
+            atomicOr(r.err, 1);
+    }
+}
+
+void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double h, double x0,
+                       double y0, hipStream_t s) {
+    if (r.nb == 0) return;
+    hipLaunchKernelGGL(k_reg_update, dim3(nblk((long)r.nb * 4 * NS, 256)), dim3(256), 0, s, d, r,
+                       pts, h, x0, y0);
+}
+
+// E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w, summed column-major like the
+// reference's sumprod; q0 (constant per component) is added to q first, as in Q = qs + q0.
+__global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
+                    double *__restrict__ fb, long sfb, const int *skip) {
+    if (skip && *skip) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * r.nb) return;
+    const int b = blockIdx.y;
+    q += b * sq;
+    const int comp = t / r.nb, k = t % r.nb;
+    const int bx = r.bidx[k], by = r.bidx[r.nb + k];
+    const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
+    const double *wy = wx + NS;
+    double acc = 0.0;
+    if (comp == 0) {
+        for (int tt = 0; tt < NS; ++tt) {
+            const int j = by + tt - SUPP;
+            const double *col = q + (long)(d.nx + 1) * (j - 1) - 1;
+            for (int s = 0; s < NS; ++s) acc += (col[bx + s - SUPP] + u0) * (wx[s] * wy[tt]);
+        }
+    } else {
+        for (int tt = 0; tt < NS; ++tt) {
+            const int j = by + tt - SUPP;
+            const double *col = q + d.nu + (long)d.nx * (j - 1) - 1;
+            for (int s = 0; s < NS; ++s) acc += (col[bx + s - SUPP] + v0) * (wx[s] * wy[tt]);
+        }
+    }
+    fb[b * sfb + t] = acc;
+}
+
+void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
+              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s) {
+    if (r.nb == 0) return;
+    hipLaunchKernelGGL(k_E, dim3(nblk(2L * r.nb, 128), nbatch), dim3(128), 0, s, d, r, q, sq, u0,
+                       v0, fb, sfb, skip);
+}
+
+// Eᵀ (Reg), structure-ops.jl:63-86: q .= 0 then scatter-add the 13×13 stencils of every point.
+// Points are handled one stencil row per thread; the adds use FP64 atomics.
+__global__ void k_zero(double *__restrict__ p, long n, long sp, const int *skip) {
+    if (skip && *skip) return;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) p[blockIdx.y * sp + t] = 0.0;
+}
+
+__global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
+                     double *__restrict__ q, long sq, const int *skip) {
+    if (skip && *skip) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * r.nb * NS * NS) return;
+    const int b = blockIdx.y;
+    q += b * sq;
+    fb += b * sfb;
+    const int s = t % NS, tt = (t / NS) % NS, kc = t / (NS * NS);
+    const int comp = kc / r.nb, k = kc % r.nb;
+    const int bx = r.bidx[k], by = r.bidx[r.nb + k];
+    const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
+    const double f = fb[comp * r.nb + k];
+    const double v = (wx[s] * wx[NS + tt]) * f;
+    const int i = bx + s - SUPP, j = by + tt - SUPP;
+    double *dst = comp == 0 ? q + (i - 1) + (long)(d.nx + 1) * (j - 1)
+                            : q + d.nu + (i - 1) + (long)d.nx * (j - 1);
+    if (comp == 0 && !isfinite(v)) atomicOr(r.err, 2);
+    atomicAdd(dst, v);
+}
+
+void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
+               int nbatch, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq, skip);
+    if (r.nb == 0) return;
+    long n = 2L * r.nb * NS * NS;
+    hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, q, sq,
+                       skip);
+}
+
+// y = A·x for the dense n×n column-major Binv (cnab/ops.jl:97)
+__global__ void k_gemv(int n, const double *__restrict__ A, const double *__restrict__ x,
+                       double *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += A[i + (long)n * j] * x[j];
+    y[i] = s;
+}
+
+void launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gemv, dim3(nblk(n, 64)), dim3(64), 0, s, n, A, x, y);
+}
+
+__global__ void k_axpy_sub(int n, double *y, const double *v, double a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] -= v[i] * a;
+}
+
+void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_axpy_sub, dim3(nblk(n, 256)), dim3(256), 0, s, n, y, v, a);
+}
+
+// ---------------------------------------------------------------- block reductions (FP64)
+template <int BS>
+__device__ double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < BS / 64; ++k) t += sh[k];
+        sh[BS / 64] = t;
+    }
+    __syncthreads();
+    t = sh[BS / 64];
+    __syncthreads();
+    return t;
+}
+
+// update_stress!, streamfcn/states.jl:53-84: per body fb = F·h/dt, CD = 2Σfb_x, CL = 2Σfb_y,
+// then fb ./= ds.  One block per body.
+__global__ void k_stress(const int *boff, const int *npts, int nb, const double *__restrict__ Fb,
+                         const double *__restrict__ ds, double h, double dt, double *CD,
+                         double *CL, double *fbs) {
+    __shared__ double sh[8];
+    const int b = blockIdx.x;
+    const int o = boff[b], n = npts[b];
+    double sx = 0.0, sy = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double fx = Fb[o + k] * h / dt, fy = Fb[nb + o + k] * h / dt;
+        sx += fx;
+        sy += fy;
+        fbs[o + k] = fx / ds[o + k];
+        fbs[nb + o + k] = fy / ds[o + k];
+    }
+    sx = block_sum<256>(sx, sh);
+    sy = block_sum<256>(sy, sh);
+    if (threadIdx.x == 0) {
+        CD[b] = 2 * sx;
+        CL[b] = 2 * sy;
+    }
+}
+
+void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
+                   const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
+                   hipStream_t s) {
+    if (nbodies == 0) return;
+    hipLaunchKernelGGL(k_stress, dim3(nbodies), dim3(256), 0, s, boff, npts, nb, Fb, ds, h, dt,
+                       CD, CL, fbs);
+}
+
+// ---------------------------------------------------------------- CG (IterativeSolvers 0.9.2)
+// cg!(x, A, b; reltol, maxiter) with the warm start x: r = b - A x, tol = reltol·‖r‖; each
+// iteration u = r + (‖r‖²/‖r_prev‖²) u, c = A u, α = ‖r‖²/(u·c), x += α u, r -= α c.  The stop
+// test (it ≥ maxiter or ‖r‖ ≤ tol) runs before every iteration.  One block does the vector
+// work; the device flag `done` makes the surrounding B-products skip once converged.
+__global__ void k_cg_init(CGDev cg) {
+    __shared__ double sh[17];
+    const int n = cg.n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double ri = cg.b[i] - cg.c[i];
+        cg.r[i] = ri;
+        cg.u[i] = 0.0;
+        s += ri * ri;
+    }
+    s = block_sum<1024>(s, sh);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(s);
+        cg.sc[0] = res;
+        cg.sc[1] = 1.0;
+        cg.sc[2] = cg.reltol * res;
+        *cg.it = 0;
+        *cg.done = (cg.maxiter <= 0 || res <= cg.reltol * res) ? 1 : 0;
+    }
+}
+
+__global__ void k_cg_dir(CGDev cg) {
+    if (*cg.done) return;
+    const double res = cg.sc[0], prev = cg.sc[1];
+    const double beta = (res * res) / (prev * prev);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cg.n; i += gridDim.x * blockDim.x)
+        cg.u[i] = cg.r[i] + beta * cg.u[i];
+}
+
+__global__ void k_cg_update(CGDev cg) {
+    __shared__ double sh[17];
+    if (*cg.done) return;
+    const int n = cg.n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.u[i] * cg.c[i];
+    s = block_sum<1024>(s, sh);
+    const double res = cg.sc[0];
+    const double alpha = (res * res) / s;
+    double s2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        cg.x[i] += alpha * cg.u[i];
+        const double ri = cg.r[i] - alpha * cg.c[i];
+        cg.r[i] = ri;
+        s2 += ri * ri;
+    }
+    s2 = block_sum<1024>(s2, sh);
+    if (threadIdx.x == 0) {
+        const double nres = sqrt(s2);
+        cg.sc[1] = res;
+        cg.sc[0] = nres;
+        const int it = *cg.it + 1;
+        *cg.it = it;
+        if (it >= cg.maxiter || nres <= cg.sc[2]) *cg.done = 1;
+    }
+}
+
+void launch_cg_init(const CGDev &cg, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1024), 0, s, cg);
+}
+void launch_cg_dir(const CGDev &cg, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_dir, dim3(nblk(cg.n, 256)), dim3(256), 0, s, cg);
+}
+void launch_cg_update(const CGDev &cg, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, s, cg);
+}
+
+// ---------------------------------------------------------------- dense inverse
+// Gauss–Jordan elimination with partial pivoting on [A | Inv] (the role of LAPACK getrf/getri
+// behind inv(Bmat), cnab/ops.jl:96).  Step k: pick the pivot row, swap and scale row k, then
+// eliminate column k from every other row.  ws = [pivot value, multipliers (n)].
+__global__ void k_gj_pivot(int n, const double *__restrict__ A, int k, int *piv, double *ws,
+                           int *err) {
+    __shared__ double bv[1024];
+    __shared__ int bi[1024];
+    double best = -1.0;
+    int bidx = k;
+    for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+        const double v = fabs(A[i + (long)n * k]);
+        if (v > best) { best = v; bidx = i; }
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double v2 = bv[threadIdx.x + o];
+            const int i2 = bi[threadIdx.x + o];
+            if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+                bv[threadIdx.x] = v2;
+                bi[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *piv = bi[0];
+        ws[0] = A[bi[0] + (long)n * k];
+        if (bv[0] == 0.0) atomicOr(err, 4);
+    }
+}
+
+// swap rows k and piv of both matrices, then scale row k by 1/pivot
+__global__ void k_gj_swap_scale(int n, double *A, double *Inv, int k, const int *piv,
+                                const double *ws) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 2 * n) return;
+    double *M = j < n ? A : Inv;
+    const int c = j < n ? j : j - n;
+    const int p = *piv;
+    const double pv = ws[0];
+    double a = M[k + (long)n * c], b = M[p + (long)n * c];
+    if (p != k) {
+        M[p + (long)n * c] = a;
+        a = b;
+    }
+    M[k + (long)n * c] = a / pv;
+}
+
+__global__ void k_gj_col(int n, const double *A, int k, double *ws) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ws[1 + i] = i == k ? 0.0 : A[i + (long)n * k];
+}
+
+__global__ void k_gj_elim(int n, double *A, double *Inv, int k, const double *ws) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int i = (int)(t % n);
+    const long jj = t / n;
+    if (jj >= 2L * n) return;
+    const int j = (int)jj;
+    double *M = j < n ? A : Inv;
+    const int c = j < n ? j : j - n;
+    if (j < n && c < k) return;  // already eliminated columns of A stay e_c
+    const double f = ws[1 + i];
+    if (f != 0.0) M[i + (long)n * c] -= f * M[k + (long)n * c];
+}
+
+void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err,
+                    hipStream_t s) {
+    double *ws = reinterpret_cast<double *>(piv_row + 2);  // caller sizes the workspace
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, n, A, k, piv_row, ws, err);
+    hipLaunchKernelGGL(k_gj_swap_scale, dim3(nblk(2L * n, 256)), dim3(256), 0, s, n, A, Inv, k,
+                       piv_row, ws);
+    hipLaunchKernelGGL(k_gj_col, dim3(nblk(n, 256)), dim3(256), 0, s, n, A, k, ws);
+    hipLaunchKernelGGL(k_gj_elim, dim3(nblk(2L * n * n, 256)), dim3(256), 0, s, n, A, Inv, k, ws);
+}
+
+}  // namespace ibh

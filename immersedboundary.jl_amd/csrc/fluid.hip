@@ -1,0 +1,328 @@
+// Grid stencils of the streamfunction–vorticity discretization (fluid-ops.jl), one thread per
+// output value, x-fastest so consecutive lanes touch consecutive addresses.  Every kernel is
+// batched over levels (or over independent right-hand sides) with element strides.
+#include "ibh_internal.h"
+
+namespace ibh {
+
+#define SKIP_GUARD \
+    if (skip && *skip) return;
+
+// 1-based accessors matching the reference's indexing
+#define GI(a, i, j) (a)[((i)-1) + (long)d.m * ((j)-1)]
+#define QX(a, i, j) (a)[((i)-1) + (long)(d.nx + 1) * ((j)-1)]
+#define QY(a, i, j) (a)[d.nu + ((i)-1) + (long)d.nx * ((j)-1)]
+#define BC(a, k) (a)[(k)-1]
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182 — each q entry written once.
+__global__ void k_curl_bc(Dom d, const double *__restrict__ psi, long spsi,
+                          const double *__restrict__ pbc, long sbc, double *__restrict__ q,
+                          long sq, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.nq) return;
+    const int b = blockIdx.y;
+    psi += b * spsi;
+    pbc += b * sbc;
+    q += b * sq;
+    const int nx = d.nx, ny = d.ny;
+    double v;
+    if (t < d.nu) {
+        const int i = (int)(t % (nx + 1)) + 1, j = (int)(t / (nx + 1)) + 1;
+        if (i == 1) v = BC(pbc, (d.Lo + 1) + j) - BC(pbc, d.Lo + j);
+        else if (i == nx + 1) v = BC(pbc, (d.Ro + 1) + j) - BC(pbc, d.Ro + j);
+        else if (j == 1) v = GI(psi, i - 1, j) - BC(pbc, d.Bo + i);
+        else if (j == ny) v = BC(pbc, i + d.To) - GI(psi, i - 1, j - 1);
+        else v = GI(psi, i - 1, j) - GI(psi, i - 1, j - 1);
+    } else {
+        const long u = t - d.nu;
+        const int i = (int)(u % nx) + 1, j = (int)(u / nx) + 1;
+        if (j == 1) v = BC(pbc, d.Bo + i) - BC(pbc, (d.Bo + 1) + i);
+        else if (j == ny + 1) v = BC(pbc, d.To + i) - BC(pbc, (d.To + 1) + i);
+        else if (i == 1) v = BC(pbc, d.Lo + j) - GI(psi, i, j - 1);
+        else if (i == nx) v = GI(psi, i - 1, j - 1) - BC(pbc, d.Ro + j);
+        else v = GI(psi, i - 1, j - 1) - GI(psi, i, j - 1);
+    }
+    q[t] = v;
+}
+
+void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
+                    double *q, long sq, int nbatch, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl_bc, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, d, psi, spsi,
+                       pbc, sbc, q, sq, skip);
+}
+
+// rot! (C'), fluid-ops.jl:117-129
+__global__ void k_rot(Dom d, const double *__restrict__ q, long sq, double *__restrict__ g,
+                      long sg, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    q += b * sq;
+    g += b * sg;
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;  // Γ[i-1, j-1]
+    g[t] = (QX(q, i, j - 1) - QX(q, i, j)) + (QY(q, i, j) - QY(q, i - 1, j));
+}
+
+void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
+                const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_rot, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, q, sq, g, sg,
+                       skip);
+}
+
+// coarsify!(Γc, Γ), fluid-ops.jl:184-202: fine (i, j) = (2a + 2, 2b + 2) → coarse
+// (nx/4 + 1 + a, ny/4 + 1 + b), a in 0..nx/2-3, b in 0..ny/2-3.
+__global__ void k_coarsify(Dom d, double *__restrict__ gc, const double *__restrict__ g,
+                           long sgc, long sg, const int *skip) {
+    SKIP_GUARD
+    const int na = d.nx / 2 - 1, nbb = d.ny / 2 - 1;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)na * nbb) return;
+    const int bz = blockIdx.y;
+    gc += bz * sgc;
+    g += bz * sg;
+    const int a = (int)(t % na), b = (int)(t / na);
+    const int i = 2 + 2 * a, j = 2 + 2 * b;
+    const int ic = d.nx / 4 + 1 + a, jc = d.ny / 4 + 1 + b;
+    GI(gc, ic, jc) = GI(g, i, j) +
+                     0.5 * (GI(g, i + 1, j) + GI(g, i, j + 1) + GI(g, i - 1, j) + GI(g, i, j - 1)) +
+                     0.25 * (GI(g, i + 1, j + 1) + GI(g, i + 1, j - 1) + GI(g, i - 1, j - 1) +
+                             GI(g, i - 1, j + 1));
+}
+
+void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
+                     const int *skip, hipStream_t s) {
+    long n = (long)(d.nx / 2 - 1) * (d.ny / 2 - 1);
+    hipLaunchKernelGGL(k_coarsify, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, gc, g, sgc,
+                       sg, skip);
+}
+
+// get_bc!(rbc, r), fluid-ops.jl:210-239, including the trailing ×0.25
+__global__ void k_get_bc(Dom d, const double *__restrict__ r, long sr, double *__restrict__ rbc,
+                         long sbc, const int *skip) {
+    SKIP_GUARD
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nbc) return;
+    const int b = blockIdx.y;
+    r += b * sr;
+    rbc += b * sbc;
+    const int nx = d.nx, ny = d.ny;
+    const int k = t + 1;  // 1-based bc index
+    double v;
+    if (k > d.To) {  // top: ibc = k - T in 1..nx+1
+        const int ibc = k - d.To;
+        if (ibc & 1) v = GI(r, nx / 4 + (ibc - 1) / 2, 3 * ny / 4);
+        else {
+            const int i = nx / 4 + ibc / 2;
+            v = 0.5 * (GI(r, i, 3 * ny / 4) + GI(r, i - 1, 3 * ny / 4));
+        }
+    } else if (k > d.Bo) {  // bottom
+        const int ibc = k - d.Bo;
+        if (ibc & 1) v = GI(r, nx / 4 + (ibc - 1) / 2, ny / 4);
+        else {
+            const int i = nx / 4 + ibc / 2;
+            v = 0.5 * (GI(r, i, ny / 4) + GI(r, i - 1, ny / 4));
+        }
+    } else if (k > d.Ro) {  // right: jbc in 1..ny+1
+        const int jbc = k - d.Ro;
+        if (jbc & 1) v = GI(r, 3 * nx / 4, ny / 4 + (jbc - 1) / 2);
+        else {
+            const int j = ny / 4 + jbc / 2;
+            v = 0.5 * (GI(r, 3 * nx / 4, j) + GI(r, 3 * nx / 4, j - 1));
+        }
+    } else {  // left
+        const int jbc = k - d.Lo;
+        if (jbc & 1) v = GI(r, nx / 4, ny / 4 + (jbc - 1) / 2);
+        else {
+            const int j = ny / 4 + jbc / 2;
+            v = 0.5 * (GI(r, nx / 4, j) + GI(r, nx / 4, j - 1));
+        }
+    }
+    rbc[t] = v * 0.25;
+}
+
+void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
+                   const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_get_bc, dim3(nblk(d.nbc, 256), nbatch), dim3(256), 0, s, d, r, sr, rbc,
+                       sbc, skip);
+}
+
+// out = in + fac·edges(rbc)  (apply_bc!, fluid-ops.jl:249-275, fused with the copy into Γtmp)
+__global__ void k_copy_apply_bc(Dom d, const double *__restrict__ in, long sin,
+                                double *__restrict__ out, long sout,
+                                const double *__restrict__ rbc, long sbc, double fac,
+                                const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    in += b * sin;
+    out += b * sout;
+    rbc += b * sbc;
+    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    double v = in ? in[t] : 0.0;
+    // same order as the reference: bottom, top, left, right
+    if (j == 1) v += fac * BC(rbc, (d.Bo + 1) + i);
+    if (j == d.ny - 1) v += fac * BC(rbc, (d.To + 1) + i);
+    if (i == 1) v += fac * BC(rbc, (d.Lo + 1) + j);
+    if (i == d.nx - 1) v += fac * BC(rbc, (d.Ro + 1) + j);
+    out[t] = v;
+}
+
+void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out, long sout,
+                          const double *rbc, long sbc, double fac, int nbatch, const int *skip,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_apply_bc, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, in,
+                       sin, out, sout, rbc, sbc, fac, skip);
+}
+
+void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long sbc, double fac,
+                     int nbatch, const int *skip, hipStream_t s) {
+    launch_copy_apply_bc(d, r, sr, r, sr, rbc, sbc, fac, nbatch, skip, s);
+}
+
+// Nonlinear term at one level: avg_flux! + direct_product! + rot! + 1/hc² (cnab/ops.jl:151-168,
+// fluid-ops.jl:277-401).  q0 is the constant base flux (ux·hc, uy·hc) (fluid-ops.jl:403-417).
+struct NLCtx {
+    Dom d;
+    const double *q, *g, *gbc;
+    double U, V, gs;  // q0x, q0y values; scale of gbc
+};
+
+__device__ inline double Qx_avg(const NLCtx &c, int i, int j) {  // j in 2..ny
+    const Dom &d = c.d;
+    return (QX(c.q, i, j) + QX(c.q, i, j - 1) + c.U + c.U) / 2;
+}
+__device__ inline double Qy_avg(const NLCtx &c, int i, int j) {  // i in 2..nx
+    const Dom &d = c.d;
+    return (QY(c.q, i, j) + QY(c.q, i - 1, j) + c.V + c.V) / 2;
+}
+__device__ inline double gbcv(const NLCtx &c, int k) { return c.gbc[k - 1] * c.gs; }
+
+// fqx[i, j], i in 1..nx+1, j in 1..ny (zero outside i = 2..nx)
+__device__ inline double fqx(const NLCtx &c, int i, int j) {
+    const Dom &d = c.d;
+    if (i < 2 || i > d.nx) return 0.0;
+    if (j == 1) return (Qy_avg(c, i, j + 1) * GI(c.g, i - 1, j) + Qy_avg(c, i, j) * gbcv(c, d.Bo + i)) / 2;
+    if (j == d.ny)
+        return (Qy_avg(c, i, j) * GI(c.g, i - 1, j - 1) + Qy_avg(c, i, j + 1) * gbcv(c, d.To + i)) / 2;
+    return (Qy_avg(c, i, j + 1) * GI(c.g, i - 1, j) + Qy_avg(c, i, j) * GI(c.g, i - 1, j - 1)) / 2;
+}
+// fqy[i, j], i in 1..nx, j in 1..ny+1 (zero outside j = 2..ny)
+__device__ inline double fqy(const NLCtx &c, int i, int j) {
+    const Dom &d = c.d;
+    if (j < 2 || j > d.ny) return 0.0;
+    if (i == 1)
+        return -(Qx_avg(c, i + 1, j) * GI(c.g, i, j - 1) + Qx_avg(c, i, j) * gbcv(c, d.Lo + j)) / 2;
+    if (i == d.nx)
+        return -(Qx_avg(c, i, j) * GI(c.g, i - 1, j - 1) + Qx_avg(c, i + 1, j) * gbcv(c, d.Ro + j)) / 2;
+    return -(Qx_avg(c, i + 1, j) * GI(c.g, i, j - 1) + Qx_avg(c, i, j) * GI(c.g, i - 1, j - 1)) / 2;
+}
+
+__global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
+                            const double *__restrict__ g, long sg, const double *__restrict__ gbc,
+                            long sbc, double gbc_scale, double *__restrict__ out, long sout,
+                            double ux, double uy, double h, int lev0, int q0_given) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    const int lev = lev0 + b;
+    const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
+    NLCtx c;
+    c.d = d;
+    c.q = q + b * sq;
+    c.g = g + b * sg;
+    c.gbc = gbc + b * sbc;
+    c.U = q0_given ? ux : ux * hc;
+    c.V = q0_given ? uy : uy * hc;
+    c.gs = gbc_scale;
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
+    out[b * sout + t] = v * (1 / (hc * hc));
+}
+
+void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
+                      const double *gbc, long sbc, double gbc_scale, double *out, long sout,
+                      double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
+                      int q0_given) {
+    hipLaunchKernelGGL(k_nonlinear, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, q, sq, g,
+                       sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given);
+}
+
+// accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac[l]·bcraw[l] for ℓ ≤ L-2, accbc[L-1] = 0 (the rhsbc of
+// GetTrialState accumulates over levels, timestepping.jl:41-51).
+__global__ void k_bc_prefix(int nbc, const double *__restrict__ bcraw, double *__restrict__ acc,
+                            const double *__restrict__ fac, int L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbc) return;
+    double s = 0.0;
+    acc[(long)(L - 1) * nbc + t] = 0.0;
+    for (int l = L - 2; l >= 0; --l) {
+        s += fac[l] * bcraw[(long)l * nbc + t];
+        acc[(long)l * nbc + t] = s;
+    }
+}
+
+void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_prefix, dim3(nblk(d.nbc, 256)), dim3(256), 0, s, d.nbc, bcraw, accbc,
+                       fac, L);
+}
+
+__global__ void k_fill(double *p, long n, double v) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+void launch_fill(double *p, long n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+// S[i,k] = sin(π i k / n), i, k = 1..n-1, stored (n-1)×(n-1).  The argument is reduced
+// exactly in integers (i·k mod 2n) before the sine so every entry is correctly rounded.
+__global__ void k_sine_matrix(double *S, int n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long m = n - 1;
+    if (t >= m * m) return;
+    const long i = t % m + 1, k = t / m + 1;
+    const long r = (i * k) % (2L * n);
+    S[t] = sinpi((double)r / (double)n);
+}
+
+void launch_sine_matrix(double *S, int n, hipStream_t s) {
+    long mm = (long)(n - 1) * (n - 1);
+    hipLaunchKernelGGL(k_sine_matrix, dim3(nblk(mm, 256)), dim3(256), 0, s, S, n);
+}
+
+// Eigenvalue scalings between the two transforms (fluid-ops.jl:37,50,58-61; cnab/ops.jl:26,29):
+// sD = s/Λ, sA_ℓ = s/Λexpl_ℓ, sAinv_ℓ = s/Λimpl_ℓ, with s = 1/(4 nx ny) and the 2·2 factors
+// of the two RODFT00 dimensions folded into the S matrices' caller (see host code).
+__global__ void k_scale_tables(Dom d, double *sD, double *sA, double *sAinv, int L, double h,
+                               double dt, double Re, double scale) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const double lam = -2 * (cos(M_PI * i / d.nx) + cos(M_PI * j / d.ny) - 2);
+    sD[t] = scale / lam;
+    for (int l = 0; l < L; ++l) {
+        const double hc = h * (double)(1L << l);
+        const double lexpl = 1.0 / (1.0 - lam * dt / (2 * Re * (hc * hc)));
+        const double limpl = 1.0 + lam * dt / (2 * Re * (hc * hc));
+        sA[(long)l * d.ng + t] = scale / lexpl;
+        sAinv[(long)l * d.ng + t] = scale / limpl;
+    }
+}
+
+void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
+                         double dt, double Re, hipStream_t s) {
+    // 16·s: each RODFT00 is 2·S per dimension, so DST∘DST contributes (2·2)² = 16
+    const double scale = 16.0 / (4.0 * d.nx * d.ny);
+    hipLaunchKernelGGL(k_scale_tables, dim3(nblk(d.ng, 256)), dim3(256), 0, s, d, sD, sA, sAinv,
+                       L, h, dt, Re, scale);
+}
+
+}  // namespace ibh

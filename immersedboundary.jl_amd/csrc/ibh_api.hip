@@ -1,0 +1,851 @@
+// libibhip C ABI: context lifecycle, the CNAB step orchestration and operator entry points.
+//
+// One context = one case, all state resident in HBM, one HIP stream.  The step follows
+// advance! (cnab/timestepping.jl:253-290); the independent per-level work of GetTrialState
+// (timestepping.jl:28-79) is batched over levels in single launches, and the Schur-complement
+// products B·z (cnab/ops.jl:125-143) are batched over columns for the dense-inverse setup.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ibh.h"
+#include "ibh_internal.h"
+
+using namespace ibh;
+
+namespace {
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, double>> rec;  // (kind, flops) per recorded launch
+};
+
+}  // namespace
+
+struct ibh_ctx {
+    Dom d;
+    int L = 1;
+    double h = 0, x0 = 0, y0 = 0, Re = 0, dt = 0, beta[2] = {1.5, -0.5};
+    int nbodies = 0, nb = 0;
+    std::vector<int> npts, moving, boff;
+    bool anymove = false, use_cg = false;
+    int precision = 0, force_solver = 0, cg_maxiter = 5000;
+    double cg_reltol = 1e-12;
+    hipStream_t s = nullptr;
+
+    // transforms
+    double *Sm = nullptr, *Sp = nullptr;
+    double *sD = nullptr, *sA = nullptr, *sAinv = nullptr;
+    // state (StreamFcnQuantities + CNAB.State.nonlin)
+    double *gam = nullptr, *psi = nullptr, *q = nullptr, *nl[2] = {nullptr, nullptr};
+    double ux = 0, uy = 0, t = 0;
+    // solver work
+    double *qs = nullptr, *gs = nullptr, *bcraw = nullptr, *accbc = nullptr, *psibc = nullptr;
+    double *gtmp = nullptr, *fac = nullptr, *qtmp = nullptr, *g1 = nullptr;
+    double *T1 = nullptr, *T2 = nullptr;
+    long Tcap = 0;  // elements in T1/T2
+    // B-product work (batched over nc columns)
+    int ncB = 0;
+    double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,
+           *bcB = nullptr, *g1B = nullptr;
+    // bodies
+    double *pts = nullptr, *vel = nullptr, *ds = nullptr;
+    int *d_boff = nullptr, *d_npts = nullptr;
+    RegDev reg{};
+    int *derr = nullptr;       // device error flags (bit 1 bounds, 2 non-finite, 4 singular)
+    double *Fb = nullptr, *Ftmp = nullptr, *CD = nullptr, *CL = nullptr, *fbs = nullptr;
+    double *Binv = nullptr;
+    CGDev cg{};
+    int cg_last = 0;
+    bool setup_done = false;
+    Prof prof;
+    std::string err;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int fail(ibh_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCK(expr)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(c, IBH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(ibh_ctx *c, T **p, size_t n) {
+    void *v = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&v, n * sizeof(T));
+    if (e != hipSuccess) return fail(c, IBH_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipMemsetAsync(v, 0, n * sizeof(T), c->s);
+    c->allocs.push_back(v);
+    *p = static_cast<T *>(v);
+    return IBH_OK;
+}
+
+#define ALLOC(ptr, n)                                  \
+    do {                                               \
+        int rc_ = dalloc(c, &(ptr), (size_t)(n));      \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+// --------------------------------------------------------------------------- profiling
+void prof_begin(ibh_ctx *c, hipEvent_t *e0) {
+    Prof &P = c->prof;
+    if (P.used + 2 > P.pool.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            P.pool.push_back(e);
+        }
+    }
+    *e0 = P.pool[P.used];
+    hipEventRecord(*e0, c->s);
+}
+
+void gemm(ibh_ctx *c, GemmArgs g, int epi) {
+    if (c->prof.on) {
+        hipEvent_t e0;
+        prof_begin(c, &e0);
+        launch_gemm_f64(g, epi, c->s);
+        hipEvent_t e1 = c->prof.pool[c->prof.used + 1];
+        hipEventRecord(e1, c->s);
+        c->prof.used += 2;
+        c->prof.rec.push_back({0, 2.0 * g.M * (double)g.N * g.K * g.batch});
+    } else {
+        launch_gemm_f64(g, epi, c->s);
+    }
+}
+
+GemmArgs gargs() {
+    GemmArgs g;
+    std::memset(&g, 0, sizeof g);
+    g.batch = 1;
+    return g;
+}
+
+// --------------------------------------------------------------------------- transforms
+// LaplacianInv apply (fluid-ops.jl:42-56) on `nbatch` fields: out = S((S·in·S)⊙scale)S where
+// the 16·s factor of two RODFT00 passes lives in the scale table.  The last contraction's
+// epilogue is `epi` (EPI_STORE, EPI_SUBFROM with D = out, or EPI_RHS with the extras in rhs).
+struct RhsExtras {
+    const double *N1, *N2, *bcacc;
+    double f1, f2;
+};
+
+void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, const double *scale,
+            long sscale, int nbatch, int epi, const RhsExtras *rx, const int *skip) {
+    const Dom &d = c->d;
+    const int m = d.m, p = d.p;
+    const long ng = d.ng;
+    GemmArgs g = gargs();
+    g.batch = nbatch;
+    g.skip = skip;
+    // T1 = S_m · in
+    g.M = m; g.N = p; g.K = m;
+    g.A = c->Sm; g.lda = m; g.sA = 0;
+    g.B = in; g.ldb = m; g.sB = sin;
+    g.C = c->T1; g.ldc = m; g.sC = ng;
+    gemm(c, g, EPI_STORE);
+    // T2 = (T1 · S_p) ⊙ scale
+    g.M = m; g.N = p; g.K = p;
+    g.A = c->T1; g.lda = m; g.sA = ng;
+    g.B = c->Sp; g.ldb = p; g.sB = 0;
+    g.C = c->T2; g.ldc = m; g.sC = ng;
+    g.D = scale; g.ldd = m; g.sD = sscale;
+    gemm(c, g, EPI_SCALE);
+    // T1 = S_m · T2
+    g.D = nullptr;
+    g.M = m; g.N = p; g.K = m;
+    g.A = c->Sm; g.lda = m; g.sA = 0;
+    g.B = c->T2; g.ldb = m; g.sB = ng;
+    g.C = c->T1; g.ldc = m; g.sC = ng;
+    gemm(c, g, EPI_STORE);
+    // out = T1 · S_p (+ epilogue)
+    g.M = m; g.N = p; g.K = p;
+    g.A = c->T1; g.lda = m; g.sA = ng;
+    g.B = c->Sp; g.ldb = p; g.sB = 0;
+    g.C = out; g.ldc = m; g.sC = sout;
+    if (epi == EPI_SUBFROM) {
+        g.D = out; g.ldd = m; g.sD = sout;
+    } else if (epi == EPI_RHS) {
+        g.N1 = rx->N1; g.N2 = rx->N2; g.bcacc = rx->bcacc;
+        g.f1 = rx->f1; g.f2 = rx->f2;
+        g.nbc = d.nbc; g.Lo = d.Lo; g.Ro = d.Ro; g.Bo = d.Bo; g.To = d.To;
+    }
+    gemm(c, g, epi);
+}
+
+// Vort2Flux (fluid-ops.jl:440-472) on nc independent multilevel fields.  G, Psi: per column
+// L levels of ng (column stride L·ng); Q: per column nq (level 0 only, curl_all = false) or
+// L·nq; bc: per column L·nbc.  Coarse centres of G are overwritten as in the reference.
+void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
+               const int *skip) {
+    const Dom &d = c->d;
+    const int L = c->L;
+    const long ng = d.ng, nq = d.nq;
+    const long sG = L * ng, sQ = curl_all ? L * nq : nq, sB = (long)L * d.nbc;
+    for (int lev = 1; lev < L; ++lev)
+        launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
+    // ψ_L = Δinv Γ_L with ψbc = 0
+    double *bcL = bc + (long)(L - 1) * d.nbc;
+    launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
+    for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
+    lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip);
+    if (curl_all || L == 1)
+        launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0), sQ,
+                       nc, skip, c->s);
+    for (int lev = L - 2; lev >= 0; --lev) {
+        double *bcl = bc + (long)lev * d.nbc;
+        launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
+        launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
+        lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip);
+        if (curl_all || lev == 0)
+            launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ, nc,
+                           skip, c->s);
+    }
+}
+
+// x = B·z for nc columns (B_Times, cnab/ops.jl:125-143): Γ = 0; Γ₁ = Ainv₁ C' Eᵀ z;
+// vort2flux; x = E q₁.  z, x: column stride sz, sx.
+void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip) {
+    const Dom &d = c->d;
+    const int L = c->L;
+    const long ng = d.ng;
+    launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->s);
+    launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
+    for (int col = 0; col < nc; ++col)
+        if (L > 1) launch_fill(c->gB + col * L * ng + ng, (L - 1) * ng, 0.0, c->s);
+    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip);
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip);
+    launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
+}
+
+int check_flags(ibh_ctx *c) {
+    int f = 0;
+    HIPCK(hipMemcpyAsync(&f, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCK(hipStreamSynchronize(c->s));
+    if (f) {
+        HIPCK(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
+        if (f & 1) return fail(c, IBH_EBOUNDS, "a body point's delta support leaves the finest grid");
+        if (f & 2) return fail(c, IBH_ENONFINITE, "infinite flux");
+        if (f & 4) return fail(c, IBH_ESINGULAR, "singular B matrix");
+    }
+    return IBH_OK;
+}
+
+int ensure_B_work(ibh_ctx *c, int nc) {
+    if (c->ncB >= nc) return IBH_OK;
+    const Dom &d = c->d;
+    const long L = c->L;
+    ALLOC(c->qB, (size_t)nc * d.nq);
+    ALLOC(c->g1B, (size_t)nc * d.ng);
+    ALLOC(c->gB, (size_t)nc * L * d.ng);
+    ALLOC(c->psiB, (size_t)nc * L * d.ng);
+    ALLOC(c->qLB, (size_t)nc * d.nq);
+    ALLOC(c->bcB, (size_t)nc * L * d.nbc);
+    ALLOC(c->zB, (size_t)nc * 2 * c->nb + 1);
+    if ((long)nc * d.ng > c->Tcap) {
+        ALLOC(c->T1, (size_t)nc * d.ng);
+        ALLOC(c->T2, (size_t)nc * d.ng);
+        c->Tcap = (long)nc * d.ng;
+    }
+    if ((long)nc * d.ng > (long)L * d.ng) ALLOC(c->gtmp, (size_t)nc * d.ng);
+    c->ncB = nc;
+    return IBH_OK;
+}
+
+// the dense Binv of cnab/ops.jl:79-97: Bmat[:, j] = B·e_j in column batches, then the inverse
+__global__ void k_unit_cols(double *Z, int nf, int j0, int nc) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)nf * nc) return;
+    const int i = (int)(t % nf), col = (int)(t / nf);
+    Z[t] = (i == j0 + col) ? 1.0 : 0.0;
+}
+
+int precompute_binv(ibh_ctx *c) {
+    const int nf = 2 * c->nb;
+    if (nf == 0) return IBH_OK;
+    // column batch sized to keep the B-product work under ~4 GB
+    const Dom &d = c->d;
+    const double per_col = 8.0 * (2.0 * d.nq + d.ng * (1.0 + 2.0 * c->L + 3.0));
+    int nc = (int)std::max(1.0, std::min(256.0, 4.0e9 / per_col));
+    nc = std::min(nc, nf);
+    int rc = ensure_B_work(c, nc);
+    if (rc) return rc;
+    double *Bmat = nullptr;
+    ALLOC(Bmat, (size_t)nf * nf);
+    if (!c->Binv) ALLOC(c->Binv, (size_t)nf * nf);
+    for (int j0 = 0; j0 < nf; j0 += nc) {
+        const int n = std::min(nc, nf - j0);
+        hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * n + 255) / 256)), dim3(256), 0,
+                           c->s, c->zB, nf, j0, n);
+        B_times(c, c->zB, nf, Bmat + (long)nf * j0, nf, n, nullptr);
+    }
+    // Binv = inv(Bmat): Gauss–Jordan on [Bmat | I]
+    hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
+                       c->s, c->Binv, nf, 0, nf);
+    int *ws = nullptr;
+    ALLOC(ws, (size_t)2 * (nf + 4) + 8);
+    for (int k = 0; k < nf; ++k) launch_gj_step(nf, Bmat, c->Binv, k, ws, c->derr, c->s);
+    HIPCK(hipStreamSynchronize(c->s));
+    return check_flags(c);
+}
+
+int upload_points(ibh_ctx *c, const double *r, const double *v) {
+    if (c->nb == 0) return IBH_OK;
+    if (r) HIPCK(hipMemcpyAsync(c->pts, r, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
+    if (v) HIPCK(hipMemcpyAsync(c->vel, v, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
+    return IBH_OK;
+}
+
+// the force solve of BoundaryForcer (timestepping.jl:106-139): Fb = Binv·Ftmp or cg!
+int force_solve(ibh_ctx *c) {
+    const int nf = 2 * c->nb;
+    if (nf == 0) return IBH_OK;
+    if (!c->use_cg) {
+        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->s);
+        return IBH_OK;
+    }
+    int rc = ensure_B_work(c, 1);
+    if (rc) return rc;
+    CGDev &cg = c->cg;
+    // r = b - B·x0 (warm start from the previous Fb)
+    B_times(c, cg.x, 0, cg.c, 0, 1, nullptr);
+    launch_cg_init(cg, c->s);
+    const int chunk = 16;
+    int done = 0, it = 0;
+    while (!done) {
+        for (int k = 0; k < chunk; ++k) {
+            launch_cg_dir(cg, c->s);
+            B_times(c, cg.u, 0, cg.c, 0, 1, cg.done);
+            launch_cg_update(cg, c->s);
+        }
+        HIPCK(hipMemcpyAsync(&done, cg.done, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        HIPCK(hipMemcpyAsync(&it, cg.it, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        HIPCK(hipStreamSynchronize(c->s));
+    }
+    c->cg_last = it;
+    return IBH_OK;
+}
+
+// advance!(state, solver, t), cnab/timestepping.jl:253-290
+int step(ibh_ctx *c, double t, double ux, double uy) {
+    const Dom &d = c->d;
+    const int L = c->L;
+    const long ng = d.ng;
+    c->ux = ux;
+    c->uy = uy;
+    // update!(reg) for moving bodies (timestepping.jl:262); points came from set_body_points
+    if (c->anymove) launch_reg_update(d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+
+    // ---- GetTrialState (timestepping.jl:28-79), all levels batched
+    if (L > 1) {
+        launch_get_bc(d, c->gam + ng, ng, c->bcraw, d.nbc, L - 1, nullptr, c->s);
+        launch_bc_prefix(d, c->bcraw, c->accbc, c->fac, L, c->s);
+    }
+    // bc of the coarsest level is zero; bcraw[L-1] stays zero
+    launch_nonlinear(d, c->q, d.nq, c->gam, ng, c->bcraw, d.nbc, 0.25, c->nl[0], ng, ux, uy, c->h,
+                     0, L, c->s);
+    RhsExtras rx{c->nl[0], c->nl[1], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
+    lapinv(c, c->gam, ng, c->gs, ng, c->sA, ng, L, EPI_RHS, &rx, nullptr);
+    lapinv(c, c->gs, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
+    std::swap(c->nl[0], c->nl[1]);  // cycle!(nonlin), timestepping.jl:81
+    vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr);
+
+    // ---- BoundaryForcer (timestepping.jl:106-139)
+    const int nf = 2 * c->nb;
+    if (nf) {
+        launch_E(d, c->reg, c->qs, d.nq, ux * c->h, uy * c->h, c->Ftmp, nf, 1, nullptr, c->s);
+        if (c->anymove) launch_axpy_sub(nf, c->Ftmp, c->vel, c->h, c->s);
+        int rc = force_solve(c);
+        if (rc) return rc;
+        // update_stress! (streamfcn/states.jl:53-84)
+        launch_stress(c->nbodies, c->d_boff, c->d_npts, c->nb, c->Fb, c->ds, c->h, c->dt, c->CD,
+                      c->CL, c->fbs, c->s);
+    }
+
+    // ---- CircProjecter (timestepping.jl:156-171): Γ = Γs; Γ₁ -= Ainv₁ C' Eᵀ Fb
+    std::swap(c->gam, c->gs);
+    if (nf) {
+        launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->s);
+        launch_rot(d, c->qtmp, d.nq, c->g1, ng, 1, nullptr, c->s);
+        lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr);
+    }
+    // ---- vort2flux!(qty) (fluid-ops.jl:438)
+    vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr);
+    c->t = t;
+    return IBH_OK;
+}
+
+long field_len(ibh_ctx *c, int which) {
+    const long L = c->L;
+    switch (which) {
+    case IBH_F_Q: case IBH_F_Q0: return L * c->d.nq;
+    case IBH_F_GAMMA: case IBH_F_PSI: case IBH_F_NONLIN1: case IBH_F_NONLIN2: return L * c->d.ng;
+    case IBH_F_FB: case IBH_F_FBSTRESS: case IBH_F_POINTS_R: case IBH_F_POINTS_V: return 2L * c->nb;
+    case IBH_F_CD: case IBH_F_CL: return c->nbodies;
+    }
+    return -1;
+}
+
+double *field_ptr(ibh_ctx *c, int which) {
+    switch (which) {
+    case IBH_F_Q: return c->q;
+    case IBH_F_GAMMA: return c->gam;
+    case IBH_F_PSI: return c->psi;
+    case IBH_F_NONLIN1: return c->nl[0];
+    case IBH_F_NONLIN2: return c->nl[1];
+    case IBH_F_FB: return c->Fb;
+    case IBH_F_FBSTRESS: return c->fbs;
+    case IBH_F_POINTS_R: return c->pts;
+    case IBH_F_POINTS_V: return c->vel;
+    case IBH_F_CD: return c->CD;
+    case IBH_F_CL: return c->CL;
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+int ibh_version(void) { return IBH_ABI_VERSION; }
+
+const char *ibh_last_error(const ibh_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
+    if (!out || !desc) return IBH_EINVAL;
+    *out = nullptr;
+    const ibh_case_desc &D = *desc;
+    if (D.nx % 4 || D.ny % 4 || D.nx < 16 || D.ny < 16 || D.nlev < 1 || D.nlev > 30 || D.h <= 0 ||
+        D.nbodies < 0)
+        return IBH_EINVAL;
+    if (D.precision != IBH_FP64) return IBH_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return IBH_EDEVICE;
+    ibh_ctx *c = new ibh_ctx();
+    c->d = make_dom(D.nx, D.ny);
+    c->L = D.nlev;
+    c->h = D.h; c->x0 = D.x0; c->y0 = D.y0; c->Re = D.Re; c->dt = D.dt;
+    c->beta[0] = D.beta[0]; c->beta[1] = D.beta[1];
+    c->nbodies = D.nbodies;
+    c->npts.resize(D.nbodies);
+    c->moving.resize(D.nbodies);
+    c->boff.resize(D.nbodies + 1);
+    int nb = 0;
+    for (int b = 0; b < D.nbodies; ++b) {
+        c->npts[b] = D.npts[b];
+        c->moving[b] = D.moving ? D.moving[b] : 0;
+        c->anymove |= c->moving[b] != 0;
+        c->boff[b] = nb;
+        nb += D.npts[b];
+    }
+    c->boff[D.nbodies] = nb;
+    c->nb = nb;
+    c->precision = D.precision;
+    c->force_solver = D.force_solver;
+    c->use_cg = D.force_solver == IBH_FORCE_CG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
+    c->cg_maxiter = D.cg_maxiter > 0 ? D.cg_maxiter : 5000;
+    c->cg_reltol = D.cg_reltol > 0 ? D.cg_reltol : 1e-12;
+    if (hipStreamCreate(&c->s) != hipSuccess) {
+        delete c;
+        return IBH_EDEVICE;
+    }
+    const Dom &d = c->d;
+    const long L = c->L, ng = d.ng, nq = d.nq;
+    int rc = 0;
+#define CK(x) do { if ((rc = (x)) != 0) { ibh_destroy(c); return rc; } } while (0)
+    CK(dalloc(c, &c->Sm, (size_t)d.m * d.m));
+    c->Sp = c->Sm;
+    if (d.p != d.m) CK(dalloc(c, &c->Sp, (size_t)d.p * d.p));
+    CK(dalloc(c, &c->sD, ng));
+    CK(dalloc(c, &c->sA, L * ng));
+    CK(dalloc(c, &c->sAinv, L * ng));
+    CK(dalloc(c, &c->gam, L * ng));
+    CK(dalloc(c, &c->psi, L * ng));
+    CK(dalloc(c, &c->q, L * nq));
+    CK(dalloc(c, &c->nl[0], L * ng));
+    CK(dalloc(c, &c->nl[1], L * ng));
+    CK(dalloc(c, &c->qs, nq));
+    CK(dalloc(c, &c->gs, L * ng));
+    CK(dalloc(c, &c->bcraw, L * d.nbc));
+    CK(dalloc(c, &c->accbc, L * d.nbc));
+    CK(dalloc(c, &c->psibc, L * d.nbc));
+    CK(dalloc(c, &c->gtmp, L * ng));
+    CK(dalloc(c, &c->fac, L + 1));
+    CK(dalloc(c, &c->qtmp, nq));
+    CK(dalloc(c, &c->g1, ng));
+    CK(dalloc(c, &c->T1, L * ng));
+    CK(dalloc(c, &c->T2, L * ng));
+    c->Tcap = L * ng;
+    const int nf = 2 * nb;
+    CK(dalloc(c, &c->pts, nf));
+    CK(dalloc(c, &c->vel, nf));
+    CK(dalloc(c, &c->ds, nb));
+    CK(dalloc(c, &c->d_boff, D.nbodies + 1));
+    CK(dalloc(c, &c->d_npts, D.nbodies + 1));
+    CK(dalloc(c, &c->derr, 4));
+    CK(dalloc(c, &c->reg.bidx, nf));
+    CK(dalloc(c, &c->reg.w, (size_t)nb * 4 * 13));
+    c->reg.nb = nb;
+    c->reg.err = c->derr;
+    CK(dalloc(c, &c->Fb, nf));
+    CK(dalloc(c, &c->Ftmp, nf));
+    CK(dalloc(c, &c->CD, D.nbodies + 1));
+    CK(dalloc(c, &c->CL, D.nbodies + 1));
+    CK(dalloc(c, &c->fbs, nf));
+    // CG state
+    CK(dalloc(c, &c->cg.r, nf));
+    CK(dalloc(c, &c->cg.c, nf));
+    CK(dalloc(c, &c->cg.u, nf));
+    CK(dalloc(c, &c->cg.sc, 4));
+    CK(dalloc(c, &c->cg.it, 2));
+    CK(dalloc(c, &c->cg.done, 2));
+    c->cg.x = c->Fb;
+    c->cg.b = c->Ftmp;
+    c->cg.n = nf;
+    c->cg.maxiter = c->cg_maxiter;
+    c->cg.reltol = c->cg_reltol;
+    // constants
+    launch_sine_matrix(c->Sm, D.nx, c->s);
+    if (c->Sp != c->Sm) launch_sine_matrix(c->Sp, D.ny, c->s);
+    launch_scale_tables(d, c->sD, c->sA, c->sAinv, c->L, c->h, c->dt, c->Re, c->s);
+    std::vector<double> fac(L + 1, 0.0);
+    for (int l = 0; l < L; ++l) {
+        const double hc = c->h * std::ldexp(1.0, l);
+        fac[l] = 0.25 * c->dt / (c->Re * (hc * hc));  // timestepping.jl:50
+    }
+    hipMemcpyAsync(c->fac, fac.data(), sizeof(double) * (L + 1), hipMemcpyHostToDevice, c->s);
+    if (nb) {
+        hipMemcpyAsync(c->pts, D.xb, sizeof(double) * nf, hipMemcpyHostToDevice, c->s);
+        hipMemcpyAsync(c->ds, D.ds, sizeof(double) * nb, hipMemcpyHostToDevice, c->s);
+    }
+    std::vector<int> bo(c->boff.begin(), c->boff.end()), np(c->npts);
+    np.push_back(0);
+    hipMemcpyAsync(c->d_boff, bo.data(), sizeof(int) * (D.nbodies + 1), hipMemcpyHostToDevice, c->s);
+    hipMemcpyAsync(c->d_npts, np.data(), sizeof(int) * (D.nbodies + 1), hipMemcpyHostToDevice, c->s);
+    if (hipStreamSynchronize(c->s) != hipSuccess) {
+        ibh_destroy(c);
+        return IBH_EDEVICE;
+    }
+#undef CK
+    *out = c;
+    return IBH_OK;
+}
+
+void ibh_destroy(ibh_ctx *c) {
+    if (!c) return;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (void *p : c->allocs) hipFree(p);
+    for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+}
+
+int ibh_init_state(ibh_ctx *c, double t0, double ux, double uy) {
+    if (!c) return IBH_EINVAL;
+    const long L = c->L;
+    HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * L * c->d.nq, c->s));
+    HIPCK(hipMemsetAsync(c->gam, 0, sizeof(double) * L * c->d.ng, c->s));
+    HIPCK(hipMemsetAsync(c->psi, 0, sizeof(double) * L * c->d.ng, c->s));
+    HIPCK(hipMemsetAsync(c->nl[0], 0, sizeof(double) * L * c->d.ng, c->s));
+    HIPCK(hipMemsetAsync(c->nl[1], 0, sizeof(double) * L * c->d.ng, c->s));
+    HIPCK(hipMemsetAsync(c->Fb, 0, sizeof(double) * 2 * c->nb + 8, c->s));
+    HIPCK(hipMemsetAsync(c->vel, 0, sizeof(double) * 2 * c->nb + 8, c->s));
+    c->ux = ux;
+    c->uy = uy;
+    c->t = t0;
+    HIPCK(hipStreamSynchronize(c->s));
+    return IBH_OK;
+}
+
+int ibh_setup(ibh_ctx *c) {
+    if (!c) return IBH_EINVAL;
+    // moving bodies under CG rebuild the regularization every step
+    if (!(c->use_cg && c->anymove)) {
+        launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+        int rc = check_flags(c);
+        if (rc) return rc;
+        if (!c->use_cg) {
+            rc = precompute_binv(c);
+            if (rc) return rc;
+        }
+    }
+    c->setup_done = true;
+    HIPCK(hipStreamSynchronize(c->s));
+    return IBH_OK;
+}
+
+int ibh_set_body_points(ibh_ctx *c, const double *r, const double *v) {
+    if (!c) return IBH_EINVAL;
+    return upload_points(c, r, v);
+}
+
+int ibh_advance(ibh_ctx *c, double t, double ux, double uy) {
+    if (!c) return IBH_EINVAL;
+    if (!c->setup_done) return fail(c, IBH_EINVAL, "ibh_setup must run before ibh_advance");
+    int rc = step(c, t, ux, uy);
+    if (rc) return rc;
+    if (c->anymove || c->use_cg) return check_flags(c);
+    return IBH_OK;
+}
+
+int ibh_advance_n(ibh_ctx *c, double t0, int32_t nsteps, double ux, double uy) {
+    if (!c) return IBH_EINVAL;
+    if (!c->setup_done) return fail(c, IBH_EINVAL, "ibh_setup must run before ibh_advance_n");
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = step(c, t0 + k * c->dt, ux, uy);
+        if (rc) return rc;
+    }
+    return IBH_OK;
+}
+
+int ibh_sync(ibh_ctx *c) {
+    if (!c) return IBH_EINVAL;
+    HIPCK(hipStreamSynchronize(c->s));
+    return check_flags(c);
+}
+
+int ibh_get_field(ibh_ctx *c, int32_t which, double *out) {
+    if (!c || !out) return IBH_EINVAL;
+    const long n = field_len(c, which);
+    if (n < 0) return fail(c, IBH_EINVAL, "unknown field");
+    HIPCK(hipStreamSynchronize(c->s));
+    if (which == IBH_F_Q0) {  // base_flux!, fluid-ops.jl:403-417 (never materialized)
+        const Dom &d = c->d;
+        for (int l = 0; l < c->L; ++l) {
+            const double hc = c->h * std::ldexp(1.0, l);
+            for (long k = 0; k < d.nq; ++k) out[l * d.nq + k] = k < d.nu ? c->ux * hc : c->uy * hc;
+        }
+        return IBH_OK;
+    }
+    if (n == 0) return IBH_OK;
+    HIPCK(hipMemcpy(out, field_ptr(c, which), sizeof(double) * n, hipMemcpyDeviceToHost));
+    return IBH_OK;
+}
+
+int ibh_set_field(ibh_ctx *c, int32_t which, const double *in) {
+    if (!c || !in) return IBH_EINVAL;
+    const long n = field_len(c, which);
+    if (n < 0 || which == IBH_F_Q0 || which == IBH_F_CD || which == IBH_F_CL)
+        return fail(c, IBH_EINVAL, "field cannot be set");
+    if (n == 0) return IBH_OK;
+    HIPCK(hipMemcpyAsync(field_ptr(c, which), in, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+    HIPCK(hipStreamSynchronize(c->s));
+    return IBH_OK;
+}
+
+int ibh_dims(ibh_ctx *c, int32_t *o) {
+    if (!c || !o) return IBH_EINVAL;
+    o[0] = c->d.nx; o[1] = c->d.ny; o[2] = c->L; o[3] = (int32_t)c->d.nq; o[4] = (int32_t)c->d.ng;
+    o[5] = c->nb; o[6] = c->d.nbc; o[7] = c->nbodies;
+    return IBH_OK;
+}
+
+double ibh_time(ibh_ctx *c) { return c ? c->t : 0.0; }
+
+int ibh_cg_iterations(ibh_ctx *c) { return c ? c->cg_last : -1; }
+
+// --------------------------------------------------------------------------- operators
+namespace {
+struct DevBuf {
+    ibh_ctx *c;
+    std::vector<void *> v;
+    double *get(size_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, (n ? n : 1) * sizeof(double)) != hipSuccess) return nullptr;
+        hipMemset(p, 0, (n ? n : 1) * sizeof(double));
+        v.push_back(p);
+        return (double *)p;
+    }
+    double *up(const double *h, size_t n) {
+        double *p = get(n);
+        if (p && h && n) hipMemcpy(p, h, n * sizeof(double), hipMemcpyHostToDevice);
+        return p;
+    }
+    ~DevBuf() {
+        hipDeviceSynchronize();
+        for (void *p : v) hipFree(p);
+    }
+};
+int down(ibh_ctx *c, double *h, const double *d, size_t n) {
+    HIPCK(hipStreamSynchronize(c->s));
+    HIPCK(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return IBH_OK;
+}
+}  // namespace
+
+int ibh_op_lapinv(ibh_ctx *c, int32_t mode, int32_t lev, const double *in, double *out) {
+    if (!c || lev < 0 || lev >= c->L || mode < 0 || mode > 2) return IBH_EINVAL;
+    DevBuf B{c};
+    const long ng = c->d.ng;
+    double *din = B.up(in, ng), *dout = B.get(ng);
+    const double *sc = mode == 0 ? c->sD : (mode == 1 ? c->sA : c->sAinv) + lev * ng;
+    lapinv(c, din, ng, dout, ng, sc, 0, 1, EPI_STORE, nullptr, nullptr);
+    return down(c, out, dout, ng);
+}
+
+int ibh_op_dst2(ibh_ctx *c, const double *in, double *out) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    const Dom &d = c->d;
+    double *din = B.up(in, d.ng), *dout = B.get(d.ng);
+    // RODFT00 in both dimensions = 4·S_m·X·S_p: two contractions, then ×4
+    GemmArgs g = gargs();
+    g.M = d.m; g.N = d.p; g.K = d.m;
+    g.A = c->Sm; g.lda = d.m; g.B = din; g.ldb = d.m; g.C = c->T1; g.ldc = d.m;
+    gemm(c, g, EPI_STORE);
+    double *four = B.get(d.ng);
+    launch_fill(four, d.ng, 4.0, c->s);
+    g.M = d.m; g.N = d.p; g.K = d.p;
+    g.A = c->T1; g.lda = d.m; g.B = c->Sp; g.ldb = d.p; g.C = dout; g.ldc = d.m;
+    g.D = four; g.ldd = d.m;
+    gemm(c, g, EPI_SCALE);
+    return down(c, out, dout, d.ng);
+}
+
+int ibh_op_rot(ibh_ctx *c, const double *q, double *gam) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dq = B.up(q, c->d.nq), *dg = B.get(c->d.ng);
+    launch_rot(c->d, dq, 0, dg, 0, 1, nullptr, c->s);
+    return down(c, gam, dg, c->d.ng);
+}
+
+int ibh_op_curl(ibh_ctx *c, const double *psi, const double *psibc, double *q) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dp = B.up(psi, c->d.ng), *db = B.up(psibc, c->d.nbc), *dq = B.get(c->d.nq);
+    launch_curl_bc(c->d, dp, 0, db, 0, dq, 0, 1, nullptr, c->s);
+    return down(c, q, dq, c->d.nq);
+}
+
+int ibh_op_coarsify(ibh_ctx *c, double *gc, const double *g) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dgc = B.up(gc, c->d.ng), *dg = B.up(g, c->d.ng);
+    launch_coarsify(c->d, dgc, dg, 0, 0, 1, nullptr, c->s);
+    return down(c, gc, dgc, c->d.ng);
+}
+
+int ibh_op_get_bc(ibh_ctx *c, const double *r, double *rbc) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dr = B.up(r, c->d.ng), *db = B.get(c->d.nbc);
+    launch_get_bc(c->d, dr, 0, db, 0, 1, nullptr, c->s);
+    return down(c, rbc, db, c->d.nbc);
+}
+
+int ibh_op_apply_bc(ibh_ctx *c, double *r, const double *rbc, double fac) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dr = B.up(r, c->d.ng), *db = B.up(rbc, c->d.nbc);
+    launch_apply_bc(c->d, dr, 0, db, 0, fac, 1, nullptr, c->s);
+    return down(c, r, dr, c->d.ng);
+}
+
+int ibh_op_nonlinear(ibh_ctx *c, int32_t lev, const double *q, const double *q0,
+                     const double *gam, const double *gbc, double *nonlin) {
+    if (!c || lev < 0 || lev >= c->L) return IBH_EINVAL;
+    // q0 must be the constant base flux of the level (fluid-ops.jl:403-417)
+    const Dom &d = c->d;
+    const double U = q0[0], V = q0[d.nu];
+    DevBuf B{c};
+    double *dq = B.up(q, d.nq), *dg = B.up(gam, d.ng), *db = B.up(gbc, d.nbc), *dn = B.get(d.ng);
+    launch_nonlinear(d, dq, 0, dg, 0, db, 0, 1.0, dn, 0, U, V, c->h, lev, 1, c->s, 1);
+    return down(c, nonlin, dn, d.ng);
+}
+
+int ibh_op_vort2flux(ibh_ctx *c, double *gam, double *psi, double *q) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    const Dom &d = c->d;
+    const long L = c->L;
+    double *dg = B.up(gam, L * d.ng), *dp = B.get(L * d.ng), *dq = B.get(L * d.nq),
+           *db = B.get(L * d.nbc);
+    vort2flux(c, dg, dp, dq, db, 1, true, nullptr);
+    int rc = down(c, gam, dg, L * d.ng);
+    if (!rc) rc = down(c, psi, dp, L * d.ng);
+    if (!rc) rc = down(c, q, dq, L * d.nq);
+    return rc;
+}
+
+int ibh_op_reg_update(ibh_ctx *c, const double *r) {
+    if (!c) return IBH_EINVAL;
+    int rc = upload_points(c, r, nullptr);
+    if (rc) return rc;
+    launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+    return check_flags(c);
+}
+
+int ibh_op_E(ibh_ctx *c, const double *q, double *fb) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *dq = B.up(q, c->d.nq), *df = B.get(2 * c->nb);
+    launch_E(c->d, c->reg, dq, 0, 0.0, 0.0, df, 0, 1, nullptr, c->s);
+    return down(c, fb, df, 2 * c->nb);
+}
+
+int ibh_op_ET(ibh_ctx *c, const double *fb, double *q) {
+    if (!c) return IBH_EINVAL;
+    DevBuf B{c};
+    double *df = B.up(fb, 2 * c->nb), *dq = B.get(c->d.nq);
+    launch_ET(c->d, c->reg, df, 0, dq, 0, 1, nullptr, c->s);
+    int rc = down(c, q, dq, c->d.nq);
+    if (rc) return rc;
+    return check_flags(c);
+}
+
+int ibh_op_B(ibh_ctx *c, const double *z, double *x) {
+    if (!c) return IBH_EINVAL;
+    int rc = ensure_B_work(c, 1);
+    if (rc) return rc;
+    DevBuf B{c};
+    const int nf = 2 * c->nb;
+    double *dz = B.up(z, nf), *dx = B.get(nf);
+    B_times(c, dz, 0, dx, 0, 1, nullptr);
+    return down(c, x, dx, nf);
+}
+
+int ibh_profile_enable(ibh_ctx *c, int32_t on) {
+    if (!c) return IBH_EINVAL;
+    HIPCK(hipStreamSynchronize(c->s));
+    c->prof.on = on != 0;
+    c->prof.used = 0;
+    c->prof.rec.clear();
+    return IBH_OK;
+}
+
+int ibh_profile_read(ibh_ctx *c, int32_t kind, double *ms, int64_t *launches, double *flops) {
+    if (!c) return IBH_EINVAL;
+    HIPCK(hipStreamSynchronize(c->s));
+    double tot = 0, fl = 0;
+    int64_t n = 0;
+    for (size_t k = 0; k < c->prof.rec.size(); ++k) {
+        if (c->prof.rec[k].first != kind) continue;
+        float e = 0;
+        HIPCK(hipEventElapsedTime(&e, c->prof.pool[2 * k], c->prof.pool[2 * k + 1]));
+        tot += e;
+        fl += c->prof.rec[k].second;
+        ++n;
+    }
+    if (ms) *ms = tot;
+    if (launches) *launches = n;
+    if (flops) *flops = fl;
+    return IBH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Internal declarations shared by the libibhip translation units (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ibh {
+
+// ---------------------------------------------------------------------------- grid geometry
+// Sizes of one level of the reference layout (src/streamfcn/fluids.jl:28-45).
+struct Dom {
+    int nx, ny;     // cells
+    int m, p;       // Γ is m×p = (nx-1)×(ny-1)
+    long nu, nq;    // qx block (nx+1)·ny, total flux nq
+    long ng;        // m·p
+    int nbc;        // 2(nx+1)+2(ny+1)
+    int Lo, Ro, Bo, To;  // bc offsets
+};
+
+inline Dom make_dom(int nx, int ny) {
+    Dom d;
+    d.nx = nx; d.ny = ny; d.m = nx - 1; d.p = ny - 1;
+    d.nu = (long)(nx + 1) * ny;
+    d.nq = d.nu + (long)nx * (ny + 1);
+    d.ng = (long)d.m * d.p;
+    d.nbc = 2 * (nx + 1) + 2 * (ny + 1);
+    d.Lo = 0; d.Ro = ny + 1; d.Bo = 2 * (ny + 1); d.To = 2 * (ny + 1) + nx + 1;
+    return d;
+}
+
+// ---------------------------------------------------------------------------- sine GEMM
+// C[b] = op(A[b]·B[b]) for column-major operands; batch b = blockIdx.z.
+enum Epi {
+    EPI_STORE = 0,   // C = acc
+    EPI_SCALE = 1,   // C = acc ⊙ D          (eigenvalue scaling between the two transforms)
+    EPI_SUBFROM = 2, // C = D - acc           (circulation projection Γ₁ = Γs₁ - Ainv·…)
+    EPI_RHS = 3      // C = acc + f1·N1 + f2·N2 + edges(bcacc)  (CN right-hand side)
+};
+
+struct GemmArgs {
+    int M, N, K;
+    const double *A; long lda, sA;
+    const double *B; long ldb, sB;
+    double *C; long ldc, sC;
+    const double *D; long ldd, sD;
+    // EPI_RHS extras: N1, N2 (ld = ldc, batch stride sC), edge accumulators (nbc per batch)
+    const double *N1, *N2, *bcacc;
+    double f1, f2;
+    int nbc, Lo, Ro, Bo, To;
+    const int *skip;  // if non-null and *skip != 0 the launch does nothing
+    int batch;
+};
+
+void launch_gemm_f64(const GemmArgs &g, int epi, hipStream_t s);
+
+// ---------------------------------------------------------------------------- fluid kernels
+// All take a batch count `nb` over levels (or columns) with element strides.
+void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
+                    double *q, long sq, int nbatch, const int *skip, hipStream_t s);
+void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
+                const int *skip, hipStream_t s);
+void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
+                     const int *skip, hipStream_t s);
+void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
+                   const int *skip, hipStream_t s);
+// r[b] += fac[b]·edges(rbc[b])   (fac per batch entry on the host side, passed by value array)
+void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long sbc, double fac,
+                     int nbatch, const int *skip, hipStream_t s);
+// out[b] = in[b] + edges(rbc[b])·fac
+void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out, long sout,
+                          const double *rbc, long sbc, double fac, int nbatch, const int *skip,
+                          hipStream_t s);
+// nonlinear term for levels lev0..lev0+nbatch-1 (level stride sq / sg); q0 is the constant
+// (ux·hc, uy·hc) per level (or (ux, uy) as given when q0_given); gbc per level with stride
+// sbc, multiplied by gbc_scale.
+void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
+                      const double *gbc, long sbc, double gbc_scale, double *out, long sout,
+                      double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
+                      int q0_given = 0);
+// accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac_l·bcraw[l]  for ℓ = 0..L-2, accbc[L-1] = 0
+void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
+                      hipStream_t s);
+void launch_fill(double *p, long n, double v, hipStream_t s);
+void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+1)(k+1)/n), (n-1)²
+void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
+                         double dt, double Re, hipStream_t s);
+
+// ---------------------------------------------------------------------------- body kernels
+struct RegDev {
+    int nb;
+    int *bidx;     // 2·nb  (x index block, then y)
+    double *w;     // nb × 4 × 13 : (wx_u, wy_u, wx_v, wy_v) 1-D delta factors
+    int *err;      // device error flag (1: out of bounds, 2: non-finite flux)
+};
+void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double h, double x0,
+                       double y0, hipStream_t s);
+// fb = E·(q + q0) where q0 = (u0, v0) constant (pass 0, 0 for plain E)
+void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
+              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s);
+// q = Eᵀ fb (q zeroed by the kernel pair)
+void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
+               int nbatch, const int *skip, hipStream_t s);
+void launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
+void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s);  // y -= a v
+void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
+                   const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
+                   hipStream_t s);
+
+// CG state lives on the device; see body.hip
+struct CGDev {
+    double *x, *b, *r, *c, *u;  // 2nb each
+    double *sc;                 // [residual, prev_residual, tol]
+    int *it, *done;
+    int n, maxiter;
+    double reltol;
+};
+void launch_cg_init(const CGDev &cg, hipStream_t s);        // after c = B·x: r = b - c, ...
+void launch_cg_dir(const CGDev &cg, hipStream_t s);         // u = r + β u
+void launch_cg_update(const CGDev &cg, hipStream_t s);      // after c = B·u
+
+// dense inverse (Gauss–Jordan with partial pivoting) of an n×n column-major matrix
+void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err,
+                    hipStream_t s);
+
+}  // namespace ibh

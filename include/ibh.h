@@ -1,0 +1,167 @@
+/*
+ * ibh.h — C ABI of libibhip.so: the CNAB immersed-boundary projection step of
+ * ImmersedBoundary.jl on one MI355X (gfx950), hand-written HIP.
+ *
+ * Plain C types only.  Arrays cross the boundary as host pointers in the reference's layout
+ * (Float64, column-major, x fastest):
+ *   Γ, ψ      (nx-1)(ny-1) per level, levels stacked:   [lev1 | lev2 | ...]
+ *   q, q0     nq = (nx+1)ny + nx(ny+1) per level, [qx | qy] per level
+ *   Fb        [fx_1..fx_nb, fy_1..fy_nb]
+ *   points    nb×2 column-major: [x_1..x_nb, y_1..y_nb]
+ * The library copies host data synchronously and never keeps host pointers.  A context owns
+ * its device memory and one HIP stream; it is not thread-safe (one host thread per context).
+ * Every function returns IBH_OK (0) or an ibh_status code; ibh_last_error() has the message.
+ *
+ * Reference citations are relative to the ImmersedBoundary.jl repository root.
+ */
+#ifndef IBH_H
+#define IBH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBH_ABI_VERSION 1
+
+enum ibh_status {
+    IBH_OK = 0,
+    IBH_EINVAL = 1,     /* bad sizes / arguments (e.g. nx % 4 != 0)                        */
+    IBH_EBOUNDS = 2,    /* a body point's 13×13 delta support leaves the finest grid       */
+    IBH_ENONFINITE = 3, /* non-finite spread flux ("infinite flux", structure-ops.jl:80-82) */
+    IBH_ENOCONV = 4,    /* CG reached cg_maxiter without meeting the tolerance              */
+    IBH_EDEVICE = 5,    /* HIP runtime error or missing GPU                                 */
+    IBH_ESINGULAR = 6   /* zero pivot in the dense Schur-complement inverse                 */
+};
+
+enum ibh_precision {
+    IBH_FP64 = 0,   /* every sine transform in FP64 (the drop-in path)                     */
+    IBH_FP32 = 1,   /* bulk sine transforms FP32 on MFMA, forces/CG/residuals FP64          */
+    IBH_BF16X3 = 2  /* bulk sine transforms as 3 bf16 MFMA products, FP32 accumulation     */
+};
+
+enum ibh_force_solver {
+    IBH_FORCE_AUTO = 0,  /* dense precomputed inverse when every body is fixed, else CG    */
+    IBH_FORCE_DENSE = 1, /* Binv_linearmap dense branch, cnab/ops.jl:79-102                  */
+    IBH_FORCE_CG = 2     /* Binv_linearmap CG branch, cnab/ops.jl:104-114                    */
+};
+
+/* Field selectors for ibh_get_field / ibh_set_field (StreamFcnQuantities, streamfcn/states.jl:9-17;
+ * CNAB.State.nonlin, cnab/states.jl:1-6). */
+enum ibh_field {
+    IBH_F_Q = 0,        /* q   nq×L                                        */
+    IBH_F_Q0 = 1,       /* q0  nq×L (base flux, fluid-ops.jl:403-417)      */
+    IBH_F_GAMMA = 2,    /* Γ   nΓ×L                                        */
+    IBH_F_PSI = 3,      /* ψ   nΓ×L                                        */
+    IBH_F_FB = 4,       /* Fb  2nb                                         */
+    IBH_F_NONLIN1 = 5,  /* nonlin[1] nΓ×L                                  */
+    IBH_F_NONLIN2 = 6,  /* nonlin[2] nΓ×L                                  */
+    IBH_F_CD = 7,       /* per-body drag coefficient (nbodies)             */
+    IBH_F_CL = 8,       /* per-body lift coefficient (nbodies)             */
+    IBH_F_FBSTRESS = 9, /* BodyStress.fb stacked nb×2                       */
+    IBH_F_POINTS_R = 10,/* body point positions nb×2                       */
+    IBH_F_POINTS_V = 11 /* body point velocities nb×2                      */
+};
+
+/* Problem description: Problem(fluid, bodies, scheme), src/ImmersedBoundary.jl:34-38, with
+ * UniformGrid/MultiLevel (src/fluids/fluids.jl:53-102), StreamFcnFluid Re
+ * (src/streamfcn/fluids.jl:49-58), RigidBody xb/ds (src/bodies/rigid-bodies.jl:4-8) and
+ * SchemeCNAB dt/β (src/schemes.jl:19-26). */
+typedef struct ibh_case_desc {
+    int32_t nx, ny, nlev;   /* finest cells (multiples of 4, ≥ 16) and level count        */
+    double h, x0, y0;       /* finest spacing and lower-left corner of the finest grid    */
+    double Re, dt;
+    double beta[2];         /* Adams–Bashforth weights, (1.5, -0.5)                       */
+    int32_t nbodies;
+    const int32_t *npts;    /* points per body                                            */
+    const int32_t *moving;  /* per body: 0 fixed in the grid frame, 1 moving (may be NULL)*/
+    const double *xb;       /* nb×2 initial points, column-major                          */
+    const double *ds;       /* nb segment lengths                                         */
+    int32_t precision;      /* enum ibh_precision                                         */
+    int32_t force_solver;   /* enum ibh_force_solver                                      */
+    int32_t cg_maxiter;     /* ≤0 → 5000 (cnab/ops.jl:110)                                */
+    double cg_reltol;       /* ≤0 → 1e-12 (cnab/ops.jl:110)                               */
+} ibh_case_desc;
+
+typedef struct ibh_ctx ibh_ctx;
+
+/* ---- lifecycle ----------------------------------------------------------------------- */
+int ibh_version(void);
+/* Allocates device state for one case (replaces the array allocations of
+ * StreamFcnQuantities, streamfcn/states.jl:19-41, and Solver, cnab/timestepping.jl:193-249). */
+int ibh_create(const ibh_case_desc *desc, ibh_ctx **out);
+void ibh_destroy(ibh_ctx *ctx);
+const char *ibh_last_error(const ibh_ctx *ctx);
+/* initstate(prob, t0): zero q, Γ, ψ, Fb, nonlin; base flux at t0
+ * (src/ImmersedBoundary.jl:51 → cnab/states.jl:8-16, streamfcn/states.jl:19-41). */
+int ibh_init_state(ibh_ctx *ctx, double t0, double ux, double uy);
+/* initsolver(prob, state): Reg(domain, points) (structure-ops.jl:24-35) and, for the dense
+ * force solver, Bmat by unit-vector B-products then inv(Bmat) (cnab/ops.jl:79-97), on the GPU
+ * (src/ImmersedBoundary.jl:52 → cnab/timestepping.jl:193-249). */
+int ibh_setup(ibh_ctx *ctx);
+/* update_bodies! output for moving bodies (src/bodies/body-points.jl:47-76): stacked points
+ * and velocities at the coming step's t.  NULL keeps the current array. */
+int ibh_set_body_points(ibh_ctx *ctx, const double *r, const double *v);
+/* advance!(state, solver, t) (cnab/timestepping.jl:253-290), with the freestream (ux, uy)
+ * = fluid.freestream(t) evaluated by the caller (fluid-ops.jl:403-417).  Enqueues the step
+ * on the context stream and returns; getters synchronize. */
+int ibh_advance(ibh_ctx *ctx, double t, double ux, double uy);
+/* advance! for nsteps steps at t0 + k·dt (k = 0..nsteps-1) with a constant freestream and
+ * fixed bodies: the inner loop of solve! (src/solve.jl:198-212) without callbacks. */
+int ibh_advance_n(ibh_ctx *ctx, double t0, int32_t nsteps, double ux, double uy);
+int ibh_sync(ibh_ctx *ctx);
+
+/* ---- quantities / checkpoint ------------------------------------------------------------ */
+/* Copy a field to / from host memory (copy!/similar of streamfcn/states.jl:90-112 and
+ * cnab/states.jl:23-27).  Sizes follow enum ibh_field. */
+int ibh_get_field(ibh_ctx *ctx, int32_t which, double *host_out);
+int ibh_set_field(ibh_ctx *ctx, int32_t which, const double *host_in);
+/* out8 = {nx, ny, nlev, nq, nΓ, nb, nbc, nbodies} */
+int ibh_dims(ibh_ctx *ctx, int32_t *out8);
+double ibh_time(ibh_ctx *ctx);
+/* iterations of the last CG force solve (IterativeSolvers cg!, cnab/ops.jl:110) */
+int ibh_cg_iterations(ibh_ctx *ctx);
+
+/* ---- operator-level entry points (host arrays in/out; each runs the device kernels) ---- */
+/* LaplacianInv apply (fluid-ops.jl:42-56): mode 0 Δinv (Λ), 1 A_ℓ (Λexpl), 2 Ainv_ℓ (Λimpl),
+ * cnab/ops.jl:14-33; lev is 0-based.  in/out: nΓ. */
+int ibh_op_lapinv(ibh_ctx *ctx, int32_t mode, int32_t lev, const double *in, double *out);
+/* FFTW RODFT00 along both axes of the (nx-1)×(ny-1) array (fluid-ops.jl:25,49,51). */
+int ibh_op_dst2(ibh_ctx *ctx, const double *in, double *out);
+/* rot! (C'), fluid-ops.jl:117-129: q (nq) → Γ (nΓ). */
+int ibh_op_rot(ibh_ctx *ctx, const double *q, double *gam);
+/* curl!(q, ψ, ψbc), fluid-ops.jl:138-182: ψ (nΓ), ψbc (nbc) → q (nq). */
+int ibh_op_curl(ibh_ctx *ctx, const double *psi, const double *psibc, double *q);
+/* coarsify!(Γc, Γ), fluid-ops.jl:184-202: writes the centre of gc (in/out, nΓ). */
+int ibh_op_coarsify(ibh_ctx *ctx, double *gc, const double *g);
+/* get_bc!(rbc, r), fluid-ops.jl:210-239: r (nΓ) → rbc (nbc). */
+int ibh_op_get_bc(ibh_ctx *ctx, const double *r, double *rbc);
+/* apply_bc!(r, rbc, fac), fluid-ops.jl:249-275: r (nΓ, in/out). */
+int ibh_op_apply_bc(ibh_ctx *ctx, double *r, const double *rbc, double fac);
+/* Nonlinear (cnab/ops.jl:151-168; avg_flux!/direct_product!/rot!, fluid-ops.jl:277-401,474-492)
+ * at level lev (0-based): q, q0 (nq), Γ (nΓ), Γbc (nbc) → nonlin (nΓ). */
+int ibh_op_nonlinear(ibh_ctx *ctx, int32_t lev, const double *q, const double *q0,
+                     const double *gam, const double *gbc, double *nonlin);
+/* Vort2Flux (fluid-ops.jl:440-472): Γ (nΓ×L, in/out: coarse centres are overwritten) →
+ * ψ (nΓ×L), q (nq×L). */
+int ibh_op_vort2flux(ibh_ctx *ctx, double *gam, double *psi, double *q);
+/* Reg update! (structure-ops.jl:37-61) from stacked points r (nb×2). */
+int ibh_op_reg_update(ibh_ctx *ctx, const double *r);
+/* E (RegT, structure-ops.jl:88-105): q (nq, level 1) → fb (2nb). */
+int ibh_op_E(ibh_ctx *ctx, const double *q, double *fb);
+/* Eᵀ (Reg, structure-ops.jl:63-86): fb (2nb) → q (nq). */
+int ibh_op_ET(ibh_ctx *ctx, const double *fb, double *q);
+/* B·z (B_Times, cnab/ops.jl:125-143): z (2nb) → x (2nb). */
+int ibh_op_B(ibh_ctx *ctx, const double *z, double *x);
+
+/* ---- profiling (HIP events around each launch of the sine-transform kernels) ----------- */
+int ibh_profile_enable(ibh_ctx *ctx, int32_t on);
+/* kind 0: FP64 transform GEMMs, 1: FP32, 2: bf16x3.  Returns the summed kernel time (ms),
+ * launch count and algorithmic flops since the last enable. */
+int ibh_profile_read(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches, double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBH_H */
