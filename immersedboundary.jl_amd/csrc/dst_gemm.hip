@@ -90,7 +90,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
         __syncthreads();
     }
 
-    // epilogue: acc[ni][mi][r] holds C[m][n] with m = l16 (+16 mi), n = 4·lq + r (+16 ni)
+    // epilogue.  v_mfma_f64_16x16x4f64 leaves D[row][col] in lane col + 16·(row % 4),
+    // register row / 4 (row = lane/16 + 4·r); here row ↔ n and col ↔ m, so acc[ni][mi][r]
+    // holds C[m][n] with m = l16 (+16 mi), n = lq + 4·r (+16 ni).
     const double *D = g.D ? g.D + bz * g.sD : nullptr;
 #pragma unroll
     for (int ni = 0; ni < 2; ++ni)
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm * 32 + mi * 16 + l16;
-                const int n = n0 + wn * 32 + ni * 16 + lq * 4 + r;
+                const int n = n0 + wn * 32 + ni * 16 + lq + 4 * r;
                 if (m < M && n < N) {
                     double v = acc[ni][mi][r];
                     const long ci = m + ldc * (long)n;

@@ -563,8 +563,10 @@ int ibh_init_state(ibh_ctx *c, double t0, double ux, double uy) {
     HIPCK(hipMemsetAsync(c->psi, 0, sizeof(double) * L * c->d.ng, c->s));
     HIPCK(hipMemsetAsync(c->nl[0], 0, sizeof(double) * L * c->d.ng, c->s));
     HIPCK(hipMemsetAsync(c->nl[1], 0, sizeof(double) * L * c->d.ng, c->s));
-    HIPCK(hipMemsetAsync(c->Fb, 0, sizeof(double) * 2 * c->nb + 8, c->s));
-    HIPCK(hipMemsetAsync(c->vel, 0, sizeof(double) * 2 * c->nb + 8, c->s));
+    if (c->nb) {
+        HIPCK(hipMemsetAsync(c->Fb, 0, sizeof(double) * 2 * c->nb, c->s));
+        HIPCK(hipMemsetAsync(c->vel, 0, sizeof(double) * 2 * c->nb, c->s));
+    }
     c->ux = ux;
     c->uy = uy;
     c->t = t0;
