@@ -228,6 +228,36 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 // iteration u = r + (‖r‖²/‖r_prev‖²) u, c = A u, α = ‖r‖²/(u·c), x += α u, r -= α c.  The stop
 // test (it ≥ maxiter or ‖r‖ ≤ tol) runs before every iteration.  One block does the vector
 // work; the device flag `done` makes the surrounding B-products skip once converged.
+// z = P r for PCG (dense n×n preconditioner), skipped once converged
+__global__ void k_pc_apply(CGDev cg) {
+    if (*cg.done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cg.n) return;
+    const int n = cg.n;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += cg.P[i + (long)n * j] * cg.r[j];
+    cg.z[i] = s;
+}
+
+// PCG direction (IterativeSolvers 0.9.2 PCGIterable): ρ_prev = ρ; ρ = z·r; u = z + (ρ/ρ_prev) u
+__global__ void k_pcg_dir(CGDev cg) {
+    __shared__ double sh[17];
+    if (*cg.done) return;
+    const int n = cg.n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.z[i] * cg.r[i];
+    s = block_sum<1024>(s, sh);
+    const double beta = s / cg.sc[3];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) cg.u[i] = cg.z[i] + beta * cg.u[i];
+    __syncthreads();
+    if (threadIdx.x == 0) cg.sc[3] = s;
+}
+
+void launch_pcg_dir(const CGDev &cg, hipStream_t s) {
+    hipLaunchKernelGGL(k_pc_apply, dim3(nblk(cg.n, 64)), dim3(64), 0, s, cg);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(1), dim3(1024), 0, s, cg);
+}
+
 __global__ void k_cg_init(CGDev cg) {
     __shared__ double sh[17];
     const int n = cg.n;
@@ -244,6 +274,7 @@ __global__ void k_cg_init(CGDev cg) {
         cg.sc[0] = res;
         cg.sc[1] = 1.0;
         cg.sc[2] = cg.reltol * res;
+        cg.sc[3] = 1.0;
         *cg.it = 0;
         *cg.done = (cg.maxiter <= 0 || res <= cg.reltol * res) ? 1 : 0;
     }
@@ -265,7 +296,7 @@ __global__ void k_cg_update(CGDev cg) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.u[i] * cg.c[i];
     s = block_sum<1024>(s, sh);
     const double res = cg.sc[0];
-    const double alpha = (res * res) / s;
+    const double alpha = (cg.P ? cg.sc[3] : res * res) / s;
     double s2 = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         cg.x[i] += alpha * cg.u[i];
@@ -292,6 +323,34 @@ void launch_cg_dir(const CGDev &cg, hipStream_t s) {
 }
 void launch_cg_update(const CGDev &cg, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, s, cg);
+}
+
+// ---------------------------------------------------------------- preconditioner assembly
+// Bm[j, c] = Bm[c, j] for rows j in J (fresh columns) and columns c outside J: the rows of
+// the moving unknowns are taken from the fresh columns by the near-symmetry of B.
+__global__ void k_mirror_rows(int n, double *Bm, const int *J, int nJ, const int *inJ) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)n * nJ) return;
+    const int c = (int)(t % n), j = J[t / n];
+    if (inJ[c]) return;
+    Bm[j + (long)n * c] = Bm[c + (long)n * j];
+}
+
+void launch_mirror_rows(int n, double *Bm, const int *cols, int ncols, const int *is_col,
+                        hipStream_t s) {
+    if (ncols == 0) return;
+    hipLaunchKernelGGL(k_mirror_rows, dim3(nblk((long)n * ncols, 256)), dim3(256), 0, s, n, Bm,
+                       cols, ncols, is_col);
+}
+
+__global__ void k_copy(double *dst, const double *src, long n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+void launch_copy(double *dst, const double *src, long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n, 256)), dim3(256), 0, s, dst, src, n);
 }
 
 // ---------------------------------------------------------------- dense inverse

@@ -62,6 +62,13 @@ struct ibh_ctx {
     double *Binv = nullptr;
     CGDev cg{};
     int cg_last = 0;
+    // PCG preconditioner: dense inverse of the assembled B (moving columns refreshed)
+    bool use_pcg = false, pc_valid = false;
+    int pc_refresh = 50, steps_since_pc = 0;
+    double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
+    int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
+    std::vector<int> J;                      // unknowns of moving bodies
+    std::vector<std::pair<int, int>> Jrange; // contiguous [start, count) runs of J
     bool setup_done = false;
     Prof prof;
     std::string err;
@@ -274,33 +281,83 @@ __global__ void k_unit_cols(double *Z, int nf, int j0, int nc) {
     Z[t] = (i == j0 + col) ? 1.0 : 0.0;
 }
 
+// Bm[:, j0:j0+cnt] = B·e_j (unit-vector products in column batches, cnab/ops.jl:88-94)
+int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt) {
+    const int nf = 2 * c->nb;
+    // column batch sized to keep the B-product work under ~12 GB
+    const Dom &d = c->d;
+    const double per_col = 8.0 * (2.0 * d.nq + d.ng * (1.0 + 2.0 * c->L + 3.0));
+    int nc = (int)std::max(1.0, std::min(512.0, 12.0e9 / per_col));
+    nc = std::min(nc, cnt);
+    int rc = ensure_B_work(c, nc);
+    if (rc) return rc;
+    for (int j = j0; j < j0 + cnt; j += nc) {
+        const int n = std::min(nc, j0 + cnt - j);
+        hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * n + 255) / 256)), dim3(256), 0,
+                           c->s, c->zB, nf, j, n);
+        B_times(c, c->zB, nf, Bm + (long)nf * j, nf, n, nullptr);
+    }
+    return IBH_OK;
+}
+
+// Inv = inverse(A) by Gauss–Jordan on [A | I]; A is destroyed
+int invert(ibh_ctx *c, double *A, double *Inv) {
+    const int nf = 2 * c->nb;
+    hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
+                       c->s, Inv, nf, 0, nf);
+    if (!c->gjws) ALLOC(c->gjws, (size_t)2 * (nf + 4) + 8);
+    for (int k = 0; k < nf; ++k) launch_gj_step(nf, A, Inv, k, c->gjws, c->derr, c->s);
+    HIPCK(hipStreamSynchronize(c->s));
+    return check_flags(c);
+}
+
 int precompute_binv(ibh_ctx *c) {
     const int nf = 2 * c->nb;
     if (nf == 0) return IBH_OK;
-    // column batch sized to keep the B-product work under ~4 GB
-    const Dom &d = c->d;
-    const double per_col = 8.0 * (2.0 * d.nq + d.ng * (1.0 + 2.0 * c->L + 3.0));
-    int nc = (int)std::max(1.0, std::min(256.0, 4.0e9 / per_col));
-    nc = std::min(nc, nf);
-    int rc = ensure_B_work(c, nc);
-    if (rc) return rc;
     double *Bmat = nullptr;
     ALLOC(Bmat, (size_t)nf * nf);
     if (!c->Binv) ALLOC(c->Binv, (size_t)nf * nf);
-    for (int j0 = 0; j0 < nf; j0 += nc) {
-        const int n = std::min(nc, nf - j0);
-        hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * n + 255) / 256)), dim3(256), 0,
-                           c->s, c->zB, nf, j0, n);
-        B_times(c, c->zB, nf, Bmat + (long)nf * j0, nf, n, nullptr);
+    int rc = B_columns(c, Bmat, 0, nf);
+    if (rc) return rc;
+    return invert(c, Bmat, c->Binv);  // Binv = inv(Bmat), cnab/ops.jl:96
+}
+
+// PCG preconditioner: the first build assembles every column of B; later builds refresh the
+// moving bodies' columns and take their rows from the fresh columns (B is symmetric up to
+// the multilevel boundary coupling), then invert.
+int refresh_pc(ibh_ctx *c) {
+    const int nf = 2 * c->nb;
+    int rc;
+    if (!c->Bfull) {
+        ALLOC(c->Bfull, (size_t)nf * nf);
+        ALLOC(c->Bwork, (size_t)nf * nf);
+        ALLOC(c->Pc, (size_t)nf * nf);
+        std::vector<int> inJ(nf, 0);
+        for (int j : c->J) inJ[j] = 1;
+        ALLOC(c->dJ, c->J.size() + 1);
+        ALLOC(c->dinJ, nf);
+        if (!c->J.empty())
+            HIPCK(hipMemcpyAsync(c->dJ, c->J.data(), sizeof(int) * c->J.size(),
+                                 hipMemcpyHostToDevice, c->s));
+        HIPCK(hipMemcpyAsync(c->dinJ, inJ.data(), sizeof(int) * nf, hipMemcpyHostToDevice, c->s));
+        HIPCK(hipStreamSynchronize(c->s));
     }
-    // Binv = inv(Bmat): Gauss–Jordan on [Bmat | I]
-    hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
-                       c->s, c->Binv, nf, 0, nf);
-    int *ws = nullptr;
-    ALLOC(ws, (size_t)2 * (nf + 4) + 8);
-    for (int k = 0; k < nf; ++k) launch_gj_step(nf, Bmat, c->Binv, k, ws, c->derr, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
-    return check_flags(c);
+    if (!c->pc_valid) {
+        rc = B_columns(c, c->Bfull, 0, nf);
+        if (rc) return rc;
+    } else {
+        for (auto &r : c->Jrange) {
+            rc = B_columns(c, c->Bfull, r.first, r.second);
+            if (rc) return rc;
+        }
+        launch_mirror_rows(nf, c->Bfull, c->dJ, (int)c->J.size(), c->dinJ, c->s);
+    }
+    launch_copy(c->Bwork, c->Bfull, (long)nf * nf, c->s);
+    rc = invert(c, c->Bwork, c->Pc);
+    if (rc) return rc;
+    c->pc_valid = true;
+    c->steps_since_pc = 0;
+    return IBH_OK;
 }
 
 int upload_points(ibh_ctx *c, const double *r, const double *v) {
@@ -318,17 +375,24 @@ int force_solve(ibh_ctx *c) {
         launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->s);
         return IBH_OK;
     }
-    int rc = ensure_B_work(c, 1);
+    int rc;
+    if (c->use_pcg && (!c->pc_valid || c->steps_since_pc >= c->pc_refresh)) {
+        rc = refresh_pc(c);
+        if (rc) return rc;
+    }
+    rc = ensure_B_work(c, 1);
     if (rc) return rc;
     CGDev &cg = c->cg;
+    cg.P = c->use_pcg ? c->Pc : nullptr;
     // r = b - B·x0 (warm start from the previous Fb)
     B_times(c, cg.x, 0, cg.c, 0, 1, nullptr);
     launch_cg_init(cg, c->s);
-    const int chunk = 16;
+    const int chunk = c->use_pcg ? 4 : 16;
     int done = 0, it = 0;
     while (!done) {
         for (int k = 0; k < chunk; ++k) {
-            launch_cg_dir(cg, c->s);
+            if (c->use_pcg) launch_pcg_dir(cg, c->s);
+            else launch_cg_dir(cg, c->s);
             B_times(c, cg.u, 0, cg.c, 0, 1, cg.done);
             launch_cg_update(cg, c->s);
         }
@@ -337,6 +401,11 @@ int force_solve(ibh_ctx *c) {
         HIPCK(hipStreamSynchronize(c->s));
     }
     c->cg_last = it;
+    if (c->use_pcg) {
+        ++c->steps_since_pc;
+        // a preconditioner that has drifted far from B is rebuilt at the next step
+        if (it > 24) c->steps_since_pc = c->pc_refresh;
+    }
     return IBH_OK;
 }
 
@@ -457,7 +526,19 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     c->nb = nb;
     c->precision = D.precision;
     c->force_solver = D.force_solver;
-    c->use_cg = D.force_solver == IBH_FORCE_CG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
+    c->use_pcg = D.force_solver == IBH_FORCE_PCG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
+    c->use_cg = D.force_solver == IBH_FORCE_CG || c->use_pcg;
+    c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : 50;
+    {
+        // unknowns of moving bodies, as contiguous runs in [fx | fy]
+        for (int comp = 0; comp < 2; ++comp)
+            for (int b = 0; b < D.nbodies; ++b)
+                if (D.moving && D.moving[b] && D.npts[b] > 0) {
+                    const int j0 = comp * nb + c->boff[b];
+                    c->Jrange.push_back({j0, D.npts[b]});
+                    for (int k = 0; k < D.npts[b]; ++k) c->J.push_back(j0 + k);
+                }
+    }
     c->cg_maxiter = D.cg_maxiter > 0 ? D.cg_maxiter : 5000;
     c->cg_reltol = D.cg_reltol > 0 ? D.cg_reltol : 1e-12;
     if (hipStreamCreate(&c->s) != hipSuccess) {
@@ -511,6 +592,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.r, nf));
     CK(dalloc(c, &c->cg.c, nf));
     CK(dalloc(c, &c->cg.u, nf));
+    CK(dalloc(c, &c->cg.z, nf));
     CK(dalloc(c, &c->cg.sc, 4));
     CK(dalloc(c, &c->cg.it, 2));
     CK(dalloc(c, &c->cg.done, 2));

@@ -108,14 +108,21 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 // CG state lives on the device; see body.hip
 struct CGDev {
     double *x, *b, *r, *c, *u;  // 2nb each
-    double *sc;                 // [residual, prev_residual, tol]
+    double *z;                  // preconditioned residual (PCG)
+    const double *P;            // dense preconditioner n×n (PCG) or null
+    double *sc;                 // [residual, prev_residual, tol, ρ]
     int *it, *done;
     int n, maxiter;
     double reltol;
 };
 void launch_cg_init(const CGDev &cg, hipStream_t s);        // after c = B·x: r = b - c, ...
-void launch_cg_dir(const CGDev &cg, hipStream_t s);         // u = r + β u
+void launch_cg_dir(const CGDev &cg, hipStream_t s);         // u = r + β u   (CG)
+void launch_pcg_dir(const CGDev &cg, hipStream_t s);        // z = P r; u = z + β u (PCG)
 void launch_cg_update(const CGDev &cg, hipStream_t s);      // after c = B·u
+// mirror fresh columns' transposes into the stale rows of an assembled n×n matrix
+void launch_mirror_rows(int n, double *Bm, const int *cols, int ncols, const int *is_col,
+                        hipStream_t s);
+void launch_copy(double *dst, const double *src, long n, hipStream_t s);
 
 // dense inverse (Gauss–Jordan with partial pivoting) of an n×n column-major matrix
 void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err,

@@ -57,6 +57,7 @@ class DeviceCase:
         d.force_solver = FORCE_SOLVERS[sch.force_solver]
         d.cg_maxiter = sch.cg_maxiter
         d.cg_reltol = sch.cg_reltol
+        d.pc_refresh = sch.pc_refresh
         self._desc = d
         h = C.c_void_p()
         rc = L.ibh_create(C.byref(d), C.byref(h))

@@ -205,7 +205,7 @@ def circle(center, r, h, frame=None) -> RigidBody:
 
 # --------------------------------------------------------------------------- scheme / problem
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
-FORCE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2}
+FORCE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2, "pcg": 3}
 
 
 @dataclass
@@ -214,9 +214,11 @@ class SchemeCNAB:
 
     The extra keywords select the device implementation and have no reference counterpart:
     `precision` of the bulk sine transforms ("fp64", "fp32", "bf16x3") and the surface-force
-    solver ("auto": the precomputed dense inverse for bodies fixed in the grid, CG
-    otherwise; "dense"; "cg").  cg_maxiter / cg_reltol are the reference's cg! settings
-    (cnab/ops.jl:110).
+    solver ("auto": the precomputed dense inverse for bodies fixed in the grid, PCG
+    otherwise; "dense"; "cg": the reference's unpreconditioned cg!; "pcg": cg! with the dense
+    inverse of B as preconditioner, rebuilt for moving bodies every `pc_refresh` steps).
+    cg_maxiter / cg_reltol are the reference's cg! settings (cnab/ops.jl:110); every CG
+    variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.
     """
 
     dt: float
@@ -224,6 +226,7 @@ class SchemeCNAB:
     force_solver: str = "auto"
     cg_maxiter: int = 5000
     cg_reltol: float = 1e-12
+    pc_refresh: int = 50
     beta: tuple = (1.5, -0.5)
 
     def __post_init__(self):

@@ -42,9 +42,12 @@ enum ibh_precision {
 };
 
 enum ibh_force_solver {
-    IBH_FORCE_AUTO = 0,  /* dense precomputed inverse when every body is fixed, else CG    */
+    IBH_FORCE_AUTO = 0,  /* dense precomputed inverse when every body is fixed, else PCG   */
     IBH_FORCE_DENSE = 1, /* Binv_linearmap dense branch, cnab/ops.jl:79-102                  */
-    IBH_FORCE_CG = 2     /* Binv_linearmap CG branch, cnab/ops.jl:104-114                    */
+    IBH_FORCE_CG = 2,    /* Binv_linearmap CG branch, cnab/ops.jl:104-114                    */
+    IBH_FORCE_PCG = 3    /* the CG branch with a preconditioner: the dense inverse of B,     */
+                         /* rebuilt for the moving bodies' columns every pc_refresh steps;  */
+                         /* same stopping rule (‖b−Bx‖ ≤ reltol·‖b−Bx₀‖), far fewer products */
 };
 
 /* Field selectors for ibh_get_field / ibh_set_field (StreamFcnQuantities, streamfcn/states.jl:9-17;
@@ -82,6 +85,7 @@ typedef struct ibh_case_desc {
     int32_t force_solver;   /* enum ibh_force_solver                                      */
     int32_t cg_maxiter;     /* ≤0 → 5000 (cnab/ops.jl:110)                                */
     double cg_reltol;       /* ≤0 → 1e-12 (cnab/ops.jl:110)                               */
+    int32_t pc_refresh;     /* PCG: steps between preconditioner rebuilds (≤0 → 50)       */
 } ibh_case_desc;
 
 typedef struct ibh_ctx ibh_ctx;

@@ -173,7 +173,7 @@ def deltah(rf, rb, dr):
 
 
 # ----------------------------------------------------------------------- stateful case
-FORCE = {"auto": 0, "dense": 1, "cg": 2}
+FORCE = {"auto": 0, "dense": 1, "cg": 2, "pcg": 2}  # the oracle solves every CG variant with plain cg!
 
 
 class OracleRun:

@@ -41,7 +41,8 @@ def pitching_plate_frame(center, theta0_deg, amp_deg, omega, phase):
 
 
 def cylinder_plate_case(n=64, nlev=3, Re=200.0, dt=None, precision="fp64", cg_reltol=1e-12,
-                        seed=2, domain=None, plate_pts=None, cyl_spacing=0.5):
+                        seed=2, domain=None, plate_pts=None, cyl_spacing=0.5,
+                        force_solver="auto", pc_refresh=50):
     """Config 2 geometry (cylinder d=1 at the origin + pitching flat plate of chord 1 at
     (2.5, 0), 30° ± 10°, ω = 1) on a smaller grid."""
     x0, x1, y0, y1 = domain or (-1.28, 3.84, -2.56, 2.56)
@@ -54,5 +55,6 @@ def cylinder_plate_case(n=64, nlev=3, Re=200.0, dt=None, precision="fp64", cg_re
     phase = float(np.random.default_rng(seed).uniform(0, 2 * math.pi))
     plate = RigidBody(pitching_plate_frame((2.5, 0.0), 30.0, 10.0, 1.0, phase), xb, ds)
     dt = dt if dt is not None else 0.2 * h
-    sch = SchemeCNAB(dt, precision=precision, cg_reltol=cg_reltol)
+    sch = SchemeCNAB(dt, precision=precision, cg_reltol=cg_reltol, force_solver=force_solver,
+                     pc_refresh=pc_refresh)
     return Problem(fluid, BodyGroup(cyl, plate), sch)

@@ -180,3 +180,27 @@ def test_step_parity_moving_plate(ibhip):
     prob = cylinder_plate_case(n=64, nlev=2, cg_reltol=1e-15, cyl_spacing=1.0)
     state, orc, hist = run_pair(ibhip, prob, 3)
     assert_state_close(state.case, orc, 1e-8)
+
+
+def test_step_parity_moving_plate_pcg(ibhip):
+    # the preconditioned solve reaches the same tight tolerance as the oracle's plain cg!
+    prob = cylinder_plate_case(n=64, nlev=2, cg_reltol=1e-15, cyl_spacing=1.0,
+                               force_solver="pcg", pc_refresh=2)
+    state, orc, hist = run_pair(ibhip, prob, 5)
+    assert state.case.cg_iters() < orc.cg_iters()
+    assert_state_close(state.case, orc, 1e-8)
+
+
+def test_pcg_matches_dense_on_fixed_bodies(ibhip):
+    import ibhip as m
+
+    p1 = cylinder_case(n=64, nlev=3, force_solver="dense")
+    p2 = cylinder_case(n=64, nlev=3, force_solver="pcg", cg_reltol=1e-14)
+    s1, s2 = m.initstate(p1), m.initstate(p2)
+    v1, v2 = m.initsolver(p1, s1), m.initsolver(p2, s2)
+    for k in range(5):
+        m.advance_(s1, v1, k * p1.scheme.dt)
+        m.advance_(s2, v2, k * p2.scheme.dt)
+        assert s2.case.cg_iters() <= 3
+    assert rel(s2.case.Fb(), s1.case.Fb()) < 1e-10
+    assert rel(s2.case.gamma(), s1.case.gamma()) < 1e-10
