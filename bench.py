@@ -63,6 +63,33 @@ def cpu_baseline(make_problem, n_ref_iters, cap=8):
     }
 
 
+def lift_drag_error(ibhip, make, nsteps, dt):
+    """Run the workload in FP64 and in the requested precision side by side (untimed) and
+    return the drag and lift errors: max over steps 2..n and bodies of |ΔCD| / max|CD| and
+    |ΔCL| / max|CD| (CL is normalised by the drag scale because it is roundoff-seeded in
+    symmetric cases; step 1 carries the impulsive-start spike and is reported apart)."""
+    import numpy as np
+
+    runs = []
+    for prec in ("fp64", None):
+        prob = make(precision="fp64") if prec else make()
+        st = ibhip.initstate(prob)
+        sv = ibhip.initsolver(prob, st)
+        cds, cls = [], []
+        for k in range(nsteps):
+            ibhip.advance_(st, sv, k * dt)
+            cds.append(st.case.CD())
+            cls.append(st.case.CL())
+        runs.append((np.array(cds), np.array(cls)))
+        st.case.close()
+    (cd0, cl0), (cd1, cl1) = runs
+    scale = np.abs(cd0[1:]).max(axis=0)
+    return {"steps": nsteps,
+            "CD_rel": float((np.abs(cd1[1:] - cd0[1:]).max(axis=0) / scale).max()),
+            "CL_rel_to_CD": float((np.abs(cl1[1:] - cl0[1:]).max(axis=0) / scale).max()),
+            "step1_CD_rel": float((np.abs(cd1[0] - cd0[0]) / np.abs(cd0[0])).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +98,9 @@ def main():
     ap.add_argument("--workload", default="cylinder_plate")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one elementwise copy of 2^25 doubles first (256 MiB read + 256 MiB "
+                         "written) so that profiler byte counters can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for "
@@ -83,7 +113,13 @@ def main():
     import ibhip
     from ibhip.workloads import WORKLOADS
 
-    make = lambda **kw: WORKLOADS[args.workload](precision=args.precision, **kw)  # noqa: E731
+    if args.calib_copy:
+        x = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+    make = lambda **kw: WORKLOADS[args.workload](**{"precision": args.precision, **kw})  # noqa: E731
     prob = make()
     fl = prob.fluid
     cells = fl.nx * fl.ny * fl.nlevels
@@ -106,6 +142,7 @@ def main():
     log(f"bench: warmup done ({args.warmup} steps)")
 
     iters = []
+    marker = torch.zeros(1, device="cuda")  # a fill kernel that marks the timed window in traces
     torch.cuda.synchronize()
     case.profile(True)
     t0 = time.perf_counter()
@@ -118,7 +155,13 @@ def main():
     case.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms, launches, flops = case.profile_read(0)
+    marker.zero_()
+    torch.cuda.synchronize()
+    accuracy = None
+    if args.precision != "fp64":
+        accuracy = lift_drag_error(ibhip, make, args.warmup + args.steps, dt)
+        log(f"bench: lift/drag error vs FP64 {accuracy}")
+    ms, launches, flops = case.profile_read({"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision])
     case.profile(False)
     ms_step = 1e3 * elapsed / args.steps
     value = cells * args.steps / elapsed
@@ -133,7 +176,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / peak,
         "traffic": None,
-        "kernel": "k_gemm_f64 (sine-transform contraction)",
+        "kernel": f"k_gemm<{args.precision}> (sine-transform contraction)",
         "avg_launch_us": avg_launch_s * 1e6,
         "launches": launches,
         "gemm_share_of_step": (ms / 1e3) / elapsed,
@@ -159,7 +202,8 @@ def main():
                    "nx": fl.nx, "ny": fl.ny, "levels": fl.nlevels,
                    "body_points": prob.bodies.npoints, "precision": args.precision,
                    "finest_only_cell_steps_per_s": value / fl.nlevels,
-                   "setup_s": t_setup, "force_solve": cg},
+                   "setup_s": t_setup, "force_solve": cg,
+                   "lift_drag_error_vs_fp64": accuracy},
         "roofline": roofline,
         "cpu_baseline": None,
     }

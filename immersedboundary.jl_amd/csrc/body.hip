@@ -148,19 +148,41 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
                        skip);
 }
 
-// y = A·x for the dense n×n column-major Binv (cnab/ops.jl:97)
-__global__ void k_gemv(int n, const double *__restrict__ A, const double *__restrict__ x,
-                       double *__restrict__ y) {
+// y = A·x for a dense n×n column-major matrix (the Binv GEMV of cnab/ops.jl:97 and the PCG
+// preconditioner).  Phase 1: block (row block, column chunk of GV_CH) writes partial sums,
+// one thread per row so every column read is coalesced; phase 2 sums the chunks in a fixed
+// order (deterministic).  ws holds ceil(n/GV_CH)·n doubles.
+static constexpr int GV_CH = 64;
+
+__global__ void k_gemv_part(int n, const double *__restrict__ A, const double *__restrict__ x,
+                            double *__restrict__ ws, const int *skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j0 = blockIdx.y * GV_CH, j1 = min(n, j0 + GV_CH);
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = j0; j < j1; ++j) s += A[i + (long)n * j] * x[j];
+    ws[(long)blockIdx.y * n + i] = s;
+}
+
+__global__ void k_gemv_sum(int n, int nch, const double *__restrict__ ws, double *__restrict__ y,
+                           const int *skip) {
+    if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < n; ++j) s += A[i + (long)n * j] * x[j];
+    for (int c = 0; c < nch; ++c) s += ws[(long)c * n + i];
     y[i] = s;
 }
 
-void launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s) {
+long gemv_ws_size(int n) { return (long)((n + GV_CH - 1) / GV_CH) * n; }
+
+void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
+                 hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_gemv, dim3(nblk(n, 64)), dim3(64), 0, s, n, A, x, y);
+    const int nch = (n + GV_CH - 1) / GV_CH;
+    hipLaunchKernelGGL(k_gemv_part, dim3(nblk(n, 256), nch), dim3(256), 0, s, n, A, x, ws, skip);
+    hipLaunchKernelGGL(k_gemv_sum, dim3(nblk(n, 256)), dim3(256), 0, s, n, nch, ws, y, skip);
 }
 
 __global__ void k_axpy_sub(int n, double *y, const double *v, double a) {
@@ -229,16 +251,6 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 // test (it ≥ maxiter or ‖r‖ ≤ tol) runs before every iteration.  One block does the vector
 // work; the device flag `done` makes the surrounding B-products skip once converged.
 // z = P r for PCG (dense n×n preconditioner), skipped once converged
-__global__ void k_pc_apply(CGDev cg) {
-    if (*cg.done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cg.n) return;
-    const int n = cg.n;
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) s += cg.P[i + (long)n * j] * cg.r[j];
-    cg.z[i] = s;
-}
-
 // PCG direction (IterativeSolvers 0.9.2 PCGIterable): ρ_prev = ρ; ρ = z·r; u = z + (ρ/ρ_prev) u
 __global__ void k_pcg_dir(CGDev cg) {
     __shared__ double sh[17];
@@ -254,7 +266,7 @@ __global__ void k_pcg_dir(CGDev cg) {
 }
 
 void launch_pcg_dir(const CGDev &cg, hipStream_t s) {
-    hipLaunchKernelGGL(k_pc_apply, dim3(nblk(cg.n, 64)), dim3(64), 0, s, cg);
+    launch_gemv(cg.n, cg.P, cg.r, cg.z, cg.ws, cg.done, s);  // z = P r
     hipLaunchKernelGGL(k_pcg_dir, dim3(1), dim3(1024), 0, s, cg);
 }
 

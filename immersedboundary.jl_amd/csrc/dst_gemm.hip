@@ -1,142 +1,532 @@
-// Sine-matrix contractions on MFMA (gfx950), FP64.
+// Sine-matrix contractions on MFMA (gfx950): FP64, FP32 and bf16x3.
 //
 // A 2-D DST-I of an m×p field X is S_m·X·S_p with S[i,k] = sin(π i k / n) (FFTW RODFT00 is
-// 2·S per dimension, fluid-ops.jl:25,49,51).  Each contraction is a column-major batched GEMM
-// C = A·B computed here with v_mfma_f64_16x16x4f64.  The MFMA is fed the transposed problem
-// (Cᵀ = Bᵀ·Aᵀ) so that lane%16 walks the contiguous (x) index of C and stores coalesce.
+// 2·S per dimension, fluid-ops.jl:25,49,51).  Every contraction is computed as a "TN" GEMM
+// C = A·B whose operands are both read k-fastest:
+//   left  contraction  T = S·X : A = S (symmetric, so column i of S is row i), B = X,
+//   right contraction  W = T·S : A = Tᵀ (written transposed by the previous contraction),
+//                                B = S.
+// Tiles are staged in LDS as [row][k].  The output is written column-major (x fastest) or
+// transposed, by choosing which staged tile feeds the MFMA's A operand, so that lane%16
+// always walks the contiguous index of C.  Block tile BM×BN, K tile BK, WM×WN waves; K tiles
+// are double-buffered in LDS with the next tile prefetched into registers.
 //
-// Block tile 64×64, K tile 16, 256 threads = 4 waves in 2×2; each wave owns a 32×32 sub-tile
-// (2×2 MFMA blocks).  K tiles are double-buffered through LDS; global loads of the next tile
-// overlap the MFMAs of the current one.  Epilogues fuse the eigenvalue scaling between the
-// two transforms, the Crank–Nicolson right-hand side, and the circulation projection.
+// Precision policies:
+//   FP64   v_mfma_f64_16x16x4f64,  D row = lane/16 + 4r
+//   FP32   v_mfma_f32_16x16x4f32,  D row = 4(lane/16) + r, operands rounded from FP64
+//   BF16X3 v_mfma_f32_16x16x32_bf16, a·b ≈ ah·bh + ah·bl + al·bh with a = ah + al split on
+//          the LDS store, FP32 accumulation, D row = 4(lane/16) + r
+// (the D layouts were measured on gfx950 with an in-bounds probe).
+#include <hip/hip_bf16.h>
+
 #include "ibh_internal.h"
 
 namespace ibh {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-static constexpr int BM = 64, BN = 64, BK = 16, LDP = 80;  // LDS row pitch (doubles)
+// ------------------------------------------------------------------------ precision policies
+struct PF64 {
+    typedef double E;    // LDS element
+    typedef d4 Acc;
+    static constexpr int KS = 4;      // K per MFMA
+    static constexpr int PAD = 4;     // LDS row padding (elements)
+    __device__ static Acc zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
+    __device__ static int drow(int lq, int r) { return lq + 4 * r; }
+};
+struct PF32 {
+    typedef float E;
+    typedef f4 Acc;
+    static constexpr int KS = 4;
+    static constexpr int PAD = 4;
+    __device__ static Acc zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static int drow(int lq, int r) { return 4 * lq + r; }
+};
+struct PBF3 {
+    typedef __bf16 E;    // hi and lo planes
+    typedef f4 Acc;
+    static constexpr int KS = 32;
+    static constexpr int PAD = 8;
+    __device__ static Acc zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static int drow(int lq, int r) { return 4 * lq + r; }
+};
 
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
+template <class P> struct Frag;
+template <> struct Frag<PF64> {
+    double v;
+    __device__ void load(const double *row, int kk, int lq) { v = row[kk * 4 + lq]; }
+};
+template <> struct Frag<PF32> {
+    float v;
+    __device__ void load(const float *row, int kk, int lq) { v = row[kk * 4 + lq]; }
+};
+template <> struct Frag<PBF3> {
+    bf8 hi, lo;
+};
+
+__device__ inline d4 mma(const Frag<PF64> &a, const Frag<PF64> &b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a.v, b.v, c, 0, 0, 0);
+}
+__device__ inline f4 mma(const Frag<PF32> &a, const Frag<PF32> &b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a.v, b.v, c, 0, 0, 0);
+}
+__device__ inline f4 mma(const Frag<PBF3> &a, const Frag<PBF3> &b, f4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------------ LDS tiles
+template <class P, int ROWS, int BK> struct Tile {
+    typename P::E s[ROWS][BK + P::PAD];
+};
+template <int ROWS, int BK> struct Tile<PBF3, ROWS, BK> {
+    __bf16 hi[ROWS][BK + PBF3::PAD];
+    __bf16 lo[ROWS][BK + PBF3::PAD];
+};
+
+template <class P, int ROWS, int BK>
+__device__ inline void tput(Tile<P, ROWS, BK> &t, int r, int k, double v) {
+    t.s[r][k] = (typename P::E)v;
+}
+template <int ROWS, int BK>
+__device__ inline void tput(Tile<PBF3, ROWS, BK> &t, int r, int k, double v) {
+    const float f = (float)v;
+    const __bf16 h = __float2bfloat16(f);
+    t.hi[r][k] = h;
+    t.lo[r][k] = __float2bfloat16(f - __bfloat162float(h));
+}
+
+template <class P, int ROWS, int BK>
+__device__ inline void fload(Frag<P> &f, const Tile<P, ROWS, BK> &t, int row, int kk, int lq) {
+    f.load(t.s[row], kk, lq);
+}
+template <int ROWS, int BK>
+__device__ inline void fload(Frag<PBF3> &f, const Tile<PBF3, ROWS, BK> &t, int row, int kk,
+                             int lq) {
+    f.hi = *reinterpret_cast<const bf8 *>(&t.hi[row][kk * 32 + lq * 8]);
+    f.lo = *reinterpret_cast<const bf8 *>(&t.lo[row][kk * 32 + lq * 8]);
+}
+
+// ------------------------------------------------------------------------ kernel
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT>
+__global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
+    constexpr int FM = TM / 16, FN = TN / 16;  // MFMA blocks per wave
+    constexpr int KS = P::KS;
+    static_assert(BK % KS == 0 && TM % 16 == 0 && TN % 16 == 0, "tile shape");
+    constexpr int LA = BM * BK / NT, LB = BN * BK / NT;  // elements per thread per tile
+    static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "load split");
+
     if (g.skip && *g.skip) return;
     const int bz = blockIdx.z;
-    const double *__restrict__ A = g.A + bz * g.sA;
-    const double *__restrict__ B = g.B + bz * g.sB;
+    const double *__restrict__ A = g.A + bz * g.sA;  // A(m, k) at A[k + lda·m]
+    const double *__restrict__ B = g.B + bz * g.sB;  // B(k, n) at B[k + ldb·n]
     double *__restrict__ C = g.C + bz * g.sC;
     const int M = g.M, N = g.N, K = g.K;
-    const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
+    const int l16 = lane & 15, lq = lane >> 4;
 
-    __shared__ double As[2][BK][LDP];  // [k][m]
-    __shared__ double Bs[2][BK][LDP];  // [k][n]
+    __shared__ Tile<P, BM, BK> As[2];
+    __shared__ Tile<P, BN, BK> Bs[2];
 
-    d4 acc[2][2];
+    typename P::Acc acc[FN][FM];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < FN; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < FM; ++b) acc[a][b] = P::zero();
 
-    double ra[4], rb[4];
-    const int am = tid & 63, ak = (tid >> 6) * 4;   // A: 64 m × 4 k-quads
-    const int bn = tid >> 2, bk = (tid & 3) * 4;    // B: 64 n × 4 k-quads
-    const bool a_in = (m0 + am) < M;
-    const bool b_in = (n0 + bn) < N;
-    const double *Ap = A + (m0 + am);
-    const double *Bp = B + ldb * (long)(n0 + bn);
-
+    // global → register prefetch: thread t covers element (row = e / BK, k = e % BK) for
+    // e = t + NT·u, so a wave reads whole k-runs of consecutive rows
+    double ra[LA], rb[LB];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int k = k0 + ak + u;
-            ra[u] = (a_in && k < K) ? Ap[lda * (long)k] : 0.0;
+        for (int u = 0; u < LA; ++u) {
+            const int e = tid + NT * u, r = e / BK, k = k0 + e % BK;
+            ra[u] = (m0 + r < M && k < K) ? A[k + g.lda * (long)(m0 + r)] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int k = k0 + bk + u;
-            rb[u] = (b_in && k < K) ? Bp[k] : 0.0;
+        for (int u = 0; u < LB; ++u) {
+            const int e = tid + NT * u, r = e / BK, k = k0 + e % BK;
+            rb[u] = (n0 + r < N && k < K) ? B[k + g.ldb * (long)(n0 + r)] : 0.0;
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) As[buf][ak + u][am] = ra[u];
+        for (int u = 0; u < LA; ++u) {
+            const int e = tid + NT * u;
+            tput(As[buf], e / BK, e % BK, ra[u]);
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) Bs[buf][bk + u][bn] = rb[u];
+        for (int u = 0; u < LB; ++u) {
+            const int e = tid + NT * u;
+            tput(Bs[buf], e / BK, e % BK, rb[u]);
+        }
     };
 
     const int nk = (K + BK - 1) / BK;
     gload(0);
     sstore(0);
     __syncthreads();
-    const int l16 = lane & 15, lq = lane >> 4;
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) gload((kt + 1) * BK);
 #pragma unroll
-        for (int kk = 0; kk < BK / 4; ++kk) {
-            const int kr = kk * 4 + lq;
-            double a0 = Bs[cur][kr][wn * 32 + l16];
-            double a1 = Bs[cur][kr][wn * 32 + 16 + l16];
-            double b0 = As[cur][kr][wm * 32 + l16];
-            double b1 = As[cur][kr][wm * 32 + 16 + l16];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        for (int kk = 0; kk < BK / KS; ++kk) {
+            Frag<P> fa[FM], fb[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fload(fa[i], As[cur], wm * TM + i * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fload(fb[j], Bs[cur], wn * TN + j * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    // column-major C: MFMA rows ↔ n (B tile), lanes ↔ m (A tile);
+                    // transposed C: MFMA rows ↔ m, lanes ↔ n
+                    if (TOUT) acc[j][i] = mma(fa[i], fb[j], acc[j][i]);
+                    else acc[j][i] = mma(fb[j], fa[i], acc[j][i]);
+                }
         }
         if (kt + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
 
-    // epilogue.  v_mfma_f64_16x16x4f64 leaves D[row][col] in lane col + 16·(row % 4),
-    // register row / 4 (row = lane/16 + 4·r); here row ↔ n and col ↔ m, so acc[ni][mi][r]
-    // holds C[m][n] with m = l16 (+16 mi), n = lq + 4·r (+16 ni).
     const double *D = g.D ? g.D + bz * g.sD : nullptr;
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * 32 + mi * 16 + l16;
-                const int n = n0 + wn * 32 + ni * 16 + lq + 4 * r;
-                if (m < M && n < N) {
-                    double v = acc[ni][mi][r];
-                    const long ci = m + ldc * (long)n;
+                int m, n;
+                if (TOUT) {
+                    m = m0 + wm * TM + i * 16 + P::drow(lq, r);
+                    n = n0 + wn * TN + j * 16 + l16;
+                } else {
+                    m = m0 + wm * TM + i * 16 + l16;
+                    n = n0 + wn * TN + j * 16 + P::drow(lq, r);
+                }
+                if (m >= M || n >= N) continue;
+                double v = (double)acc[j][i][r];
+                if (TOUT) {  // C(m, n) at C[n + ldc·m]
+                    C[n + g.ldc * (long)m] = v;
+                    continue;
+                }
+                const long ci = m + g.ldc * (long)n;
+                if (EPI == EPI_STORE) {
+                    C[ci] = v;
+                } else if (EPI == EPI_SCALE) {
+                    C[ci] = v * D[m + g.ldd * (long)n];
+                } else if (EPI == EPI_SUBFROM) {
+                    C[ci] = D[m + g.ldd * (long)n] - v;
+                } else {  // EPI_RHS: rhs = AΓ; rhs += dtβ1 N1; rhs += dtβ2 N2; rhs += rhsbc
+                    const double *N1 = g.N1 + bz * g.sC, *N2 = g.N2 + bz * g.sC;
+                    v = v + g.f1 * N1[ci];
+                    v = v + g.f2 * N2[ci];
+                    const double *bc = g.bcacc + (long)bz * g.nbc;
+                    // apply_bc! edge terms (fluid-ops.jl:256-272), fac already folded in
+                    double e = 0.0;
+                    if (n == 0) e += bc[g.Bo + 1 + m];
+                    if (n == N - 1) e += bc[g.To + 1 + m];
+                    if (m == 0) e += bc[g.Lo + 1 + n];
+                    if (m == M - 1) e += bc[g.Ro + 1 + n];
+                    C[ci] = v + e;
+                }
+            }
+}
+
+// ------------------------------------------------------------------------ even/odd split
+// DST-I symmetry: with n even, h = n/2 and a'' = 0..h-1, rows a = a''+1 and n−a of S·x are
+//   O ± E,  O = Σ_k'' So[a''][k'']·x[2k''],  E = Σ_k'' Se[a''][k'']·x[2k''+1]
+// (0-based x), So[a''][k''] = sin(π(a''+1)(2k''+1)/n), Se[a''][k''] = sin(2π(a''+1)(k''+1)/n)
+// with Se's last row and column zero.  Each part is an h×h dense contraction, so a transform
+// costs half the MFMA work of the full m×m one.  SIDE 0: C = S·X (pairs on the M index, data
+// X as the B operand); SIDE 1: C = X·S (pairs on the N index, data as the A operand).  The
+// data operand is read as 2·BK contiguous k values and split by parity on the LDS store; the
+// butterfly runs in the epilogue.
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE>
+__global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int TM = BM / WM, TN = BN / WN;
+    constexpr int FM = TM / 16, FN = TN / 16;
+    constexpr int KS = P::KS;
+    static_assert(BK % KS == 0 && TM % 16 == 0 && TN % 16 == 0, "tile shape");
+    constexpr int RS = SIDE == 0 ? BM : BN;  // rows of the sine tiles (pair index)
+    constexpr int RD = SIDE == 0 ? BN : BM;  // rows of the data tile
+    constexpr int LS = RS * BK / NT, LD = RD * 2 * BK / NT;
+    static_assert(LS * NT == RS * BK && LD * NT == RD * 2 * BK, "load split");
+
+    if (g.skip && *g.skip) return;
+    const int bz = blockIdx.z;
+    const int h = g.h;                       // pairs
+    const int Kd = g.K;                      // data length along k (= n - 1)
+    const double *__restrict__ X = SIDE == 0 ? g.B + bz * g.sB : g.A + bz * g.sA;
+    const long ldx = SIDE == 0 ? g.ldb : g.lda;
+    double *__restrict__ C = g.C + bz * g.sC;
+    const int M = g.M, N = g.N;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int r0s = SIDE == 0 ? m0 : n0, r0d = SIDE == 0 ? n0 : m0;
+    const int nrd = SIDE == 0 ? N : M;       // valid data rows
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    __shared__ Tile<P, BM, BK> AsO[2], AsE[2];
+    __shared__ Tile<P, BN, BK> BsO[2], BsE[2];
+
+    typename P::Acc accO[FN][FM], accE[FN][FM];
+#pragma unroll
+    for (int a = 0; a < FN; ++a)
+#pragma unroll
+        for (int b = 0; b < FM; ++b) accO[a][b] = accE[a][b] = P::zero();
+
+    double rso[LS], rse[LS], rd[LD];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < LS; ++u) {
+            const int e = tid + NT * u, r = r0s + e / BK, k = k0 + e % BK;
+            const bool in = r < h && k < h;
+            rso[u] = in ? g.So[k + (long)h * r] : 0.0;
+            rse[u] = in ? g.Se[k + (long)h * r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LD; ++u) {
+            const int e = tid + NT * u, r = r0d + e / (2 * BK), q = 2 * k0 + e % (2 * BK);
+            rd[u] = (r < nrd && q < Kd) ? X[q + ldx * (long)r] : 0.0;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < LS; ++u) {
+            const int e = tid + NT * u;
+            if (SIDE == 0) {
+                tput(AsO[buf], e / BK, e % BK, rso[u]);
+                tput(AsE[buf], e / BK, e % BK, rse[u]);
+            } else {
+                tput(BsO[buf], e / BK, e % BK, rso[u]);
+                tput(BsE[buf], e / BK, e % BK, rse[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LD; ++u) {
+            const int e = tid + NT * u, r = e / (2 * BK), q = e % (2 * BK);
+            if (SIDE == 0) {
+                if (q & 1) tput(BsE[buf], r, q >> 1, rd[u]);
+                else tput(BsO[buf], r, q >> 1, rd[u]);
+            } else {
+                if (q & 1) tput(AsE[buf], r, q >> 1, rd[u]);
+                else tput(AsO[buf], r, q >> 1, rd[u]);
+            }
+        }
+    };
+
+    const int nk = (h + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / KS; ++kk) {
+            Frag<P> fa[FM], fb[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fload(fa[i], AsO[cur], wm * TM + i * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fload(fb[j], BsO[cur], wn * TN + j * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    if (TOUT) accO[j][i] = mma(fa[i], fb[j], accO[j][i]);
+                    else accO[j][i] = mma(fb[j], fa[i], accO[j][i]);
+                }
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fload(fa[i], AsE[cur], wm * TM + i * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fload(fb[j], BsE[cur], wn * TN + j * 16 + l16, kk, lq);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    if (TOUT) accE[j][i] = mma(fa[i], fb[j], accE[j][i]);
+                    else accE[j][i] = mma(fb[j], fa[i], accE[j][i]);
+                }
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const double *D = g.D ? g.D + bz * g.sD : nullptr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int mi, ni;  // tile-local row (M side) and column (N side) of this value
+                if (TOUT) {
+                    mi = wm * TM + i * 16 + P::drow(lq, r);
+                    ni = wn * TN + j * 16 + l16;
+                } else {
+                    mi = wm * TM + i * 16 + l16;
+                    ni = wn * TN + j * 16 + P::drow(lq, r);
+                }
+                const int pi = SIDE == 0 ? m0 + mi : n0 + ni;   // pair index a''
+                const int di = SIDE == 0 ? n0 + ni : m0 + mi;   // data row index
+                if (pi >= h || di >= nrd) continue;
+                const double o = (double)accO[j][i][r], e = (double)accE[j][i][r];
+                for (int side2 = 0; side2 < 2; ++side2) {
+                    const int pr = side2 == 0 ? pi : Kd - 1 - pi;
+                    if (side2 == 1 && pr == pi) break;
+                    double v = side2 == 0 ? o + e : o - e;
+                    const int m = SIDE == 0 ? pr : di, n = SIDE == 0 ? di : pr;
+                    if (TOUT) {
+                        C[n + g.ldc * (long)m] = v;
+                        continue;
+                    }
+                    const long ci = m + g.ldc * (long)n;
                     if (EPI == EPI_STORE) {
                         C[ci] = v;
                     } else if (EPI == EPI_SCALE) {
                         C[ci] = v * D[m + g.ldd * (long)n];
                     } else if (EPI == EPI_SUBFROM) {
                         C[ci] = D[m + g.ldd * (long)n] - v;
-                    } else {  // EPI_RHS: rhs = AΓ; rhs += dtβ1 N1; rhs += dtβ2 N2; rhs += rhsbc
+                    } else {
                         const double *N1 = g.N1 + bz * g.sC, *N2 = g.N2 + bz * g.sC;
                         v = v + g.f1 * N1[ci];
                         v = v + g.f2 * N2[ci];
                         const double *bc = g.bcacc + (long)bz * g.nbc;
-                        // apply_bc! edge terms (fluid-ops.jl:256-272), fac already folded in
-                        double e = 0.0;
-                        if (n == 0) e += bc[g.Bo + 1 + m];
-                        if (n == N - 1) e += bc[g.To + 1 + m];
-                        if (m == 0) e += bc[g.Lo + 1 + n];
-                        if (m == M - 1) e += bc[g.Ro + 1 + n];
-                        C[ci] = v + e;
+                        double eb = 0.0;
+                        if (n == 0) eb += bc[g.Bo + 1 + m];
+                        if (n == N - 1) eb += bc[g.To + 1 + m];
+                        if (m == 0) eb += bc[g.Lo + 1 + n];
+                        if (m == M - 1) eb += bc[g.Ro + 1 + n];
+                        C[ci] = v + eb;
                     }
                 }
             }
 }
 
-void launch_gemm_f64(const GemmArgs &g, int epi, hipStream_t s) {
-    dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, g.batch);
-    dim3 block(256);
-    switch (epi) {
-    case EPI_STORE: hipLaunchKernelGGL(k_gemm_f64<EPI_STORE>, grid, block, 0, s, g); break;
-    case EPI_SCALE: hipLaunchKernelGGL(k_gemm_f64<EPI_SCALE>, grid, block, 0, s, g); break;
-    case EPI_SUBFROM: hipLaunchKernelGGL(k_gemm_f64<EPI_SUBFROM>, grid, block, 0, s, g); break;
-    default: hipLaunchKernelGGL(k_gemm_f64<EPI_RHS>, grid, block, 0, s, g); break;
-    }
+// ------------------------------------------------------------------------ variant tables
+typedef void (*KFn)(GemmArgs);
+struct Variant {
+    int bm, bn, nthreads;
+    KFn fn[4][2];  // [epi][tout]
+    const char *name;
+};
+
+#define EPIS(P, BM, BN, BK, WM, WN)                                                       \
+    {{k_gemm<P, BM, BN, BK, WM, WN, EPI_STORE, false>,                                    \
+      k_gemm<P, BM, BN, BK, WM, WN, EPI_STORE, true>},                                    \
+     {k_gemm<P, BM, BN, BK, WM, WN, EPI_SCALE, false>, nullptr},                          \
+     {k_gemm<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false>, nullptr},                        \
+     {k_gemm<P, BM, BN, BK, WM, WN, EPI_RHS, false>, nullptr}}
+#define VAR(P, BM, BN, BK, WM, WN) \
+    {BM, BN, WM * WN * 64, EPIS(P, BM, BN, BK, WM, WN), #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN}
+
+static const Variant VF64[] = {
+    VAR(PF64, 64, 64, 16, 2, 2), VAR(PF64, 64, 64, 32, 2, 2), VAR(PF64, 128, 64, 16, 2, 2),
+    VAR(PF64, 64, 128, 16, 2, 2), VAR(PF64, 128, 128, 16, 2, 2), VAR(PF64, 32, 64, 32, 1, 2),
+    VAR(PF64, 64, 32, 32, 2, 1), VAR(PF64, 128, 64, 32, 2, 2),
+};
+static const Variant VF32[] = {
+    VAR(PF32, 64, 64, 32, 2, 2), VAR(PF32, 128, 64, 32, 2, 2), VAR(PF32, 64, 128, 32, 2, 2),
+    VAR(PF32, 128, 128, 16, 2, 2), VAR(PF32, 128, 128, 32, 2, 2), VAR(PF32, 64, 64, 64, 2, 2),
+};
+static const Variant VBF3[] = {
+    VAR(PBF3, 64, 64, 32, 2, 2), VAR(PBF3, 128, 64, 32, 2, 2), VAR(PBF3, 64, 128, 32, 2, 2),
+    VAR(PBF3, 128, 128, 32, 2, 2), VAR(PBF3, 128, 128, 64, 2, 2), VAR(PBF3, 64, 64, 64, 2, 2),
+};
+
+static const Variant *table(int prec, int *n) {
+    if (prec == 1) { *n = sizeof(VF32) / sizeof(VF32[0]); return VF32; }
+    if (prec == 2) { *n = sizeof(VBF3) / sizeof(VBF3[0]); return VBF3; }
+    *n = sizeof(VF64) / sizeof(VF64[0]);
+    return VF64;
+}
+
+int gemm_variant_count(int prec) {
+    int n;
+    table(prec, &n);
+    return n;
+}
+
+const char *gemm_variant_name(int prec, int v) {
+    int n;
+    const Variant *t = table(prec, &n);
+    return (v >= 0 && v < n) ? t[v].name : "?";
+}
+
+void launch_gemm(const GemmArgs &g, int prec, int variant, int epi, bool tout, hipStream_t s) {
+    int n;
+    const Variant *t = table(prec, &n);
+    const Variant &V = t[(variant >= 0 && variant < n) ? variant : 0];
+    dim3 grid((g.M + V.bm - 1) / V.bm, (g.N + V.bn - 1) / V.bn, g.batch);
+    KFn fn = V.fn[epi][tout ? 1 : 0];
+    hipLaunchKernelGGL(fn, grid, dim3(V.nthreads), 0, s, g);
+}
+
+// even/odd split variants: [side][epi] (side 0 is only used with the transposed store)
+struct VariantEO {
+    int bm, bn, nthreads;
+    KFn fn[2][4];
+    const char *name;
+};
+#define EOV(P, BM, BN, BK, WM, WN)                                                           \
+    {BM, BN, WM * WN * 64,                                                                   \
+     {{k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0>, nullptr, nullptr, nullptr},     \
+      {k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1>,                                \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1>,                                \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1>,                              \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1>}},                                \
+     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN}
+
+static const VariantEO EF64[] = {
+    EOV(PF64, 64, 64, 16, 2, 2), EOV(PF64, 32, 64, 16, 1, 2), EOV(PF64, 64, 32, 16, 2, 1),
+    EOV(PF64, 32, 64, 32, 2, 2), EOV(PF64, 64, 32, 32, 2, 2), EOV(PF64, 64, 64, 32, 2, 2),
+};
+static const VariantEO EF32[] = {
+    EOV(PF32, 64, 64, 16, 2, 2), EOV(PF32, 32, 64, 32, 1, 2), EOV(PF32, 64, 32, 32, 2, 1),
+    EOV(PF32, 64, 64, 32, 2, 2),
+};
+static const VariantEO EBF3[] = {
+    EOV(PBF3, 64, 64, 32, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2),
+    EOV(PBF3, 32, 64, 32, 1, 2),
+};
+
+static const VariantEO *table_eo(int prec, int *n) {
+    if (prec == 1) { *n = sizeof(EF32) / sizeof(EF32[0]); return EF32; }
+    if (prec == 2) { *n = sizeof(EBF3) / sizeof(EBF3[0]); return EBF3; }
+    *n = sizeof(EF64) / sizeof(EF64[0]);
+    return EF64;
+}
+
+int gemm_eo_variant_count(int prec) {
+    int n;
+    table_eo(prec, &n);
+    return n;
+}
+
+const char *gemm_eo_variant_name(int prec, int v) {
+    int n;
+    const VariantEO *t = table_eo(prec, &n);
+    return (v >= 0 && v < n) ? t[v].name : "?";
+}
+
+void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi, hipStream_t s) {
+    int n;
+    const VariantEO *t = table_eo(prec, &n);
+    const VariantEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
+    const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
+    dim3 grid((gm + V.bm - 1) / V.bm, (gn + V.bn - 1) / V.bn, g.batch);
+    hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], grid, dim3(V.nthreads), 0, s, g);
 }
 
 }  // namespace ibh

@@ -293,6 +293,24 @@ __global__ void k_sine_matrix(double *S, int n) {
     S[t] = sinpi((double)r / (double)n);
 }
 
+// Odd and even parts of the DST-I matrix for the split contraction, arguments reduced
+// exactly in integers before the sine.
+__global__ void k_sine_split(double *So, double *Se, int n) {
+    const long h = n / 2;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= h * h) return;
+    const long k = t % h, a = t / h;  // element [k + h·a]
+    So[t] = sinpi((double)(((a + 1) * (2 * k + 1)) % (2L * n)) / (double)n);
+    Se[t] = (a == h - 1 || k == h - 1)
+                ? 0.0
+                : sinpi((double)((2 * (a + 1) * (k + 1)) % (2L * n)) / (double)n);
+}
+
+void launch_sine_split(double *So, double *Se, int n, hipStream_t s) {
+    long hh = (long)(n / 2) * (n / 2);
+    hipLaunchKernelGGL(k_sine_split, dim3(nblk(hh, 256)), dim3(256), 0, s, So, Se, n);
+}
+
 void launch_sine_matrix(double *S, int n, hipStream_t s) {
     long mm = (long)(n - 1) * (n - 1);
     hipLaunchKernelGGL(k_sine_matrix, dim3(nblk(mm, 256)), dim3(256), 0, s, S, n);

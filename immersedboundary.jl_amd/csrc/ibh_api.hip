@@ -6,9 +6,12 @@
 // products B·z (cnab/ops.jl:125-143) are batched over columns for the dense-inverse setup.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <utility>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ibh.h"
@@ -17,6 +20,16 @@
 using namespace ibh;
 
 namespace {
+
+// kind: 0 full contraction, 1 split left (C = S·X), 2 split right (C = X·S)
+struct TuneKey {
+    int prec, kind, M, N, K, batch;
+    bool tout;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(prec, kind, M, N, K, batch, tout) <
+               std::tie(o.prec, o.kind, o.M, o.N, o.K, o.batch, o.tout);
+    }
+};
 
 struct Prof {
     bool on = false;
@@ -40,6 +53,9 @@ struct ibh_ctx {
 
     // transforms
     double *Sm = nullptr, *Sp = nullptr;
+    // odd/even parts of the sine matrices for the split contraction (x and y dimensions)
+    double *Som = nullptr, *Sem = nullptr, *Sop = nullptr, *Sep = nullptr;
+    bool use_eo = true;
     double *sD = nullptr, *sA = nullptr, *sAinv = nullptr;
     // state (StreamFcnQuantities + CNAB.State.nonlin)
     double *gam = nullptr, *psi = nullptr, *q = nullptr, *nl[2] = {nullptr, nullptr};
@@ -73,6 +89,12 @@ struct ibh_ctx {
     Prof prof;
     std::string err;
     std::vector<void *> allocs;
+    // GEMM autotuner: chosen variant per shape, scratch output, log of the choices
+    std::map<TuneKey, int> tuned;
+    int tune_fixed = -1;
+    double *tune_buf = nullptr;
+    size_t tune_cap = 0;
+    std::vector<std::string> tune_log;
 };
 
 namespace {
@@ -121,17 +143,81 @@ void prof_begin(ibh_ctx *c, hipEvent_t *e0) {
     hipEventRecord(*e0, c->s);
 }
 
-void gemm(ibh_ctx *c, GemmArgs g, int epi) {
+// --------------------------------------------------------------------------- autotuner
+// Every tile variant of a precision sums k in the same order, so they give identical
+// results; the first use of a GEMM shape times each variant on the real operands (output to
+// a scratch buffer) and keeps the fastest.
+void launch_kind(const GemmArgs &g, int prec, int kind, int v, int epi, bool tout,
+                 hipStream_t s) {
+    if (kind == 0) launch_gemm(g, prec, v, epi, tout, s);
+    else launch_gemm_eo(g, prec, v, kind - 1, epi, s);
+}
+
+int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
+    const TuneKey key{prec, kind, g0.M, g0.N, g0.K, g0.batch, tout};
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) return it->second;
+    if (c->tune_fixed >= 0) return c->tuned[key] = c->tune_fixed;
+    const long ext = std::max((long)g0.M, (long)g0.N) * g0.ldc;
+    const size_t need = (size_t)std::max(0L, (long)(g0.batch - 1)) * std::max(g0.sC, 0L) + ext;
+    if (need > c->tune_cap) {
+        if (c->tune_buf) hipFree(c->tune_buf);
+        c->tune_buf = nullptr;
+        if (hipMalloc(&c->tune_buf, need * sizeof(double)) != hipSuccess) return 0;
+        c->tune_cap = need;
+    }
+    GemmArgs g = g0;
+    g.C = c->tune_buf;
+    g.skip = nullptr;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int best = 0;
+    float best_ms = 1e30f;
+    const int nv = kind == 0 ? gemm_variant_count(prec) : gemm_eo_variant_count(prec);
+    for (int v = 0; v < nv; ++v) {
+        launch_kind(g, prec, kind, v, EPI_STORE, tout, c->s);
+        hipEventRecord(e0, c->s);
+        for (int r = 0; r < 3; ++r) launch_kind(g, prec, kind, v, EPI_STORE, tout, c->s);
+        hipEventRecord(e1, c->s);
+        hipEventSynchronize(e1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (hipGetLastError() == hipSuccess && ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    const char *vname = kind == 0 ? gemm_variant_name(prec, best) : gemm_eo_variant_name(prec, best);
+    c->tune_log.push_back(std::string(vname) + (kind == 2 ? " right" : kind == 1 ? " left" : "") +
+                          " M=" + std::to_string(g0.M) + " N=" + std::to_string(g0.N) + " K=" +
+                          std::to_string(g0.K) + " batch=" + std::to_string(g0.batch) +
+                          (tout ? " T" : "") + " " + std::to_string(best_ms / 3 * 1e3) + "us");
+    return c->tuned[key] = best;
+}
+
+// MFMA flops a launch executes: 2MNK for the full contraction, 2·2·h·h·(data rows) for the
+// split one (two h×h parts)
+double gemm_flops(const GemmArgs &g, int kind) {
+    if (kind == 0) return 2.0 * g.M * (double)g.N * g.K * g.batch;
+    const double rows = kind == 1 ? g.N : g.M;
+    return 4.0 * g.h * (double)g.h * rows * g.batch;
+}
+
+void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
+    const int v = tune(c, g, prec, kind, tout);
     if (c->prof.on) {
         hipEvent_t e0;
         prof_begin(c, &e0);
-        launch_gemm_f64(g, epi, c->s);
+        launch_kind(g, prec, kind, v, epi, tout, c->s);
         hipEvent_t e1 = c->prof.pool[c->prof.used + 1];
         hipEventRecord(e1, c->s);
         c->prof.used += 2;
-        c->prof.rec.push_back({0, 2.0 * g.M * (double)g.N * g.K * g.batch});
+        c->prof.rec.push_back({prec, gemm_flops(g, kind)});
     } else {
-        launch_gemm_f64(g, epi, c->s);
+        launch_kind(g, prec, kind, v, epi, tout, c->s);
     }
 }
 
@@ -151,37 +237,77 @@ struct RhsExtras {
     double f1, f2;
 };
 
+// The four contractions are TN GEMMs (both operands k-fastest): the left ones read S_m
+// through its symmetry and write their result transposed, so the right ones read it
+// k-fastest too.
 void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, const double *scale,
-            long sscale, int nbatch, int epi, const RhsExtras *rx, const int *skip) {
+            long sscale, int nbatch, int epi, const RhsExtras *rx, const int *skip,
+            int prec = -1) {
     const Dom &d = c->d;
     const int m = d.m, p = d.p;
     const long ng = d.ng;
+    if (prec < 0) prec = c->precision;
     GemmArgs g = gargs();
     g.batch = nbatch;
     g.skip = skip;
-    // T1 = S_m · in
+    if (c->use_eo) {
+        // same four contractions through the odd/even split (half the MFMA work)
+        g.M = m; g.N = p; g.K = m; g.h = d.nx / 2;
+        g.So = c->Som; g.Se = c->Sem;
+        g.B = in; g.ldb = m; g.sB = sin;
+        g.C = c->T1; g.ldc = p; g.sC = ng;
+        gemm(c, g, EPI_STORE, prec, true, 1);
+        g.K = p; g.h = d.ny / 2;
+        g.So = c->Sop; g.Se = c->Sep;
+        g.B = nullptr;
+        g.A = c->T1; g.lda = p; g.sA = ng;
+        g.C = c->T2; g.ldc = m; g.sC = ng;
+        g.D = scale; g.ldd = m; g.sD = sscale;
+        gemm(c, g, EPI_SCALE, prec, false, 2);
+        g.D = nullptr; g.A = nullptr;
+        g.K = m; g.h = d.nx / 2;
+        g.So = c->Som; g.Se = c->Sem;
+        g.B = c->T2; g.ldb = m; g.sB = ng;
+        g.C = c->T1; g.ldc = p; g.sC = ng;
+        gemm(c, g, EPI_STORE, prec, true, 1);
+        g.K = p; g.h = d.ny / 2;
+        g.So = c->Sop; g.Se = c->Sep;
+        g.B = nullptr;
+        g.A = c->T1; g.lda = p; g.sA = ng;
+        g.C = out; g.ldc = m; g.sC = sout;
+        if (epi == EPI_SUBFROM) {
+            g.D = out; g.ldd = m; g.sD = sout;
+        } else if (epi == EPI_RHS) {
+            g.N1 = rx->N1; g.N2 = rx->N2; g.bcacc = rx->bcacc;
+            g.f1 = rx->f1; g.f2 = rx->f2;
+            g.nbc = d.nbc; g.Lo = d.Lo; g.Ro = d.Ro; g.Bo = d.Bo; g.To = d.To;
+        }
+        gemm(c, g, epi, prec, false, 2);
+        return;
+    }
+    // T1ᵀ = (S_m · in)ᵀ
     g.M = m; g.N = p; g.K = m;
     g.A = c->Sm; g.lda = m; g.sA = 0;
     g.B = in; g.ldb = m; g.sB = sin;
-    g.C = c->T1; g.ldc = m; g.sC = ng;
-    gemm(c, g, EPI_STORE);
+    g.C = c->T1; g.ldc = p; g.sC = ng;
+    gemm(c, g, EPI_STORE, prec, true);
     // T2 = (T1 · S_p) ⊙ scale
     g.M = m; g.N = p; g.K = p;
-    g.A = c->T1; g.lda = m; g.sA = ng;
+    g.A = c->T1; g.lda = p; g.sA = ng;
     g.B = c->Sp; g.ldb = p; g.sB = 0;
     g.C = c->T2; g.ldc = m; g.sC = ng;
     g.D = scale; g.ldd = m; g.sD = sscale;
-    gemm(c, g, EPI_SCALE);
-    // T1 = S_m · T2
+    gemm(c, g, EPI_SCALE, prec);
+    // T1ᵀ = (S_m · T2)ᵀ
     g.D = nullptr;
     g.M = m; g.N = p; g.K = m;
     g.A = c->Sm; g.lda = m; g.sA = 0;
     g.B = c->T2; g.ldb = m; g.sB = ng;
-    g.C = c->T1; g.ldc = m; g.sC = ng;
-    gemm(c, g, EPI_STORE);
+    g.C = c->T1; g.ldc = p; g.sC = ng;
+    gemm(c, g, EPI_STORE, prec, true);
     // out = T1 · S_p (+ epilogue)
     g.M = m; g.N = p; g.K = p;
-    g.A = c->T1; g.lda = m; g.sA = ng;
+    g.A = c->T1; g.lda = p; g.sA = ng;
     g.B = c->Sp; g.ldb = p; g.sB = 0;
     g.C = out; g.ldc = m; g.sC = sout;
     if (epi == EPI_SUBFROM) {
@@ -191,7 +317,7 @@ void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, cons
         g.f1 = rx->f1; g.f2 = rx->f2;
         g.nbc = d.nbc; g.Lo = d.Lo; g.Ro = d.Ro; g.Bo = d.Bo; g.To = d.To;
     }
-    gemm(c, g, epi);
+    gemm(c, g, epi, prec);
 }
 
 // Vort2Flux (fluid-ops.jl:440-472) on nc independent multilevel fields.  G, Psi: per column
@@ -372,7 +498,7 @@ int force_solve(ibh_ctx *c) {
     const int nf = 2 * c->nb;
     if (nf == 0) return IBH_OK;
     if (!c->use_cg) {
-        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->s);
+        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->cg.ws, nullptr, c->s);
         return IBH_OK;
     }
     int rc;
@@ -502,7 +628,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     if (D.nx % 4 || D.ny % 4 || D.nx < 16 || D.ny < 16 || D.nlev < 1 || D.nlev > 30 || D.h <= 0 ||
         D.nbodies < 0)
         return IBH_EINVAL;
-    if (D.precision != IBH_FP64) return IBH_EINVAL;
+    if (D.precision < IBH_FP64 || D.precision > IBH_BF16X3) return IBH_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return IBH_EDEVICE;
     ibh_ctx *c = new ibh_ctx();
@@ -525,6 +651,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     c->boff[D.nbodies] = nb;
     c->nb = nb;
     c->precision = D.precision;
+    if (const char *fv = std::getenv("IBH_GEMM_VARIANT")) c->tune_fixed = std::atoi(fv);
     c->force_solver = D.force_solver;
     c->use_pcg = D.force_solver == IBH_FORCE_PCG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
     c->use_cg = D.force_solver == IBH_FORCE_CG || c->use_pcg;
@@ -552,6 +679,18 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->Sm, (size_t)d.m * d.m));
     c->Sp = c->Sm;
     if (d.p != d.m) CK(dalloc(c, &c->Sp, (size_t)d.p * d.p));
+    {
+        const size_t hx = D.nx / 2, hy = D.ny / 2;
+        CK(dalloc(c, &c->Som, hx * hx));
+        CK(dalloc(c, &c->Sem, hx * hx));
+        c->Sop = c->Som;
+        c->Sep = c->Sem;
+        if (D.ny != D.nx) {
+            CK(dalloc(c, &c->Sop, hy * hy));
+            CK(dalloc(c, &c->Sep, hy * hy));
+        }
+        if (const char *e = std::getenv("IBH_DST_SPLIT")) c->use_eo = std::atoi(e) != 0;
+    }
     CK(dalloc(c, &c->sD, ng));
     CK(dalloc(c, &c->sA, L * ng));
     CK(dalloc(c, &c->sAinv, L * ng));
@@ -593,6 +732,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.c, nf));
     CK(dalloc(c, &c->cg.u, nf));
     CK(dalloc(c, &c->cg.z, nf));
+    CK(dalloc(c, &c->cg.ws, gemv_ws_size(nf)));
     CK(dalloc(c, &c->cg.sc, 4));
     CK(dalloc(c, &c->cg.it, 2));
     CK(dalloc(c, &c->cg.done, 2));
@@ -604,6 +744,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     // constants
     launch_sine_matrix(c->Sm, D.nx, c->s);
     if (c->Sp != c->Sm) launch_sine_matrix(c->Sp, D.ny, c->s);
+    launch_sine_split(c->Som, c->Sem, D.nx, c->s);
+    if (c->Sop != c->Som) launch_sine_split(c->Sop, c->Sep, D.ny, c->s);
     launch_scale_tables(d, c->sD, c->sA, c->sAinv, c->L, c->h, c->dt, c->Re, c->s);
     std::vector<double> fac(L + 1, 0.0);
     for (int l = 0; l < L; ++l) {
@@ -632,6 +774,7 @@ void ibh_destroy(ibh_ctx *c) {
     if (!c) return;
     if (c->s) hipStreamSynchronize(c->s);
     for (void *p : c->allocs) hipFree(p);
+    if (c->tune_buf) hipFree(c->tune_buf);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
@@ -790,14 +933,14 @@ int ibh_op_dst2(ibh_ctx *c, const double *in, double *out) {
     // RODFT00 in both dimensions = 4·S_m·X·S_p: two contractions, then ×4
     GemmArgs g = gargs();
     g.M = d.m; g.N = d.p; g.K = d.m;
-    g.A = c->Sm; g.lda = d.m; g.B = din; g.ldb = d.m; g.C = c->T1; g.ldc = d.m;
-    gemm(c, g, EPI_STORE);
+    g.A = c->Sm; g.lda = d.m; g.B = din; g.ldb = d.m; g.C = c->T1; g.ldc = d.p;
+    gemm(c, g, EPI_STORE, c->precision, true);
     double *four = B.get(d.ng);
     launch_fill(four, d.ng, 4.0, c->s);
     g.M = d.m; g.N = d.p; g.K = d.p;
-    g.A = c->T1; g.lda = d.m; g.B = c->Sp; g.ldb = d.p; g.C = dout; g.ldc = d.m;
+    g.A = c->T1; g.lda = d.p; g.B = c->Sp; g.ldb = d.p; g.C = dout; g.ldc = d.m;
     g.D = four; g.ldd = d.m;
-    gemm(c, g, EPI_SCALE);
+    gemm(c, g, EPI_SCALE, c->precision);
     return down(c, out, dout, d.ng);
 }
 
@@ -902,6 +1045,14 @@ int ibh_op_B(ibh_ctx *c, const double *z, double *x) {
     double *dz = B.up(z, nf), *dx = B.get(nf);
     B_times(c, dz, 0, dx, 0, 1, nullptr);
     return down(c, x, dx, nf);
+}
+
+int ibh_tune_report(ibh_ctx *c, char *buf, int32_t len) {
+    if (!c || !buf || len <= 0) return IBH_EINVAL;
+    std::string r;
+    for (auto &l : c->tune_log) r += l + "\n";
+    std::snprintf(buf, (size_t)len, "%s", r.c_str());
+    return IBH_OK;
 }
 
 int ibh_profile_enable(ibh_ctx *c, int32_t on) {

@@ -28,7 +28,8 @@ inline Dom make_dom(int nx, int ny) {
 }
 
 // ---------------------------------------------------------------------------- sine GEMM
-// C[b] = op(A[b]·B[b]) for column-major operands; batch b = blockIdx.z.
+// C[b] = op(A[b]·B[b]) with A(m, k) at A[k + lda·m] and B(k, n) at B[k + ldb·n] (both
+// k-fastest); batch b = blockIdx.z.
 enum Epi {
     EPI_STORE = 0,   // C = acc
     EPI_SCALE = 1,   // C = acc ⊙ D          (eigenvalue scaling between the two transforms)
@@ -47,10 +48,21 @@ struct GemmArgs {
     double f1, f2;
     int nbc, Lo, Ro, Bo, To;
     const int *skip;  // if non-null and *skip != 0 the launch does nothing
+    // even/odd split: h = n/2 pairs, So/Se h×h sine parts stored k-fastest ([k + h·a])
+    const double *So, *Se;
+    int h;
     int batch;
 };
 
-void launch_gemm_f64(const GemmArgs &g, int epi, hipStream_t s);
+// prec: 0 FP64, 1 FP32, 2 bf16x3; variant indexes the tile table of that precision;
+// tout writes C transposed (C(m, n) at C[n + ldc·m], EPI_STORE only).
+void launch_gemm(const GemmArgs &g, int prec, int variant, int epi, bool tout, hipStream_t s);
+int gemm_variant_count(int prec);
+const char *gemm_variant_name(int prec, int v);
+// even/odd split contraction: side 0 C = S·X (transposed store), side 1 C = X·S (any epi)
+void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi, hipStream_t s);
+int gemm_eo_variant_count(int prec);
+const char *gemm_eo_variant_name(int prec, int v);
 
 // ---------------------------------------------------------------------------- fluid kernels
 // All take a batch count `nb` over levels (or columns) with element strides.
@@ -81,6 +93,9 @@ void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const do
                       hipStream_t s);
 void launch_fill(double *p, long n, double v, hipStream_t s);
 void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+1)(k+1)/n), (n-1)²
+// So[a][k] = sin(π(a+1)(2k+1)/n), Se[a][k] = sin(2π(a+1)(k+1)/n) (0 on the last row/column),
+// both h×h (h = n/2) stored [k + h·a]
+void launch_sine_split(double *So, double *Se, int n, hipStream_t s);
 void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
                          double dt, double Re, hipStream_t s);
 
@@ -99,7 +114,9 @@ void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0
 // q = Eᵀ fb (q zeroed by the kernel pair)
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, hipStream_t s);
-void launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
+void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
+                 hipStream_t s);
+long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
 void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s);  // y -= a v
 void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
                    const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
@@ -109,6 +126,7 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 struct CGDev {
     double *x, *b, *r, *c, *u;  // 2nb each
     double *z;                  // preconditioned residual (PCG)
+    double *ws;                 // GEMV workspace (gemv_ws_size(n))
     const double *P;            // dense preconditioner n×n (PCG) or null
     double *sc;                 // [residual, prev_residual, tol, ρ]
     int *it, *done;

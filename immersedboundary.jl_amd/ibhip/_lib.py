@@ -32,7 +32,7 @@ EXPORTED = [
     "ibh_set_field", "ibh_dims", "ibh_time", "ibh_cg_iterations", "ibh_op_lapinv", "ibh_op_dst2",
     "ibh_op_rot", "ibh_op_curl", "ibh_op_coarsify", "ibh_op_get_bc", "ibh_op_apply_bc",
     "ibh_op_nonlinear", "ibh_op_vort2flux", "ibh_op_reg_update", "ibh_op_E", "ibh_op_ET",
-    "ibh_op_B", "ibh_profile_enable", "ibh_profile_read",
+    "ibh_op_B", "ibh_profile_enable", "ibh_profile_read", "ibh_tune_report",
 ]
 
 

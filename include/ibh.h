@@ -161,6 +161,9 @@ int ibh_op_B(ibh_ctx *ctx, const double *z, double *x);
 
 /* ---- profiling (HIP events around each launch of the sine-transform kernels) ----------- */
 int ibh_profile_enable(ibh_ctx *ctx, int32_t on);
+/* The GEMM autotuner's choices so far (one line per contraction shape: tile variant, sizes,
+ * measured time), as text.  No reference counterpart. */
+int ibh_tune_report(ibh_ctx *ctx, char *buf, int32_t len);
 /* kind 0: FP64 transform GEMMs, 1: FP32, 2: bf16x3.  Returns the summed kernel time (ms),
  * launch count and algorithmic flops since the last enable. */
 int ibh_profile_read(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches, double *flops);

@@ -204,3 +204,51 @@ def test_pcg_matches_dense_on_fixed_bodies(ibhip):
         assert s2.case.cg_iters() <= 3
     assert rel(s2.case.Fb(), s1.case.Fb()) < 1e-10
     assert rel(s2.case.gamma(), s1.case.gamma()) < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 5e-6), ("bf16x3", 1e-4)])
+def test_dst2_mixed_precision(ibhip, precision, tol):
+    import scipy.fft as sf
+
+    prob = cylinder_case(n=200, nlev=2, precision=precision)
+    case = ibhip.DeviceCase(prob)
+    x = np.random.default_rng(9).standard_normal(199 * 199)
+    want = sf.dstn(x.reshape(199, 199, order="F"), type=1).reshape(-1, order="F")
+    err = rel(case.dst2(x), want)
+    assert err < tol, err
+
+
+def test_gemm_tile_variants_identical(ibhip, monkeypatch):
+    # every tile variant sums k in the same order, so the autotuner's choice cannot change
+    # the numbers
+    prob = cylinder_case(n=200, nlev=2)
+    x = np.random.default_rng(10).standard_normal(199 * 199)
+    monkeypatch.setenv("IBH_DST_SPLIT", "0")
+    outs = []
+    for v in range(8):
+        monkeypatch.setenv("IBH_GEMM_VARIANT", str(v))
+        outs.append(ibhip.DeviceCase(prob).lapinv(2, 0, x))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_split_contraction_matches_full(ibhip, monkeypatch):
+    prob = cylinder_case(n=200, nlev=2)
+    x = np.random.default_rng(12).standard_normal(199 * 199)
+    monkeypatch.setenv("IBH_DST_SPLIT", "0")
+    full = [ibhip.DeviceCase(prob).lapinv(mode, 0, x) for mode in (0, 2)]
+    monkeypatch.setenv("IBH_DST_SPLIT", "1")
+    split = [ibhip.DeviceCase(prob).lapinv(mode, 0, x) for mode in (0, 2)]
+    for a, b in zip(split, full):
+        assert rel(a, b) < 1e-13
+
+
+def test_split_tile_variants_identical(ibhip, monkeypatch):
+    prob = cylinder_case(n=200, nlev=2)
+    x = np.random.default_rng(13).standard_normal(199 * 199)
+    outs = []
+    for v in range(6):
+        monkeypatch.setenv("IBH_GEMM_VARIANT", str(v))
+        outs.append(ibhip.DeviceCase(prob).lapinv(2, 0, x))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
