@@ -76,8 +76,13 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
 // E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w, summed column-major like the
 // reference's sumprod; q0 (constant per component) is added to q first, as in Q = qs + q0.
 __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
-                    double *__restrict__ fb, long sfb, const int *skip) {
+                    double *__restrict__ fb, long sfb, const int *skip, const double *uv,
+                    double uvscale) {
     if (skip && *skip) return;
+    if (uv) {  // q0 = freestream·h from the device parameter block
+        u0 = uv[0] * uvscale;
+        v0 = uv[1] * uvscale;
+    }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * r.nb) return;
     const int b = blockIdx.y;
@@ -104,10 +109,11 @@ __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, doub
 }
 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
-              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s) {
+              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s, const double *uv,
+              double uvscale) {
     if (r.nb == 0) return;
     hipLaunchKernelGGL(k_E, dim3(nblk(2L * r.nb, 128), nbatch), dim3(128), 0, s, d, r, q, sq, u0,
-                       v0, fb, sfb, skip);
+                       v0, fb, sfb, skip, uv, uvscale);
 }
 
 // Eᵀ (Reg), structure-ops.jl:63-86: q .= 0 then scatter-add the 13×13 stencils of every point.

@@ -250,7 +250,7 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
 // X as the B operand); SIDE 1: C = X·S (pairs on the N index, data as the A operand).  The
 // data operand is read as 2·BK contiguous k values and split by parity on the LDS store; the
 // butterfly runs in the epilogue.
-template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE>
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB = 2>
 __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
     constexpr int NT = WM * WN * 64;
     constexpr int TM = BM / WM, TN = BN / WN;
@@ -277,8 +277,8 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
     const int wm = wave % WM, wn = wave / WM;
     const int l16 = lane & 15, lq = lane >> 4;
 
-    __shared__ Tile<P, BM, BK> AsO[2], AsE[2];
-    __shared__ Tile<P, BN, BK> BsO[2], BsE[2];
+    __shared__ Tile<P, BM, BK> AsO[NB], AsE[NB];
+    __shared__ Tile<P, BN, BK> BsO[NB], BsE[NB];
 
     typename P::Acc accO[FN][FM], accE[FN][FM];
 #pragma unroll
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
     sstore(0);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
+        const int cur = NB == 2 ? (kt & 1) : 0;
         if (kt + 1 < nk) gload((kt + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK / KS; ++kk) {
@@ -359,8 +359,16 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
                     else accE[j][i] = mma(fb[j], fa[i], accE[j][i]);
                 }
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
-        __syncthreads();
+        if (NB == 2) {
+            if (kt + 1 < nk) sstore(cur ^ 1);
+            __syncthreads();
+        } else {  // one LDS buffer: wait for every wave to finish reading before refilling
+            __syncthreads();
+            if (kt + 1 < nk) {
+                sstore(0);
+                __syncthreads();
+            }
+        }
     }
 
     const double *D = g.D ? g.D + bz * g.sD : nullptr;
@@ -479,26 +487,30 @@ struct VariantEO {
     KFn fn[2][4];
     const char *name;
 };
-#define EOV(P, BM, BN, BK, WM, WN)                                                           \
+#define EOV(P, BM, BN, BK, WM, WN, NB)                                                       \
     {BM, BN, WM * WN * 64,                                                                   \
-     {{k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0>, nullptr, nullptr, nullptr},     \
-      {k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1>,                                \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1>,                                \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1>,                              \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1>}},                                \
-     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN}
+     {{k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB>, nullptr, nullptr, nullptr}, \
+      {k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB>,                            \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB>,                            \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB>,                          \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB>}},                            \
+     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB}
 
 static const VariantEO EF64[] = {
-    EOV(PF64, 64, 64, 16, 2, 2), EOV(PF64, 32, 64, 16, 1, 2), EOV(PF64, 64, 32, 16, 2, 1),
-    EOV(PF64, 32, 64, 32, 2, 2), EOV(PF64, 64, 32, 32, 2, 2), EOV(PF64, 64, 64, 32, 2, 2),
+    EOV(PF64, 64, 64, 16, 2, 2, 2), EOV(PF64, 32, 64, 16, 1, 2, 2), EOV(PF64, 64, 32, 16, 2, 1, 2),
+    EOV(PF64, 32, 64, 32, 2, 2, 2), EOV(PF64, 64, 32, 32, 2, 2, 2), EOV(PF64, 64, 64, 32, 2, 2, 2),
+    EOV(PF64, 64, 64, 16, 2, 2, 1), EOV(PF64, 32, 64, 16, 2, 2, 1), EOV(PF64, 64, 32, 16, 2, 2, 1),
+    EOV(PF64, 32, 32, 16, 2, 2, 1), EOV(PF64, 64, 64, 8, 2, 2, 1),  EOV(PF64, 32, 64, 8, 2, 2, 1),
 };
 static const VariantEO EF32[] = {
-    EOV(PF32, 64, 64, 16, 2, 2), EOV(PF32, 32, 64, 32, 1, 2), EOV(PF32, 64, 32, 32, 2, 1),
-    EOV(PF32, 64, 64, 32, 2, 2),
+    EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
+    EOV(PF32, 64, 64, 32, 2, 2, 2), EOV(PF32, 64, 64, 16, 2, 2, 1), EOV(PF32, 32, 64, 16, 2, 2, 1),
+    EOV(PF32, 64, 32, 16, 2, 2, 1), EOV(PF32, 64, 64, 32, 2, 2, 1),
 };
 static const VariantEO EBF3[] = {
-    EOV(PBF3, 64, 64, 32, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2),
-    EOV(PBF3, 32, 64, 32, 1, 2),
+    EOV(PBF3, 64, 64, 32, 2, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2, 2),
+    EOV(PBF3, 32, 64, 32, 1, 2, 2), EOV(PBF3, 64, 64, 32, 2, 2, 1), EOV(PBF3, 128, 64, 32, 2, 2, 1),
+    EOV(PBF3, 64, 128, 32, 2, 2, 1), EOV(PBF3, 128, 128, 32, 2, 2, 1),
 };
 
 static const VariantEO *table_eo(int prec, int *n) {

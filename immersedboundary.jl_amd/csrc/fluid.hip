@@ -225,9 +225,14 @@ __device__ inline double fqy(const NLCtx &c, int i, int j) {
 __global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
                             const double *__restrict__ g, long sg, const double *__restrict__ gbc,
                             long sbc, double gbc_scale, double *__restrict__ out, long sout,
-                            double ux, double uy, double h, int lev0, int q0_given) {
+                            double ux, double uy, double h, int lev0, int q0_given,
+                            const double *__restrict__ uv) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
+    if (uv) {  // freestream from the device parameter block (graph-replayable steps)
+        ux = uv[0];
+        uy = uv[1];
+    }
     const int b = blockIdx.y;
     const int lev = lev0 + b;
     const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
@@ -247,9 +252,9 @@ __global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
                       const double *gbc, long sbc, double gbc_scale, double *out, long sout,
                       double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
-                      int q0_given) {
+                      int q0_given, const double *uv) {
     hipLaunchKernelGGL(k_nonlinear, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, q, sq, g,
-                       sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given);
+                       sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given, uv);
 }
 
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac[l]·bcraw[l] for ℓ ≤ L-2, accbc[L-1] = 0 (the rhsbc of

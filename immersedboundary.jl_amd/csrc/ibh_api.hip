@@ -86,6 +86,14 @@ struct ibh_ctx {
     std::vector<int> J;                      // unknowns of moving bodies
     std::vector<std::pair<int, int>> Jrange; // contiguous [start, count) runs of J
     bool setup_done = false;
+    // step graphs (per pointer parity) and the device parameter/status words
+    double *uv = nullptr;      // freestream (ux, uy) of the current step
+    int *status = nullptr;     // [CG done, CG iterations, error flags, pad]
+    bool use_graphs = true, capturing = false;
+    int parity = 0;
+    long steps_done = 0;
+    hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter = nullptr,
+                   gPost[2] = {nullptr, nullptr};
     Prof prof;
     std::string err;
     std::vector<void *> allocs;
@@ -158,6 +166,7 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) return it->second;
     if (c->tune_fixed >= 0) return c->tuned[key] = c->tune_fixed;
+    if (c->capturing) return 0;  // never time inside a graph capture (shapes are tuned before)
     const long ext = std::max((long)g0.M, (long)g0.N) * g0.ldc;
     const size_t need = (size_t)std::max(0L, (long)(g0.batch - 1)) * std::max(g0.sC, 0L) + ext;
     if (need > c->tune_cap) {
@@ -378,8 +387,11 @@ int check_flags(ibh_ctx *c) {
     return IBH_OK;
 }
 
+void drop_graphs(ibh_ctx *c);
+
 int ensure_B_work(ibh_ctx *c, int nc) {
     if (c->ncB >= nc) return IBH_OK;
+    drop_graphs(c);  // captured graphs hold the old work pointers
     const Dom &d = c->d;
     const long L = c->L;
     ALLOC(c->qB, (size_t)nc * d.nq);
@@ -493,86 +505,77 @@ int upload_points(ibh_ctx *c, const double *r, const double *v) {
     return IBH_OK;
 }
 
-// the force solve of BoundaryForcer (timestepping.jl:106-139): Fb = Binv·Ftmp or cg!
-int force_solve(ibh_ctx *c) {
-    const int nf = 2 * c->nb;
-    if (nf == 0) return IBH_OK;
-    if (!c->use_cg) {
-        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->cg.ws, nullptr, c->s);
-        return IBH_OK;
-    }
-    int rc;
-    if (c->use_pcg && (!c->pc_valid || c->steps_since_pc >= c->pc_refresh)) {
-        rc = refresh_pc(c);
-        if (rc) return rc;
-    }
-    rc = ensure_B_work(c, 1);
-    if (rc) return rc;
-    CGDev &cg = c->cg;
-    cg.P = c->use_pcg ? c->Pc : nullptr;
-    // r = b - B·x0 (warm start from the previous Fb)
-    B_times(c, cg.x, 0, cg.c, 0, 1, nullptr);
-    launch_cg_init(cg, c->s);
-    const int chunk = c->use_pcg ? 4 : 16;
-    int done = 0, it = 0;
-    while (!done) {
-        for (int k = 0; k < chunk; ++k) {
-            if (c->use_pcg) launch_pcg_dir(cg, c->s);
-            else launch_cg_dir(cg, c->s);
-            B_times(c, cg.u, 0, cg.c, 0, 1, cg.done);
-            launch_cg_update(cg, c->s);
-        }
-        HIPCK(hipMemcpyAsync(&done, cg.done, sizeof(int), hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipMemcpyAsync(&it, cg.it, sizeof(int), hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipStreamSynchronize(c->s));
-    }
-    c->cg_last = it;
-    if (c->use_pcg) {
-        ++c->steps_since_pc;
-        // a preconditioner that has drifted far from B is rebuilt at the next step
-        if (it > 24) c->steps_since_pc = c->pc_refresh;
-    }
-    return IBH_OK;
-}
+// ---------------------------------------------------------------------- the step
+// advance!(state, solver, t), cnab/timestepping.jl:253-290, as three segments that only
+// enqueue device work (no host synchronisation inside), so each can be captured in a HIP
+// graph: trial (+ start of the force solve), CG iterations, and post (stress, projection,
+// vort2flux).  The freestream is read from the device parameter block uv.
 
-// advance!(state, solver, t), cnab/timestepping.jl:253-290
-int step(ibh_ctx *c, double t, double ux, double uy) {
+// cycle!(nonlin) (timestepping.jl:81) and Γ = Γs (timestepping.jl:161) are pointer swaps
+inline void trial_swap(ibh_ctx *c) { std::swap(c->nl[0], c->nl[1]); }
+inline void post_swap(ibh_ctx *c) { std::swap(c->gam, c->gs); }
+
+void seg_trial(ibh_ctx *c) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
-    c->ux = ux;
-    c->uy = uy;
     // update!(reg) for moving bodies (timestepping.jl:262); points came from set_body_points
     if (c->anymove) launch_reg_update(d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
-
     // ---- GetTrialState (timestepping.jl:28-79), all levels batched
     if (L > 1) {
         launch_get_bc(d, c->gam + ng, ng, c->bcraw, d.nbc, L - 1, nullptr, c->s);
         launch_bc_prefix(d, c->bcraw, c->accbc, c->fac, L, c->s);
     }
     // bc of the coarsest level is zero; bcraw[L-1] stays zero
-    launch_nonlinear(d, c->q, d.nq, c->gam, ng, c->bcraw, d.nbc, 0.25, c->nl[0], ng, ux, uy, c->h,
-                     0, L, c->s);
+    launch_nonlinear(d, c->q, d.nq, c->gam, ng, c->bcraw, d.nbc, 0.25, c->nl[0], ng, 0.0, 0.0, c->h,
+                     0, L, c->s, 0, c->uv);
     RhsExtras rx{c->nl[0], c->nl[1], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
     lapinv(c, c->gam, ng, c->gs, ng, c->sA, ng, L, EPI_RHS, &rx, nullptr);
     lapinv(c, c->gs, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
-    std::swap(c->nl[0], c->nl[1]);  // cycle!(nonlin), timestepping.jl:81
+    trial_swap(c);
     vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr);
-
-    // ---- BoundaryForcer (timestepping.jl:106-139)
+    // ---- BoundaryForcer right-hand side (timestepping.jl:106-139): E(qs + q0) [- v·h]
     const int nf = 2 * c->nb;
     if (nf) {
-        launch_E(d, c->reg, c->qs, d.nq, ux * c->h, uy * c->h, c->Ftmp, nf, 1, nullptr, c->s);
+        launch_E(d, c->reg, c->qs, d.nq, 0.0, 0.0, c->Ftmp, nf, 1, nullptr, c->s, c->uv, c->h);
         if (c->anymove) launch_axpy_sub(nf, c->Ftmp, c->vel, c->h, c->s);
-        int rc = force_solve(c);
-        if (rc) return rc;
-        // update_stress! (streamfcn/states.jl:53-84)
+    }
+}
+
+// dense: Fb = Binv·Ftmp; CG: r = b − B·x0 with the previous Fb as warm start, then init
+void seg_solve_begin(ibh_ctx *c) {
+    const int nf = 2 * c->nb;
+    if (nf == 0) return;
+    if (!c->use_cg) {
+        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->cg.ws, nullptr, c->s);
+        return;
+    }
+    CGDev &cg = c->cg;
+    cg.P = c->use_pcg ? c->Pc : nullptr;
+    B_times(c, cg.x, 0, cg.c, 0, 1, nullptr);
+    launch_cg_init(cg, c->s);
+}
+
+void seg_cg_iters(ibh_ctx *c, int k) {
+    CGDev &cg = c->cg;
+    for (int i = 0; i < k; ++i) {
+        if (c->use_pcg) launch_pcg_dir(cg, c->s);
+        else launch_cg_dir(cg, c->s);
+        B_times(c, cg.u, 0, cg.c, 0, 1, cg.done);
+        launch_cg_update(cg, c->s);
+    }
+}
+
+void seg_post(ibh_ctx *c) {
+    const Dom &d = c->d;
+    const long ng = d.ng;
+    const int nf = 2 * c->nb;
+    // update_stress! (streamfcn/states.jl:53-84)
+    if (nf)
         launch_stress(c->nbodies, c->d_boff, c->d_npts, c->nb, c->Fb, c->ds, c->h, c->dt, c->CD,
                       c->CL, c->fbs, c->s);
-    }
-
     // ---- CircProjecter (timestepping.jl:156-171): Γ = Γs; Γ₁ -= Ainv₁ C' Eᵀ Fb
-    std::swap(c->gam, c->gs);
+    post_swap(c);
     if (nf) {
         launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->s);
         launch_rot(d, c->qtmp, d.nq, c->g1, ng, 1, nullptr, c->s);
@@ -580,7 +583,106 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     }
     // ---- vort2flux!(qty) (fluid-ops.jl:438)
     vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr);
+}
+
+// status word: [CG done, CG iterations, error flags]
+int read_status(ibh_ctx *c, int *st) {
+    HIPCK(hipMemcpyAsync(st, c->status, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCK(hipStreamSynchronize(c->s));
+    return IBH_OK;
+}
+
+int flag_error(ibh_ctx *c, int f) {
+    if (!f) return IBH_OK;
+    HIPCK(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
+    if (f & 1) return fail(c, IBH_EBOUNDS, "a body point's delta support leaves the finest grid");
+    if (f & 2) return fail(c, IBH_ENONFINITE, "infinite flux");
+    return fail(c, IBH_ESINGULAR, "singular B matrix");
+}
+
+// Run one segment: eagerly, or from its graph (captured on first use, then replayed).  The
+// host-side pointer swaps inside a segment are redone by `after` on replay.
+template <class F>
+int run_seg(ibh_ctx *c, hipGraphExec_t *slot, bool use_graph, F body, void (*after)(ibh_ctx *)) {
+    if (!use_graph) {
+        body();
+        return IBH_OK;
+    }
+    if (*slot) {
+        HIPCK(hipGraphLaunch(*slot, c->s));
+        if (after) after(c);
+        return IBH_OK;
+    }
+    HIPCK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    body();
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    HIPCK(hipStreamEndCapture(c->s, &g));
+    hipError_t e = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *slot = nullptr;
+        return fail(c, IBH_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    HIPCK(hipGraphLaunch(*slot, c->s));
+    return IBH_OK;
+}
+
+void drop_graphs(ibh_ctx *c) {
+    for (hipGraphExec_t *g : {&c->gTrial[0], &c->gTrial[1], &c->gIter, &c->gPost[0], &c->gPost[1]})
+        if (*g) {
+            hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
+int step(ibh_ctx *c, double t, double ux, double uy) {
+    c->ux = ux;
+    c->uy = uy;
+    launch_fill(c->uv, 1, ux, c->s);
+    launch_fill(c->uv + 1, 1, uy, c->s);
+    const bool refresh = c->use_pcg && (!c->pc_valid || c->steps_since_pc >= c->pc_refresh);
+    // graphs from the second step on (the first one runs the autotuner), not while profiling
+    // and not on steps that rebuild the preconditioner
+    const bool g = c->use_graphs && !c->prof.on && !refresh && c->steps_done >= 1;
+    const int par = c->parity;
+    const int k1 = c->use_pcg ? 6 : 16, k2 = c->use_pcg ? 2 : 16;
+    int rc;
+    if (refresh) {  // eager: the preconditioner needs this step's regularization
+        seg_trial(c);
+        if ((rc = refresh_pc(c))) return rc;
+        if ((rc = ensure_B_work(c, 1))) return rc;
+        seg_solve_begin(c);
+        if (c->use_cg) seg_cg_iters(c, k1);
+    } else {
+        if (c->use_cg && (rc = ensure_B_work(c, 1))) return rc;
+        rc = run_seg(c, &c->gTrial[par], g, [&] {
+            seg_trial(c);
+            seg_solve_begin(c);
+            if (c->use_cg) seg_cg_iters(c, k1);
+        }, trial_swap);
+        if (rc) return rc;
+    }
+    int st[3] = {0, 0, 0};
+    if (c->use_cg) {
+        if ((rc = read_status(c, st))) return rc;
+        while (!st[0]) {
+            if ((rc = run_seg(c, &c->gIter, g, [&] { seg_cg_iters(c, k2); }, nullptr))) return rc;
+            if ((rc = read_status(c, st))) return rc;
+        }
+        c->cg_last = st[1];
+        if (c->use_pcg) {
+            ++c->steps_since_pc;
+            // a preconditioner that has drifted far from B is rebuilt at the next step
+            if (st[1] > 24) c->steps_since_pc = c->pc_refresh;
+        }
+    }
+    if ((rc = run_seg(c, &c->gPost[par], g, [&] { seg_post(c); }, post_swap))) return rc;
+    c->parity ^= 1;
+    ++c->steps_done;
     c->t = t;
+    if (c->use_cg) return flag_error(c, st[2]);
     return IBH_OK;
 }
 
@@ -717,7 +819,10 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->ds, nb));
     CK(dalloc(c, &c->d_boff, D.nbodies + 1));
     CK(dalloc(c, &c->d_npts, D.nbodies + 1));
-    CK(dalloc(c, &c->derr, 4));
+    CK(dalloc(c, &c->status, 4));
+    c->derr = c->status + 2;
+    CK(dalloc(c, &c->uv, 2));
+    if (const char *ge = std::getenv("IBH_GRAPH")) c->use_graphs = std::atoi(ge) != 0;
     CK(dalloc(c, &c->reg.bidx, nf));
     CK(dalloc(c, &c->reg.w, (size_t)nb * 4 * 13));
     c->reg.nb = nb;
@@ -734,8 +839,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.z, nf));
     CK(dalloc(c, &c->cg.ws, gemv_ws_size(nf)));
     CK(dalloc(c, &c->cg.sc, 4));
-    CK(dalloc(c, &c->cg.it, 2));
-    CK(dalloc(c, &c->cg.done, 2));
+    c->cg.done = c->status;
+    c->cg.it = c->status + 1;
     c->cg.x = c->Fb;
     c->cg.b = c->Ftmp;
     c->cg.n = nf;
@@ -773,6 +878,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
 void ibh_destroy(ibh_ctx *c) {
     if (!c) return;
     if (c->s) hipStreamSynchronize(c->s);
+    drop_graphs(c);
     for (void *p : c->allocs) hipFree(p);
     if (c->tune_buf) hipFree(c->tune_buf);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
@@ -826,7 +932,7 @@ int ibh_advance(ibh_ctx *c, double t, double ux, double uy) {
     if (!c->setup_done) return fail(c, IBH_EINVAL, "ibh_setup must run before ibh_advance");
     int rc = step(c, t, ux, uy);
     if (rc) return rc;
-    if (c->anymove || c->use_cg) return check_flags(c);
+    if (c->anymove && !c->use_cg) return check_flags(c);
     return IBH_OK;
 }
 

@@ -87,7 +87,7 @@ void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out,
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
                       const double *gbc, long sbc, double gbc_scale, double *out, long sout,
                       double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
-                      int q0_given = 0);
+                      int q0_given = 0, const double *uv = nullptr);
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac_l·bcraw[l]  for ℓ = 0..L-2, accbc[L-1] = 0
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
                       hipStream_t s);
@@ -110,7 +110,8 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
                        double y0, hipStream_t s);
 // fb = E·(q + q0) where q0 = (u0, v0) constant (pass 0, 0 for plain E)
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
-              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s);
+              double *fb, long sfb, int nbatch, const int *skip, hipStream_t s,
+              const double *uv = nullptr, double uvscale = 1.0);
 // q = Eᵀ fb (q zeroed by the kernel pair)
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, hipStream_t s);

@@ -247,8 +247,36 @@ def test_split_tile_variants_identical(ibhip, monkeypatch):
     prob = cylinder_case(n=200, nlev=2)
     x = np.random.default_rng(13).standard_normal(199 * 199)
     outs = []
-    for v in range(6):
+    for v in range(12):
         monkeypatch.setenv("IBH_GEMM_VARIANT", str(v))
         outs.append(ibhip.DeviceCase(prob).lapinv(2, 0, x))
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("moving", [False, True])
+def test_graph_replay_matches_eager(ibhip, monkeypatch, moving):
+    # steps from the second on are captured in HIP graphs and replayed (two pointer
+    # parities); every replayed step is compared with an eager run of the same case
+    import ibhip as m
+
+    def make():
+        if moving:
+            return cylinder_plate_case(n=64, nlev=2, cyl_spacing=1.0, force_solver="pcg",
+                                       pc_refresh=4)
+        return cylinder_case(n=64, nlev=3)
+
+    runs = []
+    for graph in ("1", "0"):
+        monkeypatch.setenv("IBH_GRAPH", graph)
+        prob = make()
+        st = m.initstate(prob)
+        sv = m.initsolver(prob, st)
+        hist = []
+        for k in range(7):
+            m.advance_(st, sv, k * prob.scheme.dt)
+            hist.append((st.case.gamma(), st.case.q(), st.case.Fb(), st.case.CD()))
+        runs.append(hist)
+    for (g1, q1, f1, c1), (g2, q2, f2, c2) in zip(*runs):
+        assert rel(g1, g2) < 1e-12 and rel(q1, q2) < 1e-12
+        assert rel(f1, f2) < 1e-10 and rel(c1, c2) < 1e-10
