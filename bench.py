@@ -217,7 +217,8 @@ def main():
                    "body_points": prob.bodies.npoints, "precision": args.precision,
                    "finest_only_cell_steps_per_s": value / fl.nlevels,
                    "setup_s": t_setup, "force_solve": cg,
-                   "lift_drag_error_vs_fp64": accuracy},
+                   "lift_drag_error_vs_fp64": accuracy,
+                   "autotuned_contractions": case.tune_report()},
         "roofline": roofline,
         "cpu_baseline": None,
     }

@@ -110,6 +110,22 @@ __device__ inline void fload(Frag<PBF3> &f, const Tile<PBF3, ROWS, BK> &t, int r
     f.lo = *reinterpret_cast<const bf8 *>(&t.lo[row][kk * 32 + lq * 8]);
 }
 
+// ------------------------------------------------------------------------ XCD-aware order
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id mod 8).  Remap so that each
+// XCD walks a contiguous range of tiles (x fastest): tiles sharing a data or sine panel then
+// run on the same XCD and hit its L2.  Bijective for any grid size.
+__device__ inline void xcd_tile(int &x, int &y, int &z) {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int total = gx * gy * gridDim.z;
+    const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int q = total / 8, r = total % 8;
+    const int xcd = L % 8, slot = L / 8;
+    const int T = xcd * q + min(xcd, r) + slot;
+    x = T % gx;
+    y = (T / gx) % gy;
+    z = T / (gx * gy);
+}
+
 // ------------------------------------------------------------------------ kernel
 template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT>
 __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
@@ -122,12 +138,13 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
     static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "load split");
 
     if (g.skip && *g.skip) return;
-    const int bz = blockIdx.z;
+    int bxt, byt, bz;
+    xcd_tile(bxt, byt, bz);
     const double *__restrict__ A = g.A + bz * g.sA;  // A(m, k) at A[k + lda·m]
     const double *__restrict__ B = g.B + bz * g.sB;  // B(k, n) at B[k + ldb·n]
     double *__restrict__ C = g.C + bz * g.sC;
     const int M = g.M, N = g.N, K = g.K;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int m0 = bxt * BM, n0 = byt * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int l16 = lane & 15, lq = lane >> 4;
@@ -263,14 +280,15 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
     static_assert(LS * NT == RS * BK && LD * NT == RD * 2 * BK, "load split");
 
     if (g.skip && *g.skip) return;
-    const int bz = blockIdx.z;
+    int bxt, byt, bz;
+    xcd_tile(bxt, byt, bz);
     const int h = g.h;                       // pairs
     const int Kd = g.K;                      // data length along k (= n - 1)
     const double *__restrict__ X = SIDE == 0 ? g.B + bz * g.sB : g.A + bz * g.sA;
     const long ldx = SIDE == 0 ? g.ldb : g.lda;
     double *__restrict__ C = g.C + bz * g.sC;
     const int M = g.M, N = g.N;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int m0 = bxt * BM, n0 = byt * BN;
     const int r0s = SIDE == 0 ? m0 : n0, r0d = SIDE == 0 ? n0 : m0;
     const int nrd = SIDE == 0 ? N : M;       // valid data rows
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -501,6 +519,8 @@ static const VariantEO EF64[] = {
     EOV(PF64, 32, 64, 32, 2, 2, 2), EOV(PF64, 64, 32, 32, 2, 2, 2), EOV(PF64, 64, 64, 32, 2, 2, 2),
     EOV(PF64, 64, 64, 16, 2, 2, 1), EOV(PF64, 32, 64, 16, 2, 2, 1), EOV(PF64, 64, 32, 16, 2, 2, 1),
     EOV(PF64, 32, 32, 16, 2, 2, 1), EOV(PF64, 64, 64, 8, 2, 2, 1),  EOV(PF64, 32, 64, 8, 2, 2, 1),
+    EOV(PF64, 64, 64, 16, 2, 4, 1), EOV(PF64, 64, 64, 16, 4, 2, 1), EOV(PF64, 128, 64, 16, 4, 2, 1),
+    EOV(PF64, 64, 128, 16, 2, 4, 1),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),

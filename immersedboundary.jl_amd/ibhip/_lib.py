@@ -104,6 +104,7 @@ def load():
     L.ibh_op_B.argtypes = [vp, dp, dp]
     L.ibh_profile_enable.argtypes = [vp, C.c_int32]
     L.ibh_profile_read.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
+    L.ibh_tune_report.argtypes = [vp, C.c_char_p, C.c_int32]
     _lib = L
     return L
 

@@ -220,6 +220,12 @@ class DeviceCase:
         self._op("ibh_op_B", dptr(f64(z)), dptr(out))
         return out
 
+    def tune_report(self) -> list:
+        """The GEMM autotuner's choices so far, one line per contraction shape."""
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.lib.ibh_tune_report(self.h, buf, len(buf)))
+        return [ln for ln in buf.value.decode().splitlines() if ln]
+
     def profile(self, on: bool):
         self._check(self.lib.ibh_profile_enable(self.h, int(on)))
 
