@@ -6,10 +6,14 @@ pitching 30° ± 10° at ω = 1 on a 1024² finest grid with 5 levels, Re=200, d
 A "step" is one advance! of that case; cell-steps/s = nx·ny·L·steps / seconds.  Inputs live in
 HBM before the timed region.  Prints one JSON line (see the fields below).
 
-The dominant kernel is the FP64 sine-transform contraction (MFMA); its average launch time is
-measured with HIP events on the library's stream during the timed region.  The CPU baseline
-runs the FP64 oracle (a C restatement of the reference, OpenMP) on a bounded sample of the
-same workload.
+The dominant kernel is the sine-transform contraction (MFMA).  The timed region replays the
+captured step graphs.  Right after it, a few more steps of the same run are executed eagerly
+with HIP events recorded around every contraction launch on the library's stream, and
+`roofline` is computed from those events: algorithmic flops per launch divided by the average
+launch time.  `roofline.traffic` is the per-launch HBM byte count measured for this workload
+by rocprofv3 counter runs (profiles/contraction_traffic.json), or null when none was measured.
+The CPU baseline runs the FP64 oracle (a C restatement of the reference, OpenMP) on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -181,13 +185,20 @@ def main():
     avg_launch_s = (ms / 1e3) / max(launches, 1)
     achieved = (flops / max(launches, 1)) / avg_launch_s if launches else 0.0
     peak = PEAK["fp64"] if args.precision == "fp64" else PEAK[args.precision]
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "contraction_traffic.json")) as f:
+            traffic = json.load(f).get(f"{args.workload}/{args.precision}", {}).get(
+                "bytes_per_launch")
+    except OSError:
+        pass
     roofline = {
         "bound": "mfma",
         "achieved": achieved / 1e12,
         "peak": peak / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
-        "traffic": None,
+        "traffic": traffic,
         "kernel": f"k_gemm<{args.precision}> (sine-transform contraction)",
         "avg_launch_us": avg_launch_s * 1e6,
         "launches": launches,
