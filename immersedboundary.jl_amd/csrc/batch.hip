@@ -1,0 +1,635 @@
+// Batched CNAB steps: many independent cases of uneven grid size, level count and body-point
+// count advanced together.  Each substep of advance! (cnab/timestepping.jl:253-290) becomes
+// one grouped launch over every case that takes part in it: the trial state of all levels of
+// all cases (timestepping.jl:28-79), each level of the two Vort2Flux sweeps
+// (fluid-ops.jl:440-472, coarsest level first; cases with fewer levels join later), the
+// dense-inverse force solve (cnab/ops.jl:79-102, timestepping.jl:89-120), update_stress!
+// (streamfcn/states.jl:53-84) and the circulation projection (timestepping.jl:156-171).  The
+// grouped kernels run the single-case per-element bodies, so a case's arithmetic is the same
+// as when it steps alone.  The whole step is one HIP graph per pointer parity.
+//
+// Cases are contexts made by ibh_create / ibh_init_state / ibh_setup; the batch borrows them
+// (it never frees them) and keeps their host-side state (pointer swaps, t, step count) in
+// step with what it runs, so ibh_get_field on a case reads the batch's results.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ibh.h"
+#include "ctx.h"
+#include "ibh_internal.h"
+
+using namespace ibh;
+
+namespace {
+
+enum Op {
+    OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
+    OP_E, OP_ETZ, OP_ET, OP_GEMVP, OP_GEMVS, OP_STRESS
+};
+
+inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+struct Stage {
+    int op = 0, n = 0;
+    std::vector<char> args;    // n argument structs
+    std::vector<int> blocks;   // per problem (non-GEMM ops)
+    // contractions
+    std::vector<GemmArgs> gargs;
+    int side = 0, epi = 0, kind = 0;
+    int tune_ref = -1;         // index of the stage whose variant this one reuses
+    double flops = 0;
+    int variant = -1;
+    // device copies
+    const void *d_args = nullptr;
+    const int *d_off = nullptr;   // GEMM: one (n+1) offset row per variant
+    std::vector<int> total;       // workgroups (per variant for GEMM)
+};
+
+}  // namespace
+
+struct ibh_batch {
+    std::vector<ibh_ctx *> cs;
+    int prec = 0, Lmax = 1;
+    hipStream_t s = nullptr;
+    std::vector<Stage> st[2];
+    void *arena = nullptr;
+    double *uv = nullptr;
+    int parity = 0;
+    long steps = 0;
+    std::vector<double *> gam0, gs0, nl00, nl10;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool use_graphs = true, tuned = false;
+    std::vector<std::string> tune_log;
+    ibh::Prof prof;
+    std::string err;
+};
+
+namespace {
+
+int bfail(ibh_batch *b, int code, const std::string &m) {
+    if (b) b->err = m;
+    return code;
+}
+
+#define BCK(expr)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return bfail(b, IBH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_));  \
+    } while (0)
+
+template <class A>
+void add(std::vector<Stage> &v, int op, const std::vector<A> &a, const std::vector<int> &bl) {
+    if (a.empty()) return;
+    Stage S;
+    S.op = op;
+    S.n = (int)a.size();
+    S.args.resize(a.size() * sizeof(A));
+    std::memcpy(S.args.data(), a.data(), S.args.size());
+    S.blocks = bl;
+    v.push_back(std::move(S));
+}
+
+// the four contractions of one LaplacianInv apply per case, as four grouped stages; the
+// second left/right pair reuses the first pair's tuned tile variant (same shapes)
+void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans) {
+    if (plans.empty()) return;
+    const int base = (int)v.size();
+    for (int k = 0; k < 4; ++k) {
+        Stage S;
+        S.op = OP_GEMM;
+        S.kind = plans[0].kind[k];
+        S.side = S.kind - 1;
+        S.epi = plans[0].epi[k];
+        S.n = (int)plans.size();
+        for (const LapInvPlan &P : plans) {
+            const GemmArgs &g = P.g[k];
+            S.gargs.push_back(g);
+            const double rows = S.kind == 1 ? g.N : g.M;
+            S.flops += 4.0 * g.h * (double)g.h * rows * g.batch;
+        }
+        S.tune_ref = k >= 2 ? base + k - 2 : -1;
+        v.push_back(std::move(S));
+    }
+}
+
+// Vort2Flux (fluid-ops.jl:440-472) of every case: coarsify up, then the Poisson solves from
+// each case's coarsest level down; stage j of the descent is level L-1-j of a case.
+void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
+                   const std::vector<double *> &Psi, const std::vector<double *> &Q,
+                   bool curl_all) {
+    const int n = (int)b->cs.size();
+    for (int j = 1; j < b->Lmax; ++j) {
+        std::vector<CoarsA> a;
+        std::vector<int> bl;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            if (c->L <= j) continue;
+            const Dom &d = c->d;
+            a.push_back({d, G[i] + j * d.ng, G[i] + (j - 1) * d.ng, 0, 0, 1});
+            bl.push_back(nblk(grp_elems_coarsify(d), GRP_BS));
+        }
+        add(v, OP_COARS, a, bl);
+    }
+    {
+        std::vector<FillA> f;
+        std::vector<int> fb;
+        std::vector<LapInvPlan> plans;
+        std::vector<CurlA> cu;
+        std::vector<int> cb;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            const Dom &d = c->d;
+            const int L = c->L;
+            double *bcL = c->psibc + (long)(L - 1) * d.nbc;
+            f.push_back({bcL, d.nbc, 0.0});
+            fb.push_back(nblk(d.nbc, GRP_BS));
+            LapInvPlan P;
+            lapinv_plan(c, G[i] + (L - 1) * d.ng, d.ng, Psi[i] + (L - 1) * d.ng, d.ng, c->sD, 0, 1,
+                        EPI_STORE, nullptr, nullptr, &P);
+            plans.push_back(P);
+            if (curl_all || L == 1) {
+                cu.push_back({d, Psi[i] + (L - 1) * d.ng, 0, bcL, 0,
+                              Q[i] + (curl_all ? (L - 1) * d.nq : 0), 0, 1});
+                cb.push_back(nblk(d.nq, GRP_BS));
+            }
+        }
+        add(v, OP_FILL, f, fb);
+        add_lapinv(v, plans);
+        add(v, OP_CURL, cu, cb);
+    }
+    for (int j = 1; j < b->Lmax; ++j) {
+        std::vector<GetBcA> gb;
+        std::vector<CopyBcA> cp;
+        std::vector<int> b1, b2, cb;
+        std::vector<LapInvPlan> plans;
+        std::vector<CurlA> cu;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            if (c->L <= j) continue;
+            const Dom &d = c->d;
+            const int lev = c->L - 1 - j;
+            double *bcl = c->psibc + (long)lev * d.nbc;
+            gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
+            b1.push_back(nblk(d.nbc, GRP_BS));
+            cp.push_back({d, G[i] + lev * d.ng, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
+            b2.push_back(nblk(d.ng, GRP_BS));
+            LapInvPlan P;
+            lapinv_plan(c, c->gtmp, d.ng, Psi[i] + lev * d.ng, d.ng, c->sD, 0, 1, EPI_STORE,
+                        nullptr, nullptr, &P);
+            plans.push_back(P);
+            if (curl_all || lev == 0) {
+                cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
+                              0, 1});
+                cb.push_back(nblk(d.nq, GRP_BS));
+            }
+        }
+        add(v, OP_GETBC, gb, b1);
+        add(v, OP_COPYBC, cp, b2);
+        add_lapinv(v, plans);
+        add(v, OP_CURL, cu, cb);
+    }
+}
+
+// The stage list of one step at batch parity P (pointer set of the creation snapshot, swapped
+// when P = 1), in the order of advance!.
+void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
+    const int n = (int)b->cs.size();
+    std::vector<double *> gam(n), gs(n), n0(n), n1(n), psi(n), qs(n), q(n);
+    for (int i = 0; i < n; ++i) {
+        gam[i] = P ? b->gs0[i] : b->gam0[i];
+        gs[i] = P ? b->gam0[i] : b->gs0[i];
+        n0[i] = P ? b->nl10[i] : b->nl00[i];
+        n1[i] = P ? b->nl00[i] : b->nl10[i];
+        psi[i] = b->cs[i]->psi;
+        qs[i] = b->cs[i]->qs;
+        q[i] = b->cs[i]->q;
+    }
+    // ---- GetTrialState (timestepping.jl:28-79), all levels of all cases per launch
+    {
+        std::vector<GetBcA> gb;
+        std::vector<BcPrefA> pf;
+        std::vector<int> b1, b2;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            if (c->L < 2) continue;
+            const Dom &d = c->d;
+            gb.push_back({d, gam[i] + d.ng, d.ng, c->bcraw, d.nbc, c->L - 1});
+            b1.push_back(nblk(d.nbc, GRP_BS) * (c->L - 1));
+            pf.push_back({d.nbc, c->bcraw, c->accbc, c->fac, c->L});
+            b2.push_back(nblk(d.nbc, GRP_BS));
+        }
+        add(v, OP_GETBC, gb, b1);
+        add(v, OP_BCPREF, pf, b2);
+    }
+    {
+        std::vector<NonlinA> a;
+        std::vector<int> bl;
+        std::vector<LapInvPlan> pA, pI;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            const Dom &d = c->d;
+            a.push_back({d, c->q, d.nq, gam[i], d.ng, c->bcraw, d.nbc, 0.25, n0[i], d.ng, c->h, 0,
+                         b->uv, c->L});
+            bl.push_back(nblk(d.ng, GRP_BS) * c->L);
+            RhsExtras rx{n0[i], n1[i], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
+            LapInvPlan P1, P2;
+            lapinv_plan(c, gam[i], d.ng, gs[i], d.ng, c->sA, d.ng, c->L, EPI_RHS, &rx, nullptr, &P1);
+            lapinv_plan(c, gs[i], d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
+                        nullptr, &P2);
+            pA.push_back(P1);
+            pI.push_back(P2);
+        }
+        add(v, OP_NONLIN, a, bl);
+        add_lapinv(v, pA);
+        add_lapinv(v, pI);
+    }
+    add_vort2flux(b, v, gs, psi, qs, false);
+    // ---- BoundaryForcer (timestepping.jl:106-120): Fb = Binv·E(qs + q0)
+    {
+        std::vector<EA> e;
+        std::vector<GemvA> g;
+        std::vector<int> be, bp, bs;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            if (c->nb == 0) continue;
+            const int nf = 2 * c->nb;
+            e.push_back({c->d, c->reg, c->qs, c->Ftmp, b->uv, c->h});
+            be.push_back(nblk(nf, GRP_BS));
+            g.push_back({nf, c->Binv, c->Ftmp, c->cg.ws, c->Fb});
+            bp.push_back(nblk(nf, GRP_BS) * ((nf + GV_CHUNK - 1) / GV_CHUNK));
+            bs.push_back(nblk(nf, GRP_BS));
+        }
+        add(v, OP_E, e, be);
+        add(v, OP_GEMVP, g, bp);
+        add(v, OP_GEMVS, g, bs);
+    }
+    // ---- update_stress! and CircProjecter (timestepping.jl:156-171): Γ = Γs;
+    //      Γ₁ -= Ainv₁ C' Eᵀ Fb
+    {
+        std::vector<StressA> sa;
+        std::vector<ETA> et;
+        std::vector<RotA> ro;
+        std::vector<int> b1, b2, b3, b4;
+        std::vector<LapInvPlan> plans;
+        for (int i = 0; i < n; ++i) {
+            ibh_ctx *c = b->cs[i];
+            if (c->nb == 0) continue;
+            const Dom &d = c->d;
+            sa.push_back({c->nbodies, c->d_boff, c->d_npts, c->nb, c->Fb, c->ds, c->h, c->dt, c->CD,
+                          c->CL, c->fbs});
+            b1.push_back(c->nbodies);
+            et.push_back({d, c->reg, c->Fb, c->qtmp});
+            b2.push_back(nblk(d.nq, GRP_BS));
+            b3.push_back(nblk(2L * c->nb * GRP_NS * GRP_NS, GRP_BS));
+            ro.push_back({d, c->qtmp, d.nq, c->g1, d.ng, 1});
+            b4.push_back(nblk(d.ng, GRP_BS));
+            LapInvPlan P;
+            lapinv_plan(c, c->g1, d.ng, gs[i], d.ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr, &P);
+            plans.push_back(P);
+        }
+        add(v, OP_STRESS, sa, b1);
+        add(v, OP_ETZ, et, b2);
+        add(v, OP_ET, et, b3);
+        add(v, OP_ROT, ro, b4);
+        add_lapinv(v, plans);
+    }
+    add_vort2flux(b, v, gs, psi, q, true);
+}
+
+// copy every stage's tables into one device arena
+int upload(ibh_batch *b) {
+    size_t bytes = 0;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int nv = gemm_eo_grp_variant_count(b->prec);
+    for (int P = 0; P < 2; ++P)
+        for (Stage &S : b->st[P]) {
+            if (S.op == OP_GEMM) bytes += al(S.gargs.size() * sizeof(GemmArgs)) + al(sizeof(int) * (S.n + 1) * nv);
+            else bytes += al(S.args.size()) + al(sizeof(int) * (S.n + 1));
+        }
+    BCK(hipMalloc(&b->arena, bytes + 256));
+    std::vector<char> host(bytes + 256, 0);
+    size_t o = 0;
+    for (int P = 0; P < 2; ++P)
+        for (Stage &S : b->st[P]) {
+            char *base = static_cast<char *>(b->arena);
+            if (S.op == OP_GEMM) {
+                const size_t na = S.gargs.size() * sizeof(GemmArgs);
+                std::memcpy(host.data() + o, S.gargs.data(), na);
+                S.d_args = base + o;
+                o += al(na);
+                std::vector<int> off((size_t)(S.n + 1) * nv);
+                S.total.assign(nv, 0);
+                for (int v = 0; v < nv; ++v) {
+                    int acc = 0;
+                    for (int i = 0; i < S.n; ++i) {
+                        off[(size_t)v * (S.n + 1) + i] = acc;
+                        acc += (int)grouped_eo_tiles(S.gargs[i], b->prec, v, S.side);
+                    }
+                    off[(size_t)v * (S.n + 1) + S.n] = acc;
+                    S.total[v] = acc;
+                }
+                std::memcpy(host.data() + o, off.data(), off.size() * sizeof(int));
+                S.d_off = reinterpret_cast<const int *>(base + o);
+                o += al(off.size() * sizeof(int));
+            } else {
+                std::memcpy(host.data() + o, S.args.data(), S.args.size());
+                S.d_args = base + o;
+                o += al(S.args.size());
+                std::vector<int> off(S.n + 1);
+                int acc = 0;
+                for (int i = 0; i < S.n; ++i) {
+                    off[i] = acc;
+                    acc += S.blocks[i];
+                }
+                off[S.n] = acc;
+                S.total.assign(1, acc);
+                std::memcpy(host.data() + o, off.data(), off.size() * sizeof(int));
+                S.d_off = reinterpret_cast<const int *>(base + o);
+                o += al(off.size() * sizeof(int));
+            }
+        }
+    BCK(hipMemcpy(b->arena, host.data(), o, hipMemcpyHostToDevice));
+    return IBH_OK;
+}
+
+void launch_stage(ibh_batch *b, const Stage &S, int variant) {
+    Grp g{S.d_args, S.d_off, S.n, S.total.empty() ? 0 : S.total[0]};
+    hipStream_t s = b->s;
+    switch (S.op) {
+    case OP_GEMM:
+        g.off = S.d_off + (size_t)variant * (S.n + 1);
+        g.blocks = S.total[variant];
+        launch_gemm_eo_grp(g, b->prec, variant, S.side, S.epi, s);
+        break;
+    case OP_CURL: launch_grp_curl(g, s); break;
+    case OP_ROT: launch_grp_rot(g, s); break;
+    case OP_COARS: launch_grp_coarsify(g, s); break;
+    case OP_GETBC: launch_grp_get_bc(g, s); break;
+    case OP_COPYBC: launch_grp_copy_apply_bc(g, s); break;
+    case OP_NONLIN: launch_grp_nonlinear(g, s); break;
+    case OP_BCPREF: launch_grp_bc_prefix(g, s); break;
+    case OP_FILL: launch_grp_fill(g, s); break;
+    case OP_E: launch_grp_E(g, s); break;
+    case OP_ETZ: launch_grp_ET_zero(g, s); break;
+    case OP_ET: launch_grp_ET(g, s); break;
+    case OP_GEMVP: launch_grp_gemv_part(g, s); break;
+    case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
+    case OP_STRESS: launch_grp_stress(g, s); break;
+    }
+}
+
+// Time every tile variant of a contraction stage on its real operands (each grouped
+// contraction reads operands it does not write, so re-running it leaves the same result) and
+// keep the fastest.  Variants of one precision sum k in the same order: identical results.
+int tune_stage(ibh_batch *b, Stage &S) {
+    const int nv = gemm_eo_grp_variant_count(b->prec);
+    hipEvent_t e0, e1;
+    BCK(hipEventCreate(&e0));
+    BCK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < nv; ++v) {
+        launch_stage(b, S, v);
+        BCK(hipEventRecord(e0, b->s));
+        for (int r = 0; r < 3; ++r) launch_stage(b, S, v);
+        BCK(hipEventRecord(e1, b->s));
+        BCK(hipEventSynchronize(e1));
+        float ms = 0;
+        BCK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    S.variant = best;
+    char line[256];
+    std::snprintf(line, sizeof line, "%s %s problems=%d %.1fus", gemm_eo_grp_variant_name(b->prec, best),
+                  S.kind == 1 ? "left" : "right", S.n, best_ms / 3 * 1e3);
+    b->tune_log.push_back(line);
+    return IBH_OK;
+}
+
+int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
+    for (size_t k = 0; k < v.size(); ++k) {
+        Stage &S = v[k];
+        if (S.op == OP_GEMM && S.variant < 0) {
+            if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
+            else if (may_tune) {
+                int rc = tune_stage(b, S);
+                if (rc) return rc;
+            }
+            if (S.variant < 0) S.variant = 0;
+        }
+        if (S.op == OP_GEMM && b->prof.on) {
+            Prof &P = b->prof;
+            if (P.used + 2 > P.pool.size())
+                for (int k2 = 0; k2 < 64; ++k2) {
+                    hipEvent_t e;
+                    hipEventCreate(&e);
+                    P.pool.push_back(e);
+                }
+            hipEventRecord(P.pool[P.used], b->s);
+            launch_stage(b, S, S.variant);
+            hipEventRecord(P.pool[P.used + 1], b->s);
+            P.used += 2;
+            P.rec.push_back({b->prec, S.flops});
+        } else {
+            launch_stage(b, S, S.variant);
+        }
+    }
+    return IBH_OK;
+}
+
+int batch_step(ibh_batch *b, double ux, double uy) {
+    const int P = b->parity;
+    const int n = (int)b->cs.size();
+    for (int i = 0; i < n; ++i) {
+        ibh_ctx *c = b->cs[i];
+        if (c->gam != (P ? b->gs0[i] : b->gam0[i]) || c->nl[0] != (P ? b->nl10[i] : b->nl00[i]))
+            return bfail(b, IBH_EINVAL, "case " + std::to_string(i) +
+                                            " was stepped outside the batch since the batch was created");
+    }
+    launch_fill(b->uv, 1, ux, b->s);
+    launch_fill(b->uv + 1, 1, uy, b->s);
+    std::vector<Stage> &v = b->st[P];
+    const bool eager = !b->use_graphs || b->prof.on || !b->tuned;
+    int rc;
+    if (eager) {
+        if ((rc = run_stages(b, v, !b->tuned))) return rc;
+        if (!b->tuned) {
+            // both parities have the same shapes in the same order
+            for (size_t k = 0; k < v.size(); ++k) b->st[1 - P][k].variant = v[k].variant;
+            b->tuned = true;
+        }
+    } else {
+        if (!b->graph[P]) {
+            BCK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+            rc = run_stages(b, v, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->s, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return bfail(b, IBH_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph[P], g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph[P] = nullptr;
+                return bfail(b, IBH_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+        }
+        BCK(hipGraphLaunch(b->graph[P], b->s));
+    }
+    // the cases' host-side state follows what ran: cycle!(nonlin), Γ = Γs, t, step count
+    for (ibh_ctx *c : b->cs) {
+        std::swap(c->nl[0], c->nl[1]);
+        std::swap(c->gam, c->gs);
+        c->parity ^= 1;
+        c->t = c->steps_done == 0 ? c->t : c->t + c->dt;
+        ++c->steps_done;
+        c->ux = ux;
+        c->uy = uy;
+    }
+    b->parity ^= 1;
+    ++b->steps;
+    return IBH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ibh_batch_create(int32_t n, ibh_ctx *const *cases, ibh_batch **out) {
+    if (!out || !cases || n <= 0) return IBH_EINVAL;
+    *out = nullptr;
+    ibh_batch *b = new ibh_batch();
+    auto bad = [&](const std::string &m) {
+        std::string msg = m;
+        delete b;
+        // no batch to hold the message: report through the first case
+        if (cases[0]) cases[0]->err = msg;
+        return IBH_EINVAL;
+    };
+    for (int i = 0; i < n; ++i) {
+        ibh_ctx *c = cases[i];
+        if (!c) return bad("null case");
+        if (!c->setup_done) return bad("case " + std::to_string(i) + ": ibh_setup has not run");
+        if (c->use_cg || c->anymove)
+            return bad("case " + std::to_string(i) + ": batches take fixed bodies with the dense force solver");
+        if (!c->use_eo) return bad("case " + std::to_string(i) + ": batches need the split transforms");
+        if (c->precision != cases[0]->precision) return bad("cases differ in precision");
+        for (int j = 0; j < i; ++j)
+            if (cases[j] == c) return bad("a case appears twice");
+        b->cs.push_back(c);
+        b->Lmax = std::max(b->Lmax, c->L);
+        b->gam0.push_back(c->gam);
+        b->gs0.push_back(c->gs);
+        b->nl00.push_back(c->nl[0]);
+        b->nl10.push_back(c->nl[1]);
+    }
+    b->prec = cases[0]->precision;
+    if (const char *ge = std::getenv("IBH_GRAPH")) b->use_graphs = std::atoi(ge) != 0;
+    if (hipDeviceSynchronize() != hipSuccess || hipStreamCreate(&b->s) != hipSuccess ||
+        hipMalloc(&b->uv, 2 * sizeof(double)) != hipSuccess) {
+        ibh_batch_destroy(b);
+        return IBH_EDEVICE;
+    }
+    build_step(b, 0, b->st[0]);
+    build_step(b, 1, b->st[1]);
+    int rc = upload(b);
+    if (rc) {
+        ibh_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return IBH_OK;
+}
+
+void ibh_batch_destroy(ibh_batch *b) {
+    if (!b) return;
+    if (b->s) hipStreamSynchronize(b->s);
+    for (hipGraphExec_t &g : b->graph)
+        if (g) hipGraphExecDestroy(g);
+    if (b->arena) hipFree(b->arena);
+    if (b->uv) hipFree(b->uv);
+    for (hipEvent_t e : b->prof.pool) hipEventDestroy(e);
+    if (b->s) hipStreamDestroy(b->s);
+    delete b;
+}
+
+const char *ibh_batch_last_error(const ibh_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int ibh_batch_advance(ibh_batch *b, double ux, double uy) {
+    if (!b) return IBH_EINVAL;
+    return batch_step(b, ux, uy);
+}
+
+int ibh_batch_advance_n(ibh_batch *b, int32_t nsteps, double ux, double uy) {
+    if (!b) return IBH_EINVAL;
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = batch_step(b, ux, uy);
+        if (rc) return rc;
+    }
+    return IBH_OK;
+}
+
+int ibh_batch_sync(ibh_batch *b) {
+    if (!b) return IBH_EINVAL;
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipGetLastError());
+    for (size_t i = 0; i < b->cs.size(); ++i) {
+        int f = 0;
+        BCK(hipMemcpy(&f, b->cs[i]->derr, sizeof(int), hipMemcpyDeviceToHost));
+        if (f) {
+            BCK(hipMemset(b->cs[i]->derr, 0, sizeof(int)));
+            return bfail(b, (f & 2) ? IBH_ENONFINITE : IBH_EBOUNDS,
+                         "case " + std::to_string(i) + ((f & 2) ? ": infinite flux" : ": device error flag"));
+        }
+    }
+    return IBH_OK;
+}
+
+int ibh_batch_profile_enable(ibh_batch *b, int32_t on) {
+    if (!b) return IBH_EINVAL;
+    BCK(hipStreamSynchronize(b->s));
+    b->prof.on = on != 0;
+    b->prof.used = 0;
+    b->prof.rec.clear();
+    return IBH_OK;
+}
+
+int ibh_batch_profile_read(ibh_batch *b, double *ms, int64_t *launches, double *flops) {
+    if (!b) return IBH_EINVAL;
+    BCK(hipStreamSynchronize(b->s));
+    double tot = 0, fl = 0;
+    for (size_t k = 0; k < b->prof.rec.size(); ++k) {
+        float e = 0;
+        BCK(hipEventElapsedTime(&e, b->prof.pool[2 * k], b->prof.pool[2 * k + 1]));
+        tot += e;
+        fl += b->prof.rec[k].second;
+    }
+    if (ms) *ms = tot;
+    if (launches) *launches = (int64_t)b->prof.rec.size();
+    if (flops) *flops = fl;
+    return IBH_OK;
+}
+
+int ibh_batch_tune_report(ibh_batch *b, char *buf, int32_t len) {
+    if (!b || !buf || len <= 0) return IBH_EINVAL;
+    std::string r;
+    for (auto &l : b->tune_log) r += l + "\n";
+    std::snprintf(buf, (size_t)len, "%s", r.c_str());
+    return IBH_OK;
+}
+
+int ibh_batch_launches_per_step(ibh_batch *b) {
+    return b ? (int)b->st[0].size() : -1;
+}
+
+}  // extern "C"

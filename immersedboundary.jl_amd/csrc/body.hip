@@ -7,7 +7,9 @@ namespace ibh {
 
 static constexpr int SUPP = 6, NS = 13;
 
-static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+__host__ __device__ static inline unsigned nblk(long n, int bs) {
+    return (unsigned)((n + bs - 1) / bs);
+}
 
 // δh, structure-ops.jl:114-171 — Yang et al. (JCP 2009) smoothed delta times h; the
 // coefficients are the reference's literals.
@@ -75,18 +77,8 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
 
 // E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w, summed column-major like the
 // reference's sumprod; q0 (constant per component) is added to q first, as in Q = qs + q0.
-__global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
-                    double *__restrict__ fb, long sfb, const int *skip, const double *uv,
-                    double uvscale) {
-    if (skip && *skip) return;
-    if (uv) {  // q0 = freestream·h from the device parameter block
-        u0 = uv[0] * uvscale;
-        v0 = uv[1] * uvscale;
-    }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * r.nb) return;
-    const int b = blockIdx.y;
-    q += b * sq;
+__device__ inline double E_at(const Dom &d, const RegDev &r, const double *__restrict__ q,
+                              double u0, double v0, int t) {
     const int comp = t / r.nb, k = t % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
     const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
@@ -105,7 +97,21 @@ __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, doub
             for (int s = 0; s < NS; ++s) acc += (col[bx + s - SUPP] + v0) * (wx[s] * wy[tt]);
         }
     }
-    fb[b * sfb + t] = acc;
+    return acc;
+}
+
+__global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
+                    double *__restrict__ fb, long sfb, const int *skip, const double *uv,
+                    double uvscale) {
+    if (skip && *skip) return;
+    if (uv) {  // q0 = freestream·h from the device parameter block
+        u0 = uv[0] * uvscale;
+        v0 = uv[1] * uvscale;
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * r.nb) return;
+    const int b = blockIdx.y;
+    fb[b * sfb + t] = E_at(d, r, q + b * sq, u0, v0, t);
 }
 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
@@ -124,14 +130,8 @@ __global__ void k_zero(double *__restrict__ p, long n, long sp, const int *skip)
     if (t < n) p[blockIdx.y * sp + t] = 0.0;
 }
 
-__global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
-                     double *__restrict__ q, long sq, const int *skip) {
-    if (skip && *skip) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * r.nb * NS * NS) return;
-    const int b = blockIdx.y;
-    q += b * sq;
-    fb += b * sfb;
+__device__ inline void ET_at(const Dom &d, const RegDev &r, const double *__restrict__ fb,
+                             double *__restrict__ q, int t) {
     const int s = t % NS, tt = (t / NS) % NS, kc = t / (NS * NS);
     const int comp = kc / r.nb, k = kc % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
@@ -143,6 +143,15 @@ __global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
                             : q + d.nu + (i - 1) + (long)d.nx * (j - 1);
     if (comp == 0 && !isfinite(v)) atomicOr(r.err, 2);
     atomicAdd(dst, v);
+}
+
+__global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
+                     double *__restrict__ q, long sq, const int *skip) {
+    if (skip && *skip) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * r.nb * NS * NS) return;
+    const int b = blockIdx.y;
+    ET_at(d, r, fb + b * sfb, q + b * sq, t);
 }
 
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
@@ -160,15 +169,28 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
 // order (deterministic).  ws holds ceil(n/GV_CH)·n doubles.
 static constexpr int GV_CH = 64;
 
+__device__ inline void gemv_part_at(int n, const double *__restrict__ A,
+                                    const double *__restrict__ x, double *__restrict__ ws, int i,
+                                    int chunk) {
+    const int j0 = chunk * GV_CH, j1 = min(n, j0 + GV_CH);
+    double s = 0.0;
+    for (int j = j0; j < j1; ++j) s += A[i + (long)n * j] * x[j];
+    ws[(long)chunk * n + i] = s;
+}
+
+__device__ inline void gemv_sum_at(int n, int nch, const double *__restrict__ ws,
+                                   double *__restrict__ y, int i) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += ws[(long)c * n + i];
+    y[i] = s;
+}
+
 __global__ void k_gemv_part(int n, const double *__restrict__ A, const double *__restrict__ x,
                             double *__restrict__ ws, const int *skip) {
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j0 = blockIdx.y * GV_CH, j1 = min(n, j0 + GV_CH);
     if (i >= n) return;
-    double s = 0.0;
-    for (int j = j0; j < j1; ++j) s += A[i + (long)n * j] * x[j];
-    ws[(long)blockIdx.y * n + i] = s;
+    gemv_part_at(n, A, x, ws, i, blockIdx.y);
 }
 
 __global__ void k_gemv_sum(int n, int nch, const double *__restrict__ ws, double *__restrict__ y,
@@ -176,9 +198,7 @@ __global__ void k_gemv_sum(int n, int nch, const double *__restrict__ ws, double
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double s = 0.0;
-    for (int c = 0; c < nch; ++c) s += ws[(long)c * n + i];
-    y[i] = s;
+    gemv_sum_at(n, nch, ws, y, i);
 }
 
 long gemv_ws_size(int n) { return (long)((n + GV_CH - 1) / GV_CH) * n; }
@@ -221,11 +241,11 @@ __device__ double block_sum(double v, double *sh) {
 
 // update_stress!, streamfcn/states.jl:53-84: per body fb = F·h/dt, CD = 2Σfb_x, CL = 2Σfb_y,
 // then fb ./= ds.  One block per body.
-__global__ void k_stress(const int *boff, const int *npts, int nb, const double *__restrict__ Fb,
-                         const double *__restrict__ ds, double h, double dt, double *CD,
-                         double *CL, double *fbs) {
+__device__ inline void stress_body(const int *boff, const int *npts, int nb,
+                                   const double *__restrict__ Fb, const double *__restrict__ ds,
+                                   double h, double dt, double *CD, double *CL, double *fbs,
+                                   int b) {
     __shared__ double sh[8];
-    const int b = blockIdx.x;
     const int o = boff[b], n = npts[b];
     double sx = 0.0, sy = 0.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -241,6 +261,12 @@ __global__ void k_stress(const int *boff, const int *npts, int nb, const double 
         CD[b] = 2 * sx;
         CL[b] = 2 * sy;
     }
+}
+
+__global__ void k_stress(const int *boff, const int *npts, int nb, const double *__restrict__ Fb,
+                         const double *__restrict__ ds, double h, double dt, double *CD,
+                         double *CL, double *fbs) {
+    stress_body(boff, npts, nb, Fb, ds, h, dt, CD, CL, fbs, blockIdx.x);
 }
 
 void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
@@ -451,5 +477,70 @@ void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err
     hipLaunchKernelGGL(k_gj_col, dim3(nblk(n, 256)), dim3(256), 0, s, n, A, k, ws);
     hipLaunchKernelGGL(k_gj_elim, dim3(nblk(2L * n * n, 256)), dim3(256), 0, s, n, A, Inv, k, ws);
 }
+
+// ------------------------------------------------------------------------ grouped launches
+// E, Eᵀ, the dense-inverse GEMV and the stress sums over a list of cases (one problem per
+// case; see Grp).  Same per-element bodies as the single-case kernels.
+static_assert(GRP_NS == NS && GV_CHUNK == GV_CH, "grouped constants");
+
+__global__ __launch_bounds__(GRP_BS) void kg_E(const EA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const EA A = args[p];
+    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t >= 2 * A.r.nb) return;
+    A.fb[t] = E_at(A.d, A.r, A.q, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_ET_zero(const ETA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const ETA A = args[p];
+    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t < A.d.nq) A.q[t] = 0.0;
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_ET(const ETA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const ETA A = args[p];
+    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t >= 2 * A.r.nb * NS * NS) return;
+    ET_at(A.d, A.r, A.fb, A.q, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_gemv_part(const GemvA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const GemvA A = args[p];
+    const int lb = blockIdx.x - off[p], nbx = (int)nblk(A.n, GRP_BS);
+    const int i = (lb % nbx) * GRP_BS + threadIdx.x;
+    if (i >= A.n) return;
+    gemv_part_at(A.n, A.A, A.x, A.ws, i, lb / nbx);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_gemv_sum(const GemvA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const GemvA A = args[p];
+    const int i = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (i >= A.n) return;
+    gemv_sum_at(A.n, (A.n + GV_CH - 1) / GV_CH, A.ws, A.y, i);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const StressA A = args[p];
+    stress_body(A.boff, A.npts, A.nb, A.Fb, A.ds, A.h, A.dt, A.CD, A.CL, A.fbs,
+                blockIdx.x - off[p]);
+}
+
+#define GRP_LAUNCH(NAME, KERNEL, TYPE)                                                       \
+    void NAME(const Grp &g, hipStream_t s) {                                                 \
+        if (g.n == 0 || g.blocks == 0) return;                                               \
+        hipLaunchKernelGGL(KERNEL, dim3(g.blocks), dim3(GRP_BS), 0, s,                       \
+                           static_cast<const TYPE *>(g.args), g.off, g.n);                   \
+    }
+GRP_LAUNCH(launch_grp_E, kg_E, EA)
+GRP_LAUNCH(launch_grp_ET_zero, kg_ET_zero, ETA)
+GRP_LAUNCH(launch_grp_ET, kg_ET, ETA)
+GRP_LAUNCH(launch_grp_gemv_part, kg_gemv_part, GemvA)
+GRP_LAUNCH(launch_grp_gemv_sum, kg_gemv_sum, GemvA)
+GRP_LAUNCH(launch_grp_stress, kg_stress, StressA)
 
 }  // namespace ibh

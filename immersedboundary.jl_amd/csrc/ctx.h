@@ -1,0 +1,123 @@
+// The context of one case (device state, work buffers, step graphs, tuner), shared by the
+// translation units that orchestrate single cases and batches.
+#pragma once
+#include <map>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../../include/ibh.h"
+#include "ibh_internal.h"
+
+namespace ibh {
+
+// kind: 0 full contraction, 1 split left (C = S·X), 2 split right (C = X·S)
+struct TuneKey {
+    int prec, kind, M, N, K, batch;
+    bool tout;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(prec, kind, M, N, K, batch, tout) <
+               std::tie(o.prec, o.kind, o.M, o.N, o.K, o.batch, o.tout);
+    }
+};
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, double>> rec;  // (kind, flops) per recorded launch
+};
+
+}  // namespace ibh
+
+using ibh::Dom;
+using ibh::RegDev;
+using ibh::CGDev;
+
+struct ibh_ctx {
+    Dom d;
+    int L = 1;
+    double h = 0, x0 = 0, y0 = 0, Re = 0, dt = 0, beta[2] = {1.5, -0.5};
+    int nbodies = 0, nb = 0;
+    std::vector<int> npts, moving, boff;
+    bool anymove = false, use_cg = false;
+    int precision = 0, force_solver = 0, cg_maxiter = 5000;
+    double cg_reltol = 1e-12;
+    hipStream_t s = nullptr;
+
+    // transforms
+    double *Sm = nullptr, *Sp = nullptr;
+    // odd/even parts of the sine matrices for the split contraction (x and y dimensions)
+    double *Som = nullptr, *Sem = nullptr, *Sop = nullptr, *Sep = nullptr;
+    bool use_eo = true;
+    double *sD = nullptr, *sA = nullptr, *sAinv = nullptr;
+    // state (StreamFcnQuantities + CNAB.State.nonlin)
+    double *gam = nullptr, *psi = nullptr, *q = nullptr, *nl[2] = {nullptr, nullptr};
+    double ux = 0, uy = 0, t = 0;
+    // solver work
+    double *qs = nullptr, *gs = nullptr, *bcraw = nullptr, *accbc = nullptr, *psibc = nullptr;
+    double *gtmp = nullptr, *fac = nullptr, *qtmp = nullptr, *g1 = nullptr;
+    double *T1 = nullptr, *T2 = nullptr;
+    long Tcap = 0;  // elements in T1/T2
+    // B-product work (batched over nc columns)
+    int ncB = 0;
+    double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,
+           *bcB = nullptr, *g1B = nullptr;
+    // bodies
+    double *pts = nullptr, *vel = nullptr, *ds = nullptr;
+    int *d_boff = nullptr, *d_npts = nullptr;
+    RegDev reg{};
+    int *derr = nullptr;       // device error flags (bit 1 bounds, 2 non-finite, 4 singular)
+    double *Fb = nullptr, *Ftmp = nullptr, *CD = nullptr, *CL = nullptr, *fbs = nullptr;
+    double *Binv = nullptr;
+    CGDev cg{};
+    int cg_last = 0;
+    // PCG preconditioner: dense inverse of the assembled B (moving columns refreshed)
+    bool use_pcg = false, pc_valid = false;
+    int pc_refresh = 50, steps_since_pc = 0;
+    double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
+    int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
+    std::vector<int> J;                      // unknowns of moving bodies
+    std::vector<std::pair<int, int>> Jrange; // contiguous [start, count) runs of J
+    bool setup_done = false;
+    // step graphs (per pointer parity) and the device parameter/status words
+    double *uv = nullptr;      // freestream (ux, uy) of the current step
+    int *status = nullptr;     // [CG done, CG iterations, error flags, pad]
+    bool use_graphs = true, capturing = false;
+    int parity = 0;
+    long steps_done = 0;
+    hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter = nullptr,
+                   gPost[2] = {nullptr, nullptr};
+    ibh::Prof prof;
+    std::string err;
+    std::vector<void *> allocs;
+    // GEMM autotuner: chosen variant per shape, scratch output, log of the choices
+    std::map<ibh::TuneKey, int> tuned;
+    int tune_fixed = -1;
+    double *tune_buf = nullptr;
+    size_t tune_cap = 0;
+    std::vector<std::string> tune_log;
+};
+
+
+namespace ibh {
+
+// EPI_RHS extras of the last contraction of a CN solve
+struct RhsExtras {
+    const double *N1, *N2, *bcacc;
+    double f1, f2;
+};
+
+// The four split contractions of one LaplacianInv apply (fluid-ops.jl:42-56) on nbatch fields:
+// kind 1 = left (S·X, transposed store), kind 2 = right (X·S, epilogue epi[k]).
+struct LapInvPlan {
+    GemmArgs g[4];
+    int epi[4], kind[4];
+};
+
+void lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long sout,
+                 const double *scale, long sscale, int nbatch, int epi, const RhsExtras *rx,
+                 const int *skip, LapInvPlan *plan);
+
+}  // namespace ibh

@@ -267,8 +267,8 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
 // X as the B operand); SIDE 1: C = X·S (pairs on the N index, data as the A operand).  The
 // data operand is read as 2·BK contiguous k values and split by parity on the LDS store; the
 // butterfly runs in the epilogue.
-template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB = 2>
-__global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB>
+__device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int bz) {
     constexpr int NT = WM * WN * 64;
     constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
@@ -279,9 +279,6 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
     constexpr int LS = RS * BK / NT, LD = RD * 2 * BK / NT;
     static_assert(LS * NT == RS * BK && LD * NT == RD * 2 * BK, "load split");
 
-    if (g.skip && *g.skip) return;
-    int bxt, byt, bz;
-    xcd_tile(bxt, byt, bz);
     const int h = g.h;                       // pairs
     const int Kd = g.K;                      // data length along k (= n - 1)
     const double *__restrict__ X = SIDE == 0 ? g.B + bz * g.sB : g.A + bz * g.sA;
@@ -440,6 +437,32 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
             }
 }
 
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB = 2>
+__global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
+    if (g.skip && *g.skip) return;
+    int bxt, byt, bz;
+    xcd_tile(bxt, byt, bz);
+    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB>(g, bxt, byt, bz);
+}
+
+// Grouped form: one launch over the contractions of many cases (each its own sizes, sine
+// matrices and buffers).  The linear workgroup id is remapped XCD-contiguously over the whole
+// grid, then located in the tile-offset table; the tile walks x fastest inside its problem.
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB>
+__global__ __launch_bounds__(WM *WN * 64) void kg_gemm_eo(const GemmArgs *gs, const int *off,
+                                                          int n) {
+    const int total = gridDim.x, q = total / 8, r = total % 8;
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int T = xcd * q + min(xcd, r) + slot;
+    const int p = grp_find(off, n, T);
+    const GemmArgs g = gs[p];
+    const int local = T - off[p];
+    const int rows = SIDE == 0 ? g.h : g.M, cols = SIDE == 0 ? g.N : g.h;
+    const int gx = (rows + BM - 1) / BM, gy = (cols + BN - 1) / BN;
+    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB>(g, local % gx, (local / gx) % gy,
+                                                        local / (gx * gy));
+}
+
 // ------------------------------------------------------------------------ variant tables
 typedef void (*KFn)(GemmArgs);
 struct Variant {
@@ -559,6 +582,72 @@ void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi,
     const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
     dim3 grid((gm + V.bm - 1) / V.bm, (gn + V.bn - 1) / V.bn, g.batch);
     hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], grid, dim3(V.nthreads), 0, s, g);
+}
+
+// grouped split variants: [side][epi]
+typedef void (*KGFn)(const GemmArgs *, const int *, int);
+struct VariantGEO {
+    int bm, bn, nthreads;
+    KGFn fn[2][4];
+    const char *name;
+};
+#define GEOV(P, BM, BN, BK, WM, WN, NB)                                                       \
+    {BM, BN, WM * WN * 64,                                                                    \
+     {{kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB>, nullptr, nullptr, nullptr}, \
+      {kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB>,                            \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB>,                            \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB>,                          \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB>}},                            \
+     "grouped eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB}
+
+static const VariantGEO GF64[] = {
+    GEOV(PF64, 64, 64, 16, 2, 2, 1), GEOV(PF64, 32, 64, 16, 2, 2, 1), GEOV(PF64, 64, 32, 16, 2, 2, 1),
+    GEOV(PF64, 32, 64, 32, 2, 2, 2), GEOV(PF64, 64, 32, 32, 2, 2, 2), GEOV(PF64, 64, 64, 8, 2, 2, 1),
+    GEOV(PF64, 32, 32, 16, 2, 2, 1),
+};
+static const VariantGEO GF32[] = {
+    GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),
+    GEOV(PF32, 64, 32, 16, 2, 2, 1),
+};
+static const VariantGEO GBF3[] = {
+    GEOV(PBF3, 64, 64, 32, 2, 2, 1), GEOV(PBF3, 128, 64, 32, 2, 2, 1), GEOV(PBF3, 64, 128, 32, 2, 2, 1),
+    GEOV(PBF3, 64, 64, 32, 2, 2, 2),
+};
+
+static const VariantGEO *table_geo(int prec, int *n) {
+    if (prec == 1) { *n = sizeof(GF32) / sizeof(GF32[0]); return GF32; }
+    if (prec == 2) { *n = sizeof(GBF3) / sizeof(GBF3[0]); return GBF3; }
+    *n = sizeof(GF64) / sizeof(GF64[0]);
+    return GF64;
+}
+
+int gemm_eo_grp_variant_count(int prec) {
+    int n;
+    table_geo(prec, &n);
+    return n;
+}
+
+const char *gemm_eo_grp_variant_name(int prec, int v) {
+    int n;
+    const VariantGEO *t = table_geo(prec, &n);
+    return (v >= 0 && v < n) ? t[v].name : "?";
+}
+
+long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side) {
+    int n;
+    const VariantGEO *t = table_geo(prec, &n);
+    const VariantGEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
+    const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
+    return (long)((gm + V.bm - 1) / V.bm) * ((gn + V.bn - 1) / V.bn) * g.batch;
+}
+
+void launch_gemm_eo_grp(const Grp &gr, int prec, int variant, int side, int epi, hipStream_t s) {
+    if (gr.n == 0 || gr.blocks == 0) return;
+    int n;
+    const VariantGEO *t = table_geo(prec, &n);
+    const VariantGEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
+    hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], dim3(gr.blocks), dim3(V.nthreads), 0, s,
+                       static_cast<const GemmArgs *>(gr.args), gr.off, gr.n);
 }
 
 }  // namespace ibh

@@ -14,19 +14,13 @@ namespace ibh {
 #define QY(a, i, j) (a)[d.nu + ((i)-1) + (long)d.nx * ((j)-1)]
 #define BC(a, k) (a)[(k)-1]
 
-static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+__host__ __device__ static inline unsigned nblk(long n, int bs) {
+    return (unsigned)((n + bs - 1) / bs);
+}
 
 // curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182 — each q entry written once.
-__global__ void k_curl_bc(Dom d, const double *__restrict__ psi, long spsi,
-                          const double *__restrict__ pbc, long sbc, double *__restrict__ q,
-                          long sq, const int *skip) {
-    SKIP_GUARD
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= d.nq) return;
-    const int b = blockIdx.y;
-    psi += b * spsi;
-    pbc += b * sbc;
-    q += b * sq;
+__device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
+                                 const double *__restrict__ pbc, long t) {
     const int nx = d.nx, ny = d.ny;
     double v;
     if (t < d.nu) {
@@ -45,7 +39,17 @@ __global__ void k_curl_bc(Dom d, const double *__restrict__ psi, long spsi,
         else if (i == nx) v = GI(psi, i - 1, j - 1) - BC(pbc, d.Ro + j);
         else v = GI(psi, i - 1, j - 1) - GI(psi, i, j - 1);
     }
-    q[t] = v;
+    return v;
+}
+
+__global__ void k_curl_bc(Dom d, const double *__restrict__ psi, long spsi,
+                          const double *__restrict__ pbc, long sbc, double *__restrict__ q,
+                          long sq, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.nq) return;
+    const int b = blockIdx.y;
+    q[b * sq + t] = curl_at(d, psi + b * spsi, pbc + b * sbc, t);
 }
 
 void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
@@ -55,16 +59,18 @@ void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pb
 }
 
 // rot! (C'), fluid-ops.jl:117-129
+__device__ inline double rot_at(const Dom &d, const double *__restrict__ q, long t) {
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;  // Γ[i-1, j-1]
+    return (QX(q, i, j - 1) - QX(q, i, j)) + (QY(q, i, j) - QY(q, i - 1, j));
+}
+
 __global__ void k_rot(Dom d, const double *__restrict__ q, long sq, double *__restrict__ g,
                       long sg, const int *skip) {
     SKIP_GUARD
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    q += b * sq;
-    g += b * sg;
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;  // Γ[i-1, j-1]
-    g[t] = (QX(q, i, j - 1) - QX(q, i, j)) + (QY(q, i, j) - QY(q, i - 1, j));
+    g[b * sg + t] = rot_at(d, q + b * sq, t);
 }
 
 void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
@@ -75,15 +81,9 @@ void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int 
 
 // coarsify!(Γc, Γ), fluid-ops.jl:184-202: fine (i, j) = (2a + 2, 2b + 2) → coarse
 // (nx/4 + 1 + a, ny/4 + 1 + b), a in 0..nx/2-3, b in 0..ny/2-3.
-__global__ void k_coarsify(Dom d, double *__restrict__ gc, const double *__restrict__ g,
-                           long sgc, long sg, const int *skip) {
-    SKIP_GUARD
-    const int na = d.nx / 2 - 1, nbb = d.ny / 2 - 1;
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= (long)na * nbb) return;
-    const int bz = blockIdx.y;
-    gc += bz * sgc;
-    g += bz * sg;
+__device__ inline void coarsify_at(const Dom &d, double *__restrict__ gc,
+                                   const double *__restrict__ g, long t) {
+    const int na = d.nx / 2 - 1;
     const int a = (int)(t % na), b = (int)(t / na);
     const int i = 2 + 2 * a, j = 2 + 2 * b;
     const int ic = d.nx / 4 + 1 + a, jc = d.ny / 4 + 1 + b;
@@ -91,6 +91,15 @@ __global__ void k_coarsify(Dom d, double *__restrict__ gc, const double *__restr
                      0.5 * (GI(g, i + 1, j) + GI(g, i, j + 1) + GI(g, i - 1, j) + GI(g, i, j - 1)) +
                      0.25 * (GI(g, i + 1, j + 1) + GI(g, i + 1, j - 1) + GI(g, i - 1, j - 1) +
                              GI(g, i - 1, j + 1));
+}
+
+__global__ void k_coarsify(Dom d, double *__restrict__ gc, const double *__restrict__ g,
+                           long sgc, long sg, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)(d.nx / 2 - 1) * (d.ny / 2 - 1)) return;
+    const int bz = blockIdx.y;
+    coarsify_at(d, gc + bz * sgc, g + bz * sg, t);
 }
 
 void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
@@ -101,14 +110,7 @@ void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long s
 }
 
 // get_bc!(rbc, r), fluid-ops.jl:210-239, including the trailing ×0.25
-__global__ void k_get_bc(Dom d, const double *__restrict__ r, long sr, double *__restrict__ rbc,
-                         long sbc, const int *skip) {
-    SKIP_GUARD
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.nbc) return;
-    const int b = blockIdx.y;
-    r += b * sr;
-    rbc += b * sbc;
+__device__ inline double get_bc_at(const Dom &d, const double *__restrict__ r, int t) {
     const int nx = d.nx, ny = d.ny;
     const int k = t + 1;  // 1-based bc index
     double v;
@@ -141,7 +143,16 @@ __global__ void k_get_bc(Dom d, const double *__restrict__ r, long sr, double *_
             v = 0.5 * (GI(r, nx / 4, j) + GI(r, nx / 4, j - 1));
         }
     }
-    rbc[t] = v * 0.25;
+    return v * 0.25;
+}
+
+__global__ void k_get_bc(Dom d, const double *__restrict__ r, long sr, double *__restrict__ rbc,
+                         long sbc, const int *skip) {
+    SKIP_GUARD
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nbc) return;
+    const int b = blockIdx.y;
+    rbc[b * sbc + t] = get_bc_at(d, r + b * sr, t);
 }
 
 void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
@@ -151,6 +162,18 @@ void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc
 }
 
 // out = in + fac·edges(rbc)  (apply_bc!, fluid-ops.jl:249-275, fused with the copy into Γtmp)
+__device__ inline double copy_apply_bc_at(const Dom &d, const double *__restrict__ in,
+                                          const double *__restrict__ rbc, double fac, long t) {
+    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    double v = in ? in[t] : 0.0;
+    // same order as the reference: bottom, top, left, right
+    if (j == 1) v += fac * BC(rbc, (d.Bo + 1) + i);
+    if (j == d.ny - 1) v += fac * BC(rbc, (d.To + 1) + i);
+    if (i == 1) v += fac * BC(rbc, (d.Lo + 1) + j);
+    if (i == d.nx - 1) v += fac * BC(rbc, (d.Ro + 1) + j);
+    return v;
+}
+
 __global__ void k_copy_apply_bc(Dom d, const double *__restrict__ in, long sin,
                                 double *__restrict__ out, long sout,
                                 const double *__restrict__ rbc, long sbc, double fac,
@@ -159,17 +182,7 @@ __global__ void k_copy_apply_bc(Dom d, const double *__restrict__ in, long sin,
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    in += b * sin;
-    out += b * sout;
-    rbc += b * sbc;
-    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
-    double v = in ? in[t] : 0.0;
-    // same order as the reference: bottom, top, left, right
-    if (j == 1) v += fac * BC(rbc, (d.Bo + 1) + i);
-    if (j == d.ny - 1) v += fac * BC(rbc, (d.To + 1) + i);
-    if (i == 1) v += fac * BC(rbc, (d.Lo + 1) + j);
-    if (i == d.nx - 1) v += fac * BC(rbc, (d.Ro + 1) + j);
-    out[t] = v;
+    out[b * sout + t] = copy_apply_bc_at(d, in ? in + b * sin : nullptr, rbc + b * sbc, fac, t);
 }
 
 void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out, long sout,
@@ -222,6 +235,23 @@ __device__ inline double fqy(const NLCtx &c, int i, int j) {
     return -(Qx_avg(c, i + 1, j) * GI(c.g, i, j - 1) + Qx_avg(c, i, j) * GI(c.g, i - 1, j - 1)) / 2;
 }
 
+__device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q,
+                                      const double *__restrict__ g,
+                                      const double *__restrict__ gbc, double gbc_scale,
+                                      double U, double V, double hc, long t) {
+    NLCtx c;
+    c.d = d;
+    c.q = q;
+    c.g = g;
+    c.gbc = gbc;
+    c.U = U;
+    c.V = V;
+    c.gs = gbc_scale;
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
+    return v * (1 / (hc * hc));
+}
+
 __global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
                             const double *__restrict__ g, long sg, const double *__restrict__ gbc,
                             long sbc, double gbc_scale, double *__restrict__ out, long sout,
@@ -236,17 +266,8 @@ __global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
     const int b = blockIdx.y;
     const int lev = lev0 + b;
     const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
-    NLCtx c;
-    c.d = d;
-    c.q = q + b * sq;
-    c.g = g + b * sg;
-    c.gbc = gbc + b * sbc;
-    c.U = q0_given ? ux : ux * hc;
-    c.V = q0_given ? uy : uy * hc;
-    c.gs = gbc_scale;
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
-    double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
-    out[b * sout + t] = v * (1 / (hc * hc));
+    out[b * sout + t] = nonlinear_at(d, q + b * sq, g + b * sg, gbc + b * sbc, gbc_scale,
+                                     q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, t);
 }
 
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
@@ -259,16 +280,22 @@ void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, l
 
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac[l]·bcraw[l] for ℓ ≤ L-2, accbc[L-1] = 0 (the rhsbc of
 // GetTrialState accumulates over levels, timestepping.jl:41-51).
-__global__ void k_bc_prefix(int nbc, const double *__restrict__ bcraw, double *__restrict__ acc,
-                            const double *__restrict__ fac, int L) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nbc) return;
+__device__ inline void bc_prefix_at(int nbc, const double *__restrict__ bcraw,
+                                    double *__restrict__ acc, const double *__restrict__ fac,
+                                    int L, int t) {
     double s = 0.0;
     acc[(long)(L - 1) * nbc + t] = 0.0;
     for (int l = L - 2; l >= 0; --l) {
         s += fac[l] * bcraw[(long)l * nbc + t];
         acc[(long)l * nbc + t] = s;
     }
+}
+
+__global__ void k_bc_prefix(int nbc, const double *__restrict__ bcraw, double *__restrict__ acc,
+                            const double *__restrict__ fac, int L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbc) return;
+    bc_prefix_at(nbc, bcraw, acc, fac, L, t);
 }
 
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
@@ -347,5 +374,89 @@ void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, in
     hipLaunchKernelGGL(k_scale_tables, dim3(nblk(d.ng, 256)), dim3(256), 0, s, d, sD, sA, sAinv,
                        L, h, dt, Re, scale);
 }
+
+// ------------------------------------------------------------------------ grouped launches
+// The same per-element bodies over a list of problems of different sizes: the workgroup finds
+// its problem, then its batch entry b and element t inside that problem.
+#define GRP_PROLOGUE(TYPE, ELEMS)                                    \
+    const int p_ = grp_find(off, n, blockIdx.x);                     \
+    const TYPE A = args[p_];                                         \
+    const int lb_ = blockIdx.x - off[p_];                            \
+    const int nbx_ = (int)nblk((ELEMS), GRP_BS);                     \
+    const int b = lb_ / nbx_;                                        \
+    const long t = (long)(lb_ % nbx_) * GRP_BS + threadIdx.x;        \
+    const Dom &d = A.d;                                              \
+    (void)d;
+
+__global__ __launch_bounds__(GRP_BS) void kg_curl(const CurlA *args, const int *off, int n) {
+    GRP_PROLOGUE(CurlA, A.d.nq)
+    if (t >= d.nq) return;
+    A.q[b * A.sq + t] = curl_at(d, A.psi + b * A.spsi, A.pbc + b * A.sbc, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_rot(const RotA *args, const int *off, int n) {
+    GRP_PROLOGUE(RotA, A.d.ng)
+    if (t >= d.ng) return;
+    A.g[b * A.sg + t] = rot_at(d, A.q + b * A.sq, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_coarsify(const CoarsA *args, const int *off, int n) {
+    GRP_PROLOGUE(CoarsA, grp_elems_coarsify(A.d))
+    if (t >= grp_elems_coarsify(d)) return;
+    coarsify_at(d, A.gc + b * A.sgc, A.g + b * A.sg, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_get_bc(const GetBcA *args, const int *off, int n) {
+    GRP_PROLOGUE(GetBcA, A.d.nbc)
+    if (t >= d.nbc) return;
+    A.rbc[b * A.sbc + t] = get_bc_at(d, A.r + b * A.sr, (int)t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, const int *off,
+                                                           int n) {
+    GRP_PROLOGUE(CopyBcA, A.d.ng)
+    if (t >= d.ng) return;
+    A.out[b * A.sout + t] =
+        copy_apply_bc_at(d, A.in ? A.in + b * A.sin : nullptr, A.rbc + b * A.sbc, A.fac, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, const int *off, int n) {
+    GRP_PROLOGUE(NonlinA, A.d.ng)
+    if (t >= d.ng) return;
+    const double hc = A.h * (double)(1L << (A.lev0 + b));
+    const double ux = A.uv[0], uy = A.uv[1];
+    A.out[b * A.sout + t] = nonlinear_at(d, A.q + b * A.sq, A.g + b * A.sg, A.gbc + b * A.sbc,
+                                         A.gbc_scale, ux * hc, uy * hc, hc, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const BcPrefA A = args[p];
+    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t >= A.nbc) return;
+    bc_prefix_at(A.nbc, A.bcraw, A.acc, A.fac, A.L, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_fill(const FillA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const FillA A = args[p];
+    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t < A.n) A.p[t] = A.v;
+}
+
+#define GRP_LAUNCH(NAME, KERNEL, TYPE)                                                       \
+    void NAME(const Grp &g, hipStream_t s) {                                                 \
+        if (g.n == 0 || g.blocks == 0) return;                                               \
+        hipLaunchKernelGGL(KERNEL, dim3(g.blocks), dim3(GRP_BS), 0, s,                       \
+                           static_cast<const TYPE *>(g.args), g.off, g.n);                   \
+    }
+GRP_LAUNCH(launch_grp_curl, kg_curl, CurlA)
+GRP_LAUNCH(launch_grp_rot, kg_rot, RotA)
+GRP_LAUNCH(launch_grp_coarsify, kg_coarsify, CoarsA)
+GRP_LAUNCH(launch_grp_get_bc, kg_get_bc, GetBcA)
+GRP_LAUNCH(launch_grp_copy_apply_bc, kg_copy_apply_bc, CopyBcA)
+GRP_LAUNCH(launch_grp_nonlinear, kg_nonlinear, NonlinA)
+GRP_LAUNCH(launch_grp_bc_prefix, kg_bc_prefix, BcPrefA)
+GRP_LAUNCH(launch_grp_fill, kg_fill, FillA)
 
 }  // namespace ibh

@@ -15,95 +15,10 @@
 #include <vector>
 
 #include "../../include/ibh.h"
+#include "ctx.h"
 #include "ibh_internal.h"
 
 using namespace ibh;
-
-namespace {
-
-// kind: 0 full contraction, 1 split left (C = S·X), 2 split right (C = X·S)
-struct TuneKey {
-    int prec, kind, M, N, K, batch;
-    bool tout;
-    bool operator<(const TuneKey &o) const {
-        return std::tie(prec, kind, M, N, K, batch, tout) <
-               std::tie(o.prec, o.kind, o.M, o.N, o.K, o.batch, o.tout);
-    }
-};
-
-struct Prof {
-    bool on = false;
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
-    std::vector<std::pair<int, double>> rec;  // (kind, flops) per recorded launch
-};
-
-}  // namespace
-
-struct ibh_ctx {
-    Dom d;
-    int L = 1;
-    double h = 0, x0 = 0, y0 = 0, Re = 0, dt = 0, beta[2] = {1.5, -0.5};
-    int nbodies = 0, nb = 0;
-    std::vector<int> npts, moving, boff;
-    bool anymove = false, use_cg = false;
-    int precision = 0, force_solver = 0, cg_maxiter = 5000;
-    double cg_reltol = 1e-12;
-    hipStream_t s = nullptr;
-
-    // transforms
-    double *Sm = nullptr, *Sp = nullptr;
-    // odd/even parts of the sine matrices for the split contraction (x and y dimensions)
-    double *Som = nullptr, *Sem = nullptr, *Sop = nullptr, *Sep = nullptr;
-    bool use_eo = true;
-    double *sD = nullptr, *sA = nullptr, *sAinv = nullptr;
-    // state (StreamFcnQuantities + CNAB.State.nonlin)
-    double *gam = nullptr, *psi = nullptr, *q = nullptr, *nl[2] = {nullptr, nullptr};
-    double ux = 0, uy = 0, t = 0;
-    // solver work
-    double *qs = nullptr, *gs = nullptr, *bcraw = nullptr, *accbc = nullptr, *psibc = nullptr;
-    double *gtmp = nullptr, *fac = nullptr, *qtmp = nullptr, *g1 = nullptr;
-    double *T1 = nullptr, *T2 = nullptr;
-    long Tcap = 0;  // elements in T1/T2
-    // B-product work (batched over nc columns)
-    int ncB = 0;
-    double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,
-           *bcB = nullptr, *g1B = nullptr;
-    // bodies
-    double *pts = nullptr, *vel = nullptr, *ds = nullptr;
-    int *d_boff = nullptr, *d_npts = nullptr;
-    RegDev reg{};
-    int *derr = nullptr;       // device error flags (bit 1 bounds, 2 non-finite, 4 singular)
-    double *Fb = nullptr, *Ftmp = nullptr, *CD = nullptr, *CL = nullptr, *fbs = nullptr;
-    double *Binv = nullptr;
-    CGDev cg{};
-    int cg_last = 0;
-    // PCG preconditioner: dense inverse of the assembled B (moving columns refreshed)
-    bool use_pcg = false, pc_valid = false;
-    int pc_refresh = 50, steps_since_pc = 0;
-    double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
-    int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
-    std::vector<int> J;                      // unknowns of moving bodies
-    std::vector<std::pair<int, int>> Jrange; // contiguous [start, count) runs of J
-    bool setup_done = false;
-    // step graphs (per pointer parity) and the device parameter/status words
-    double *uv = nullptr;      // freestream (ux, uy) of the current step
-    int *status = nullptr;     // [CG done, CG iterations, error flags, pad]
-    bool use_graphs = true, capturing = false;
-    int parity = 0;
-    long steps_done = 0;
-    hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter = nullptr,
-                   gPost[2] = {nullptr, nullptr};
-    Prof prof;
-    std::string err;
-    std::vector<void *> allocs;
-    // GEMM autotuner: chosen variant per shape, scratch output, log of the choices
-    std::map<TuneKey, int> tuned;
-    int tune_fixed = -1;
-    double *tune_buf = nullptr;
-    size_t tune_cap = 0;
-    std::vector<std::string> tune_log;
-};
 
 namespace {
 
@@ -241,14 +156,55 @@ GemmArgs gargs() {
 // LaplacianInv apply (fluid-ops.jl:42-56) on `nbatch` fields: out = S((S·in·S)⊙scale)S where
 // the 16·s factor of two RODFT00 passes lives in the scale table.  The last contraction's
 // epilogue is `epi` (EPI_STORE, EPI_SUBFROM with D = out, or EPI_RHS with the extras in rhs).
-struct RhsExtras {
-    const double *N1, *N2, *bcacc;
-    double f1, f2;
-};
+}  // namespace
 
 // The four contractions are TN GEMMs (both operands k-fastest): the left ones read S_m
 // through its symmetry and write their result transposed, so the right ones read it
-// k-fastest too.
+// k-fastest too.  Through the odd/even split each costs half the MFMA work of the full one.
+void ibh::lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long sout,
+                      const double *scale, long sscale, int nbatch, int epi, const RhsExtras *rx,
+                      const int *skip, LapInvPlan *P) {
+    const Dom &d = c->d;
+    const int m = d.m, p = d.p;
+    const long ng = d.ng;
+    GemmArgs g = gargs();
+    g.batch = nbatch;
+    g.skip = skip;
+    g.M = m; g.N = p; g.K = m; g.h = d.nx / 2;
+    g.So = c->Som; g.Se = c->Sem;
+    g.B = in; g.ldb = m; g.sB = sin;
+    g.C = c->T1; g.ldc = p; g.sC = ng;
+    P->g[0] = g; P->epi[0] = EPI_STORE; P->kind[0] = 1;
+    g.K = p; g.h = d.ny / 2;
+    g.So = c->Sop; g.Se = c->Sep;
+    g.B = nullptr;
+    g.A = c->T1; g.lda = p; g.sA = ng;
+    g.C = c->T2; g.ldc = m; g.sC = ng;
+    g.D = scale; g.ldd = m; g.sD = sscale;
+    P->g[1] = g; P->epi[1] = EPI_SCALE; P->kind[1] = 2;
+    g.D = nullptr; g.A = nullptr;
+    g.K = m; g.h = d.nx / 2;
+    g.So = c->Som; g.Se = c->Sem;
+    g.B = c->T2; g.ldb = m; g.sB = ng;
+    g.C = c->T1; g.ldc = p; g.sC = ng;
+    P->g[2] = g; P->epi[2] = EPI_STORE; P->kind[2] = 1;
+    g.K = p; g.h = d.ny / 2;
+    g.So = c->Sop; g.Se = c->Sep;
+    g.B = nullptr;
+    g.A = c->T1; g.lda = p; g.sA = ng;
+    g.C = out; g.ldc = m; g.sC = sout;
+    if (epi == EPI_SUBFROM) {
+        g.D = out; g.ldd = m; g.sD = sout;
+    } else if (epi == EPI_RHS) {
+        g.N1 = rx->N1; g.N2 = rx->N2; g.bcacc = rx->bcacc;
+        g.f1 = rx->f1; g.f2 = rx->f2;
+        g.nbc = d.nbc; g.Lo = d.Lo; g.Ro = d.Ro; g.Bo = d.Bo; g.To = d.To;
+    }
+    P->g[3] = g; P->epi[3] = epi; P->kind[3] = 2;
+}
+
+namespace {
+
 void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, const double *scale,
             long sscale, int nbatch, int epi, const RhsExtras *rx, const int *skip,
             int prec = -1) {
@@ -256,44 +212,15 @@ void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, cons
     const int m = d.m, p = d.p;
     const long ng = d.ng;
     if (prec < 0) prec = c->precision;
+    if (c->use_eo) {
+        LapInvPlan P;
+        lapinv_plan(c, in, sin, out, sout, scale, sscale, nbatch, epi, rx, skip, &P);
+        for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], prec, P.kind[k] == 1, P.kind[k]);
+        return;
+    }
     GemmArgs g = gargs();
     g.batch = nbatch;
     g.skip = skip;
-    if (c->use_eo) {
-        // same four contractions through the odd/even split (half the MFMA work)
-        g.M = m; g.N = p; g.K = m; g.h = d.nx / 2;
-        g.So = c->Som; g.Se = c->Sem;
-        g.B = in; g.ldb = m; g.sB = sin;
-        g.C = c->T1; g.ldc = p; g.sC = ng;
-        gemm(c, g, EPI_STORE, prec, true, 1);
-        g.K = p; g.h = d.ny / 2;
-        g.So = c->Sop; g.Se = c->Sep;
-        g.B = nullptr;
-        g.A = c->T1; g.lda = p; g.sA = ng;
-        g.C = c->T2; g.ldc = m; g.sC = ng;
-        g.D = scale; g.ldd = m; g.sD = sscale;
-        gemm(c, g, EPI_SCALE, prec, false, 2);
-        g.D = nullptr; g.A = nullptr;
-        g.K = m; g.h = d.nx / 2;
-        g.So = c->Som; g.Se = c->Sem;
-        g.B = c->T2; g.ldb = m; g.sB = ng;
-        g.C = c->T1; g.ldc = p; g.sC = ng;
-        gemm(c, g, EPI_STORE, prec, true, 1);
-        g.K = p; g.h = d.ny / 2;
-        g.So = c->Sop; g.Se = c->Sep;
-        g.B = nullptr;
-        g.A = c->T1; g.lda = p; g.sA = ng;
-        g.C = out; g.ldc = m; g.sC = sout;
-        if (epi == EPI_SUBFROM) {
-            g.D = out; g.ldd = m; g.sD = sout;
-        } else if (epi == EPI_RHS) {
-            g.N1 = rx->N1; g.N2 = rx->N2; g.bcacc = rx->bcacc;
-            g.f1 = rx->f1; g.f2 = rx->f2;
-            g.nbc = d.nbc; g.Lo = d.Lo; g.Ro = d.Ro; g.Bo = d.Bo; g.To = d.To;
-        }
-        gemm(c, g, epi, prec, false, 2);
-        return;
-    }
     // T1ᵀ = (S_m · in)ᵀ
     g.M = m; g.N = p; g.K = m;
     g.A = c->Sm; g.lda = m; g.sA = 0;
@@ -449,6 +376,40 @@ int invert(ibh_ctx *c, double *A, double *Inv) {
     return check_flags(c);
 }
 
+void dfree(ibh_ctx *c, void *p) {
+    if (!p) return;
+    for (size_t k = 0; k < c->allocs.size(); ++k)
+        if (c->allocs[k] == p) {
+            c->allocs.erase(c->allocs.begin() + k);
+            hipFree(p);
+            return;
+        }
+}
+
+// Give back the column-batch work of the dense-inverse setup (a few GB on large grids); the
+// step's own T1/T2/gtmp return to L·nΓ.
+int release_B_work(ibh_ctx *c) {
+    HIPCK(hipStreamSynchronize(c->s));
+    drop_graphs(c);
+    for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB}) {
+        dfree(c, *p);
+        *p = nullptr;
+    }
+    c->ncB = 0;
+    const long base = (long)c->L * c->d.ng;
+    if (c->Tcap > base) {
+        dfree(c, c->T1);
+        dfree(c, c->T2);
+        dfree(c, c->gtmp);
+        ALLOC(c->T1, base);
+        ALLOC(c->T2, base);
+        ALLOC(c->gtmp, base);
+        c->Tcap = base;
+    }
+    HIPCK(hipStreamSynchronize(c->s));
+    return IBH_OK;
+}
+
 int precompute_binv(ibh_ctx *c) {
     const int nf = 2 * c->nb;
     if (nf == 0) return IBH_OK;
@@ -457,7 +418,10 @@ int precompute_binv(ibh_ctx *c) {
     if (!c->Binv) ALLOC(c->Binv, (size_t)nf * nf);
     int rc = B_columns(c, Bmat, 0, nf);
     if (rc) return rc;
-    return invert(c, Bmat, c->Binv);  // Binv = inv(Bmat), cnab/ops.jl:96
+    rc = invert(c, Bmat, c->Binv);  // Binv = inv(Bmat), cnab/ops.jl:96
+    if (rc) return rc;
+    dfree(c, Bmat);
+    return release_B_work(c);
 }
 
 // PCG preconditioner: the first build assembles every column of B; later builds refresh the

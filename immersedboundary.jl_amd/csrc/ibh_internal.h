@@ -147,4 +147,77 @@ void launch_copy(double *dst, const double *src, long n, hipStream_t s);
 void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err,
                     hipStream_t s);
 
+// ---------------------------------------------------------------------------- grouped launches
+// One launch runs a list of independent problems (the cases of a batch, each with its own
+// grid size, level count, body points and buffers).  Problem i owns the workgroups
+// [off[i], off[i+1]) of a 1-D grid of off[n] workgroups; args and off live in device memory.
+struct Grp {
+    const void *args;
+    const int *off;
+    int n, blocks;
+};
+
+__device__ inline int grp_find(const int *off, int n, int b) {
+    int lo = 0, hi = n;  // off[lo] <= b < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Per-problem arguments; nbatch entries per problem with the given element strides (the same
+// meaning as the batched single-case launchers above).  Every grouped stencil kernel uses
+// 256-thread workgroups; blocks(problem) = ceil(elements / 256) · nbatch.
+struct CurlA { Dom d; const double *psi; long spsi; const double *pbc; long sbc; double *q; long sq; int nbatch; };
+struct RotA { Dom d; const double *q; long sq; double *g; long sg; int nbatch; };
+struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
+struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
+struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
+struct NonlinA {
+    Dom d; const double *q; long sq; const double *g; long sg; const double *gbc; long sbc;
+    double gbc_scale; double *out; long sout; double h; int lev0; const double *uv; int nbatch;
+};
+struct BcPrefA { int nbc; const double *bcraw; double *acc; const double *fac; int L; };
+struct FillA { double *p; long n; double v; };
+struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
+struct ETA { Dom d; RegDev r; const double *fb; double *q; };
+struct GemvA { int n; const double *A; const double *x; double *ws; double *y; };
+struct StressA {
+    int nbodies; const int *boff, *npts; int nb; const double *Fb, *ds; double h, dt;
+    double *CD, *CL, *fbs;
+};
+
+// element counts per batch entry of each grouped op (host side, for the block offsets)
+__host__ __device__ inline long grp_elems_coarsify(const Dom &d) {
+    return (long)(d.nx / 2 - 1) * (d.ny / 2 - 1);
+}
+static constexpr int GRP_BS = 256;
+static constexpr int GRP_NS = 13;  // delta support per dimension
+static constexpr int GV_CHUNK = 64; // GEMV column chunk
+
+void launch_grp_curl(const Grp &g, hipStream_t s);
+void launch_grp_rot(const Grp &g, hipStream_t s);
+void launch_grp_coarsify(const Grp &g, hipStream_t s);
+void launch_grp_get_bc(const Grp &g, hipStream_t s);
+void launch_grp_copy_apply_bc(const Grp &g, hipStream_t s);
+void launch_grp_nonlinear(const Grp &g, hipStream_t s);
+void launch_grp_bc_prefix(const Grp &g, hipStream_t s);   // blocks: ceil(nbc/256)
+void launch_grp_fill(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
+void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
+void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)
+void launch_grp_ET(const Grp &g, hipStream_t s);          // ETA; blocks: ceil(2nb·169/256)
+void launch_grp_gemv_part(const Grp &g, hipStream_t s);   // blocks: ceil(n/256)·ceil(n/64)
+void launch_grp_gemv_sum(const Grp &g, hipStream_t s);    // blocks: ceil(n/256)
+void launch_grp_stress(const Grp &g, hipStream_t s);      // blocks: nbodies
+
+// grouped even/odd split contraction: every problem's GemmArgs in device memory; problem i
+// owns tiles [off[i], off[i+1]) with tiles = ceil(rows/BM)·ceil(cols/BN)·batch for the
+// variant's tile shape (grouped_eo_tiles).
+void launch_gemm_eo_grp(const Grp &g, int prec, int variant, int side, int epi, hipStream_t s);
+int gemm_eo_grp_variant_count(int prec);
+const char *gemm_eo_grp_variant_name(int prec, int v);
+long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side);
+
 }  // namespace ibh

@@ -10,7 +10,7 @@ from .problem import (  # noqa: F401
     SchemeCNAB, StreamFcnFluid, UniformGrid, circle, timesteps,
 )
 from .cnab import (  # noqa: F401
-    BodyStress, DeviceCase, Quantities, Solver, State, StateCallback, StateResult,
+    Batch, BodyStress, DeviceCase, Quantities, Solver, State, StateCallback, StateResult,
     StopIteration, advance_, at_indices, at_times, each_timestep, initsolver, initstate,
     quantities, solve, solve_,
 )
@@ -19,7 +19,7 @@ from ._lib import IbhError, load as load_library  # noqa: F401
 __all__ = [
     "AbstractFrame", "BodyGroup", "GridFrame", "LabFrame", "MultiLevel", "Problem",
     "RelativeFrame", "RigidBody", "SchemeCNAB", "StreamFcnFluid", "UniformGrid", "circle",
-    "timesteps", "BodyStress", "DeviceCase", "Quantities", "Solver", "State", "StateCallback",
+    "timesteps", "Batch", "BodyStress", "DeviceCase", "Quantities", "Solver", "State", "StateCallback",
     "StateResult", "StopIteration", "advance_", "at_indices", "at_times", "each_timestep",
     "initsolver", "initstate", "quantities", "solve", "solve_", "IbhError", "load_library",
 ]

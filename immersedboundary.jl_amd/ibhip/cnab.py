@@ -325,6 +325,82 @@ def advance_(state: State, solver: Solver, t: float) -> None:
     state.t = float(t)
 
 
+class Batch:
+    """Independent cases advanced together: every substep of advance! is one grouped launch
+    over all cases (each with its own grid, levels, Re, dt and bodies) and the whole step is
+    one HIP graph.  A parameter sweep that the reference runs as one solve per case
+    (src/solve.jl:154-186) steps here as one batch.
+
+    The states must come from initstate + initsolver of problems with fixed bodies (dense
+    force solver) and one precision, and share the freestream: the first problem's
+    freestream at its current time is used for every case.  `advance()` is one advance! of
+    every case at its own t = t0 + k·dt; the states' fields read the batch's results after
+    `sync()`.
+    """
+
+    def __init__(self, states):
+        self.lib = _lib.load()
+        self.states = list(states)
+        if not self.states:
+            raise ValueError("a batch needs at least one case")
+        arr = (C.c_void_p * len(self.states))(*[st.case.h.value for st in self.states])
+        h = C.c_void_p()
+        rc = self.lib.ibh_batch_create(len(self.states), arr, C.byref(h))
+        if rc != 0:
+            raise _lib.IbhError(rc, self.lib.ibh_last_error(self.states[0].case.h).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise _lib.IbhError(rc, self.lib.ibh_batch_last_error(self.h).decode())
+
+    def _freestream(self):
+        st = self.states[0]
+        return st.case.prob.fluid.freestream(st.t)
+
+    def advance(self):
+        ux, uy = self._freestream()
+        self._check(self.lib.ibh_batch_advance(self.h, float(ux), float(uy)))
+        for st in self.states:
+            st.t = st.case.time()
+
+    def advance_n(self, n: int):
+        ux, uy = self._freestream()
+        self._check(self.lib.ibh_batch_advance_n(self.h, int(n), float(ux), float(uy)))
+        for st in self.states:
+            st.t = st.case.time()
+
+    def sync(self):
+        self._check(self.lib.ibh_batch_sync(self.h))
+
+    def launches_per_step(self) -> int:
+        return self.lib.ibh_batch_launches_per_step(self.h)
+
+    def profile(self, on: bool):
+        self._check(self.lib.ibh_batch_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        ms, n, fl = C.c_double(), C.c_int64(), C.c_double()
+        self._check(self.lib.ibh_batch_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(fl)))
+        return ms.value, n.value, fl.value
+
+    def tune_report(self) -> list:
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.lib.ibh_batch_tune_report(self.h, buf, len(buf)))
+        return [ln for ln in buf.value.decode().splitlines() if ln]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ibh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ------------------------------------------------------------------------------ callbacks
 class StopIteration_:
     """StopIteration() — returned by a callback to stop solve (src/solve.jl:1-6)."""

@@ -80,6 +80,41 @@ def cylinder_array(n=2048, nlev=4, precision="fp64", force_solver="auto", seed=5
                    SchemeCNAB(dt, precision=precision, force_solver=force_solver, **kw))
 
 
+def ellipse(center, a, b, rot, h) -> RigidBody:
+    """An ellipse with semi-axes a, b rotated by `rot`, its points spaced ≈h in arc length."""
+    s = np.linspace(0.0, 2.0 * math.pi, 4097)
+    x, y = a * np.cos(s), b * np.sin(s)
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(x), np.diff(y)))])
+    n = max(8, int(round(arc[-1] / h)))
+    sk = np.interp(np.arange(n) * (arc[-1] / n), arc, s)
+    xe, ye = a * np.cos(sk), b * np.sin(sk)
+    c, sn = math.cos(rot), math.sin(rot)
+    xb = np.stack([center[0] + c * xe - sn * ye, center[1] + sn * xe + c * ye], axis=1)
+    return RigidBody(GridFrame(), xb, np.full(n, arc[-1] / n))
+
+
+def ellipse_batch(ncases=64, seed=4, precision="fp32", sizes=(128, 192, 256, 384, 512),
+                  levels=(2, 3, 4, 5), **kw) -> list:
+    """Independent cases of a parameter sweep: grid n uniform over `sizes` on the 4×4 domain
+    centred at (1, 0) (h = 4/n), 2–5 levels, one fixed ellipse (a = 0.5, b ~ U[0.1, 0.5],
+    rotation ~ U[0, π)) with points spaced ≈h, Re ~ U[50, 400], dt = 0.2·h (CFL 0.2)."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(ncases):
+        n = int(rng.choice(sizes))
+        nlev = int(rng.choice(levels))
+        b = float(rng.uniform(0.1, 0.5))
+        rot = float(rng.uniform(0.0, math.pi))
+        Re = float(rng.uniform(50.0, 400.0))
+        h = 4.0 / n
+        grid = UniformGrid(h, (-1.0, 3.0), (-2.0, 2.0))
+        fluid = StreamFcnFluid(MultiLevel(grid, nlev), (1.0, 0.0), Re)
+        body = ellipse((0.0, 0.0), 0.5, b, rot, h)
+        probs.append(Problem(fluid, BodyGroup(body),
+                             SchemeCNAB(0.2 * h, precision=precision, force_solver="dense", **kw)))
+    return probs
+
+
 WORKLOADS = {
     "cylinder_re100": cylinder_re100,
     "cylinder_plate": cylinder_plate,

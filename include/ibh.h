@@ -159,6 +159,30 @@ int ibh_op_ET(ibh_ctx *ctx, const double *fb, double *q);
 /* B·z (B_Times, cnab/ops.jl:125-143): z (2nb) → x (2nb). */
 int ibh_op_B(ibh_ctx *ctx, const double *z, double *x);
 
+/* ---- batches of independent cases ------------------------------------------------------- */
+/* Many independent simulations (the dozens of cases a parameter sweep runs through
+ * solve(prob, tspan) one after another, src/solve.jl:154-186) advanced together: every
+ * substep of advance! is one launch over all cases, each with its own grid size, level count,
+ * Re, dt and bodies.  The cases are contexts after ibh_init_state + ibh_setup, with fixed
+ * bodies and the dense force solver (the reference's reachable path, cnab/ops.jl:79-102), all
+ * of one precision.  The batch borrows them (ibh_destroy them after ibh_batch_destroy) and
+ * keeps their state current: after ibh_batch_sync, ibh_get_field on a case reads its fields.
+ * A case stepped on its own after ibh_batch_create makes the next batch step fail. */
+typedef struct ibh_batch ibh_batch;
+int ibh_batch_create(int32_t ncases, ibh_ctx *const *cases, ibh_batch **out);
+void ibh_batch_destroy(ibh_batch *batch);
+const char *ibh_batch_last_error(const ibh_batch *batch);
+/* one advance! of every case (its own dt; t = t0 + k·dt as in solve!'s loop) with the same
+ * freestream (ux, uy) for all cases */
+int ibh_batch_advance(ibh_batch *batch, double ux, double uy);
+int ibh_batch_advance_n(ibh_batch *batch, int32_t nsteps, double ux, double uy);
+int ibh_batch_sync(ibh_batch *batch);
+/* grouped launches per batch step (one HIP graph replays them) */
+int ibh_batch_launches_per_step(ibh_batch *batch);
+int ibh_batch_profile_enable(ibh_batch *batch, int32_t on);
+int ibh_batch_profile_read(ibh_batch *batch, double *ms, int64_t *launches, double *flops);
+int ibh_batch_tune_report(ibh_batch *batch, char *buf, int32_t len);
+
 /* ---- profiling (HIP events around each launch of the sine-transform kernels) ----------- */
 int ibh_profile_enable(ibh_ctx *ctx, int32_t on);
 /* The GEMM autotuner's choices so far (one line per contraction shape: tile variant, sizes,

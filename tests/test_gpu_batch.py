@@ -1,0 +1,158 @@
+"""GPU parity of batches: cases of different grid sizes, level counts, Re, dt and bodies
+advanced together (one grouped launch per substep, the step replayed from HIP graphs) must
+give each case the results it gets when stepped alone, and the FP64 oracle's within 1e-10.
+"""
+import numpy as np
+import pytest
+
+import ibref
+from cases import cylinder_case, cylinder_plate_case
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.abs(b).max()
+    return np.abs(a - b).max() / (den if den > 0 else 1.0)
+
+
+def mixed_problems(precision="fp64"):
+    """Six small cases: grids 64²–128² (one rectangular), 1–4 levels, ellipses and a circle,
+    one case without bodies."""
+    from ibhip.problem import BodyGroup, MultiLevel, Problem, SchemeCNAB, StreamFcnFluid, UniformGrid
+    from ibhip.workloads import ellipse
+
+    specs = [(64, 2, 0.30, 0.3, 120.0), (96, 3, 0.45, 1.1, 300.0), (128, 4, 0.15, 2.0, 60.0),
+             (64, 1, 0.40, 0.7, 200.0), (80, 3, 0.25, 2.9, 90.0)]
+    probs = []
+    for n, L, b, rot, Re in specs:
+        h = 4.0 / n
+        grid = UniformGrid(h, (-1.0, 3.0), (-2.0, 2.0))
+        fl = StreamFcnFluid(MultiLevel(grid, L), (1.0, 0.0), Re)
+        probs.append(Problem(fl, BodyGroup(ellipse((0.0, 0.0), 0.5, b, rot, h)),
+                             SchemeCNAB(0.2 * h, precision=precision, force_solver="dense")))
+    # rectangular grid, no bodies
+    h = 0.05
+    grid = UniformGrid(h, (-1.0, -1.0 + 96 * h), (-1.6, -1.6 + 64 * h))
+    probs.append(Problem(StreamFcnFluid(MultiLevel(grid, 2), (1.0, 0.0), 150.0), BodyGroup(),
+                         SchemeCNAB(0.01, precision=precision)))
+    probs.append(cylinder_case(n=64, nlev=3, precision=precision))
+    return probs
+
+
+NAMES = ["Γ", "ψ", "q", "N1", "N2", "Fb", "CD", "CL"]
+
+
+def fields(case):
+    f = [case.gamma(), case.psi(), case.q(), case.nonlin(0), case.nonlin(1)]
+    if case.nb:
+        f += [case.Fb(), case.CD(), case.CL()]
+    return f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_batch_matches_single_cases(precision):
+    import ibhip as m
+
+    probs = mixed_problems(precision)
+    nsteps = 6
+    # alone
+    alone = []
+    for p in probs:
+        st = m.initstate(p)
+        sv = m.initsolver(p, st)
+        for k in range(nsteps):
+            m.advance_(st, sv, k * p.scheme.dt)
+        alone.append(fields(st.case))
+        st.case.close()
+    # together
+    states = [m.initstate(p) for p in probs]
+    for p, st in zip(probs, states):
+        m.initsolver(p, st)
+    batch = m.Batch(states)
+    for k in range(nsteps):
+        batch.advance()
+    batch.sync()
+    assert batch.launches_per_step() > 0
+    for i, st in enumerate(states):
+        got = fields(st.case)
+        assert len(got) == len(alone[i])
+        for name, a, b in zip(NAMES, got, alone[i]):
+            # same per-element arithmetic; only the order of the FP64 atomics of the
+            # spreading operator differs between runs.  CL is held to the drag scale (it is
+            # roundoff-seeded for the symmetric cylinder).
+            if name == "CL":
+                err = np.abs(a - b).max() / np.abs(got[6]).max()
+            else:
+                err = rel(a, b)
+            assert err < 1e-12, f"case {i} {name}: {err:.3e}"
+        assert abs(st.t - (nsteps - 1) * probs[i].scheme.dt) < 1e-12
+    batch.close()
+
+
+def test_batch_fp64_matches_oracle():
+    import ibhip as m
+
+    probs = mixed_problems("fp64")
+    pick = [1, 2, 6]  # 96² L=3 ellipse, 128² L=4 ellipse, 64² L=3 cylinder
+    states = [m.initstate(p) for p in probs]
+    for p, st in zip(probs, states):
+        m.initsolver(p, st)
+    batch = m.Batch(states)
+    orcs = {i: ibref.OracleRun(probs[i]) for i in pick}
+    for k in range(5):
+        batch.advance()
+        for i in pick:
+            orcs[i].advance(k * probs[i].scheme.dt)
+    batch.sync()
+    for i in pick:
+        c, o = states[i].case, orcs[i]
+        for lev in range(c.nlev):
+            assert rel(c.gamma()[:, lev], o.gamma()[:, lev]) < 1e-10
+            assert rel(c.q()[:, lev], o.q()[:, lev]) < 1e-10
+        assert rel(c.Fb(), o.Fb()) < 1e-10
+        assert rel(c.CD(), o.CD()) < 1e-10
+    batch.close()
+
+
+def test_batch_graph_replay_matches_eager(monkeypatch):
+    import ibhip as m
+
+    runs = []
+    for graph in ("1", "0"):
+        monkeypatch.setenv("IBH_GRAPH", graph)
+        probs = mixed_problems("fp64")[:4]
+        states = [m.initstate(p) for p in probs]
+        for p, st in zip(probs, states):
+            m.initsolver(p, st)
+        batch = m.Batch(states)
+        hist = []
+        for k in range(6):
+            batch.advance()
+            batch.sync()
+            hist.append([st.case.gamma() for st in states] + [st.case.CD() for st in states])
+        runs.append(hist)
+        batch.close()
+    for h1, h2 in zip(*runs):
+        for a, b in zip(h1, h2):
+            assert rel(a, b) < 1e-12
+
+
+def test_batch_rejects_moving_bodies_and_outside_steps():
+    import ibhip as m
+
+    p = cylinder_plate_case(n=64, nlev=2, cyl_spacing=1.0, force_solver="pcg")
+    st = m.initstate(p)
+    m.initsolver(p, st)
+    with pytest.raises(m.IbhError):
+        m.Batch([st])
+    q = cylinder_case(n=64, nlev=2)
+    s2 = m.initstate(q)
+    v2 = m.initsolver(q, s2)
+    batch = m.Batch([s2])
+    batch.advance()
+    m.advance_(s2, v2, q.scheme.dt)  # stepped on its own: the batch no longer owns its state
+    with pytest.raises(m.IbhError):
+        batch.advance()
+    batch.close()
