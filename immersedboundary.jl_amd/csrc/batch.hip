@@ -29,7 +29,7 @@ namespace {
 
 enum Op {
     OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
-    OP_E, OP_ETZ, OP_ET, OP_GEMVP, OP_GEMVS, OP_STRESS
+    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS
 };
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -40,7 +40,7 @@ struct Stage {
     std::vector<int> blocks;   // per problem (non-GEMM ops)
     // contractions
     std::vector<GemmArgs> gargs;
-    int side = 0, epi = 0, kind = 0;
+    int side = 0, epi = 0, kind = 0, prec = 0;
     int tune_ref = -1;         // index of the stage whose variant this one reuses
     double flops = 0;
     int variant = -1;
@@ -97,12 +97,13 @@ void add(std::vector<Stage> &v, int op, const std::vector<A> &a, const std::vect
 
 // the four contractions of one LaplacianInv apply per case, as four grouped stages; the
 // second left/right pair reuses the first pair's tuned tile variant (same shapes)
-void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans) {
+void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int prec) {
     if (plans.empty()) return;
     const int base = (int)v.size();
     for (int k = 0; k < 4; ++k) {
         Stage S;
         S.op = OP_GEMM;
+        S.prec = prec;
         S.kind = plans[0].kind[k];
         S.side = S.kind - 1;
         S.epi = plans[0].epi[k];
@@ -160,7 +161,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             }
         }
         add(v, OP_FILL, f, fb);
-        add_lapinv(v, plans);
+        add_lapinv(v, plans, b->prec);
         add(v, OP_CURL, cu, cb);
     }
     for (int j = 1; j < b->Lmax; ++j) {
@@ -191,7 +192,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         }
         add(v, OP_GETBC, gb, b1);
         add(v, OP_COPYBC, cp, b2);
-        add_lapinv(v, plans);
+        add_lapinv(v, plans, b->prec);
         add(v, OP_CURL, cu, cb);
     }
 }
@@ -229,25 +230,26 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
     }
     {
         std::vector<NonlinA> a;
+        std::vector<RhsA> ra;
         std::vector<int> bl;
-        std::vector<LapInvPlan> pA, pI;
+        std::vector<LapInvPlan> pI;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             const Dom &d = c->d;
             a.push_back({d, c->q, d.nq, gam[i], d.ng, c->bcraw, d.nbc, 0.25, n0[i], d.ng, c->h, 0,
                          b->uv, c->L});
             bl.push_back(nblk(d.ng, GRP_BS) * c->L);
-            RhsExtras rx{n0[i], n1[i], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
-            LapInvPlan P1, P2;
-            lapinv_plan(c, gam[i], d.ng, gs[i], d.ng, c->sA, d.ng, c->L, EPI_RHS, &rx, nullptr, &P1);
+            // A_ℓΓ as its exact stencil fused with the right-hand side, then Ainv_ℓ
+            ra.push_back({d, gam[i], n0[i], n1[i], c->accbc, gs[i], c->h, c->dt, c->Re,
+                          c->dt * c->beta[0], c->dt * c->beta[1], c->L});
+            LapInvPlan P2;
             lapinv_plan(c, gs[i], d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
                         nullptr, &P2);
-            pA.push_back(P1);
             pI.push_back(P2);
         }
         add(v, OP_NONLIN, a, bl);
-        add_lapinv(v, pA);
-        add_lapinv(v, pI);
+        add(v, OP_RHS, ra, bl);
+        add_lapinv(v, pI, b->prec);
     }
     add_vort2flux(b, v, gs, psi, qs, false);
     // ---- BoundaryForcer (timestepping.jl:106-120): Fb = Binv·E(qs + q0)
@@ -275,7 +277,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         std::vector<StressA> sa;
         std::vector<ETA> et;
         std::vector<RotA> ro;
-        std::vector<int> b1, b2, b3, b4;
+        std::vector<int> b1, b2, b3, b4, b5;
         std::vector<LapInvPlan> plans;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
@@ -284,8 +286,9 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             sa.push_back({c->nbodies, c->d_boff, c->d_npts, c->nb, c->Fb, c->ds, c->h, c->dt, c->CD,
                           c->CL, c->fbs});
             b1.push_back(c->nbodies);
-            et.push_back({d, c->reg, c->Fb, c->qtmp});
+            et.push_back({d, c->reg, c->Fb, c->qtmp, c->etscale});
             b2.push_back(nblk(d.nq, GRP_BS));
+            b5.push_back(1);
             b3.push_back(nblk(2L * c->nb * GRP_NS * GRP_NS, GRP_BS));
             ro.push_back({d, c->qtmp, d.nq, c->g1, d.ng, 1});
             b4.push_back(nblk(d.ng, GRP_BS));
@@ -295,9 +298,11 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         }
         add(v, OP_STRESS, sa, b1);
         add(v, OP_ETZ, et, b2);
+        add(v, OP_ETS, et, b5);
         add(v, OP_ET, et, b3);
+        add(v, OP_ETF, et, b2);
         add(v, OP_ROT, ro, b4);
-        add_lapinv(v, plans);
+        add_lapinv(v, plans, IBH_FP64);  // the projection stays FP64
     }
     add_vort2flux(b, v, gs, psi, q, true);
 }
@@ -306,9 +311,9 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
 int upload(ibh_batch *b) {
     size_t bytes = 0;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const int nv = gemm_eo_grp_variant_count(b->prec);
     for (int P = 0; P < 2; ++P)
         for (Stage &S : b->st[P]) {
+            const int nv = gemm_eo_grp_variant_count(S.prec);
             if (S.op == OP_GEMM) bytes += al(S.gargs.size() * sizeof(GemmArgs)) + al(sizeof(int) * (S.n + 1) * nv);
             else bytes += al(S.args.size()) + al(sizeof(int) * (S.n + 1));
         }
@@ -318,6 +323,7 @@ int upload(ibh_batch *b) {
     for (int P = 0; P < 2; ++P)
         for (Stage &S : b->st[P]) {
             char *base = static_cast<char *>(b->arena);
+            const int nv = gemm_eo_grp_variant_count(S.prec);
             if (S.op == OP_GEMM) {
                 const size_t na = S.gargs.size() * sizeof(GemmArgs);
                 std::memcpy(host.data() + o, S.gargs.data(), na);
@@ -329,7 +335,7 @@ int upload(ibh_batch *b) {
                     int acc = 0;
                     for (int i = 0; i < S.n; ++i) {
                         off[(size_t)v * (S.n + 1) + i] = acc;
-                        acc += (int)grouped_eo_tiles(S.gargs[i], b->prec, v, S.side);
+                        acc += (int)grouped_eo_tiles(S.gargs[i], S.prec, v, S.side);
                     }
                     off[(size_t)v * (S.n + 1) + S.n] = acc;
                     S.total[v] = acc;
@@ -365,7 +371,7 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant) {
     case OP_GEMM:
         g.off = S.d_off + (size_t)variant * (S.n + 1);
         g.blocks = S.total[variant];
-        launch_gemm_eo_grp(g, b->prec, variant, S.side, S.epi, s);
+        launch_gemm_eo_grp(g, S.prec, variant, S.side, S.epi, s);
         break;
     case OP_CURL: launch_grp_curl(g, s); break;
     case OP_ROT: launch_grp_rot(g, s); break;
@@ -377,10 +383,13 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant) {
     case OP_FILL: launch_grp_fill(g, s); break;
     case OP_E: launch_grp_E(g, s); break;
     case OP_ETZ: launch_grp_ET_zero(g, s); break;
+    case OP_ETS: launch_grp_ET_scale(g, s); break;
     case OP_ET: launch_grp_ET(g, s); break;
+    case OP_ETF: launch_grp_ET_fix(g, s); break;
     case OP_GEMVP: launch_grp_gemv_part(g, s); break;
     case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
     case OP_STRESS: launch_grp_stress(g, s); break;
+    case OP_RHS: launch_grp_rhs(g, s); break;
     }
 }
 
@@ -388,7 +397,7 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant) {
 // contraction reads operands it does not write, so re-running it leaves the same result) and
 // keep the fastest.  Variants of one precision sum k in the same order: identical results.
 int tune_stage(ibh_batch *b, Stage &S) {
-    const int nv = gemm_eo_grp_variant_count(b->prec);
+    const int nv = gemm_eo_grp_variant_count(S.prec);
     hipEvent_t e0, e1;
     BCK(hipEventCreate(&e0));
     BCK(hipEventCreate(&e1));
@@ -411,7 +420,7 @@ int tune_stage(ibh_batch *b, Stage &S) {
     hipEventDestroy(e1);
     S.variant = best;
     char line[256];
-    std::snprintf(line, sizeof line, "%s %s problems=%d %.1fus", gemm_eo_grp_variant_name(b->prec, best),
+    std::snprintf(line, sizeof line, "%s %s problems=%d %.1fus", gemm_eo_grp_variant_name(S.prec, best),
                   S.kind == 1 ? "left" : "right", S.n, best_ms / 3 * 1e3);
     b->tune_log.push_back(line);
     return IBH_OK;
@@ -440,7 +449,7 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
             launch_stage(b, S, S.variant);
             hipEventRecord(P.pool[P.used + 1], b->s);
             P.used += 2;
-            P.rec.push_back({b->prec, S.flops});
+            P.rec.push_back({S.prec, S.flops});
         } else {
             launch_stage(b, S, S.variant);
         }
@@ -604,18 +613,22 @@ int ibh_batch_profile_enable(ibh_batch *b, int32_t on) {
     return IBH_OK;
 }
 
-int ibh_batch_profile_read(ibh_batch *b, double *ms, int64_t *launches, double *flops) {
+int ibh_batch_profile_read(ibh_batch *b, int32_t kind, double *ms, int64_t *launches,
+                           double *flops) {
     if (!b) return IBH_EINVAL;
     BCK(hipStreamSynchronize(b->s));
     double tot = 0, fl = 0;
+    int64_t n = 0;
     for (size_t k = 0; k < b->prof.rec.size(); ++k) {
+        if (b->prof.rec[k].first != kind) continue;
+        ++n;
         float e = 0;
         BCK(hipEventElapsedTime(&e, b->prof.pool[2 * k], b->prof.pool[2 * k + 1]));
         tot += e;
         fl += b->prof.rec[k].second;
     }
     if (ms) *ms = tot;
-    if (launches) *launches = (int64_t)b->prof.rec.size();
+    if (launches) *launches = n;
     if (flops) *flops = fl;
     return IBH_OK;
 }

@@ -123,15 +123,41 @@ void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0
 }
 
 // Eᵀ (Reg), structure-ops.jl:63-86: q .= 0 then scatter-add the 13×13 stencils of every point.
-// Points are handled one stencil row per thread; the adds use FP64 atomics.
+// One thread per stencil entry.  The adds are 64-bit integer atomics on a fixed-point image of
+// q, so the sum does not depend on the order in which the adds land: runs are bitwise
+// reproducible (the order of floating-point atomics is not, and low-precision transforms
+// downstream amplify such last-bit differences).  Per column, the scale 2^(50-e) with
+// 2^e ≤ max|f| < 2^(e+1) keeps 2048 full-weight contributions of the largest force below
+// 2^62; each contribution is rounded to 2^-51·max|f|, the size of an FP64 rounding at the top
+// of the range.  A last pass converts q back to FP64 in place.
 __global__ void k_zero(double *__restrict__ p, long n, long sp, const int *skip) {
     if (skip && *skip) return;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t < n) p[blockIdx.y * sp + t] = 0.0;
 }
 
+// one 256-thread workgroup per column: scale[b] = 2^(50-e) for the column's max|f|
+__device__ inline void et_scale_body(const double *__restrict__ fb, int n2, double *scale) {
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) m = fmax(m, fabs(fb[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        *scale = (m > 0.0 && isfinite(m)) ? ldexp(1.0, 50 - ilogb(m)) : 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_et_scale(const double *__restrict__ fb, long sfb,
+                                                  int n2, double *scale, const int *skip) {
+    if (skip && *skip) return;
+    et_scale_body(fb + blockIdx.x * sfb, n2, scale + blockIdx.x);
+}
+
 __device__ inline void ET_at(const Dom &d, const RegDev &r, const double *__restrict__ fb,
-                             double *__restrict__ q, int t) {
+                             double *__restrict__ q, double sc, int t) {
     const int s = t % NS, tt = (t / NS) % NS, kc = t / (NS * NS);
     const int comp = kc / r.nb, k = kc % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
@@ -141,26 +167,47 @@ __device__ inline void ET_at(const Dom &d, const RegDev &r, const double *__rest
     const int i = bx + s - SUPP, j = by + tt - SUPP;
     double *dst = comp == 0 ? q + (i - 1) + (long)(d.nx + 1) * (j - 1)
                             : q + d.nu + (i - 1) + (long)d.nx * (j - 1);
-    if (comp == 0 && !isfinite(v)) atomicOr(r.err, 2);
-    atomicAdd(dst, v);
+    if (!isfinite(v)) {
+        atomicOr(r.err, 2);
+        return;
+    }
+    atomicAdd(reinterpret_cast<unsigned long long *>(dst), (unsigned long long)llrint(v * sc));
 }
 
 __global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
-                     double *__restrict__ q, long sq, const int *skip) {
+                     double *__restrict__ q, long sq, const double *__restrict__ scale,
+                     const int *skip) {
     if (skip && *skip) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * r.nb * NS * NS) return;
     const int b = blockIdx.y;
-    ET_at(d, r, fb + b * sfb, q + b * sq, t);
+    ET_at(d, r, fb + b * sfb, q + b * sq, scale[b], t);
+}
+
+// fixed point → FP64 in place (exact power-of-two rescale after the int64 → double rounding)
+__device__ inline void et_fix_at(double *q, double sc, long t) {
+    const long long iv = reinterpret_cast<const long long *>(q)[t];
+    q[t] = (double)iv * (1.0 / sc);
+}
+
+__global__ void k_et_fix(double *__restrict__ q, long n, long sq, const double *__restrict__ scale,
+                         const int *skip) {
+    if (skip && *skip) return;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    et_fix_at(q + blockIdx.y * sq, scale[blockIdx.y], t);
 }
 
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
-               int nbatch, const int *skip, hipStream_t s) {
+               int nbatch, const int *skip, double *scale, hipStream_t s) {
     hipLaunchKernelGGL(k_zero, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq, skip);
     if (r.nb == 0) return;
+    hipLaunchKernelGGL(k_et_scale, dim3(nbatch), dim3(256), 0, s, fb, sfb, 2 * r.nb, scale, skip);
     long n = 2L * r.nb * NS * NS;
     hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, q, sq,
-                       skip);
+                       (const double *)scale, skip);
+    hipLaunchKernelGGL(k_et_fix, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq,
+                       (const double *)scale, skip);
 }
 
 // y = A·x for a dense n×n column-major matrix (the Binv GEMV of cnab/ops.jl:97 and the PCG
@@ -498,12 +545,26 @@ __global__ __launch_bounds__(GRP_BS) void kg_ET_zero(const ETA *args, const int 
     if (t < A.d.nq) A.q[t] = 0.0;
 }
 
+// one workgroup per problem
+__global__ __launch_bounds__(GRP_BS) void kg_ET_scale(const ETA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const ETA A = args[p];
+    et_scale_body(A.fb, 2 * A.r.nb, A.scale);
+}
+
 __global__ __launch_bounds__(GRP_BS) void kg_ET(const ETA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const ETA A = args[p];
     const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
     if (t >= 2 * A.r.nb * NS * NS) return;
-    ET_at(A.d, A.r, A.fb, A.q, t);
+    ET_at(A.d, A.r, A.fb, A.q, *A.scale, t);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_ET_fix(const ETA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const ETA A = args[p];
+    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t < A.d.nq) et_fix_at(A.q, *A.scale, t);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_gemv_part(const GemvA *args, const int *off, int n) {
@@ -538,7 +599,9 @@ __global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const i
     }
 GRP_LAUNCH(launch_grp_E, kg_E, EA)
 GRP_LAUNCH(launch_grp_ET_zero, kg_ET_zero, ETA)
+GRP_LAUNCH(launch_grp_ET_scale, kg_ET_scale, ETA)
 GRP_LAUNCH(launch_grp_ET, kg_ET, ETA)
+GRP_LAUNCH(launch_grp_ET_fix, kg_ET_fix, ETA)
 GRP_LAUNCH(launch_grp_gemv_part, kg_gemv_part, GemvA)
 GRP_LAUNCH(launch_grp_gemv_sum, kg_gemv_sum, GemvA)
 GRP_LAUNCH(launch_grp_stress, kg_stress, StressA)

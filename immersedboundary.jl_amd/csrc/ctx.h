@@ -69,6 +69,8 @@ struct ibh_ctx {
     int *d_boff = nullptr, *d_npts = nullptr;
     RegDev reg{};
     int *derr = nullptr;       // device error flags (bit 1 bounds, 2 non-finite, 4 singular)
+    double *etscale = nullptr; // fixed-point scale of the spreading, one per column
+    int etcap = 0;
     double *Fb = nullptr, *Ftmp = nullptr, *CD = nullptr, *CL = nullptr, *fbs = nullptr;
     double *Binv = nullptr;
     CGDev cg{};

@@ -304,6 +304,50 @@ void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const do
                        fac, L);
 }
 
+// Right-hand side of the trial solve (timestepping.jl:44-57) with A_ℓ applied as its exact
+// 5-point stencil: A_ℓ = I − a(−L) with a = dt/(2 Re hc²) (cnab/ops.jl:26; the sine transform
+// of fluid-ops.jl:42-56 diagonalises the same operator with zero values beyond the edges),
+// then + f1·N1 + f2·N2 + the edge terms of the accumulated boundary vector, in the order of
+// the reference's additions.
+__device__ inline double rhs_at(const Dom &d, const double *__restrict__ g,
+                                const double *__restrict__ N1, const double *__restrict__ N2,
+                                const double *__restrict__ bc, double a, double f1, double f2,
+                                long t) {
+    const int i = (int)(t % d.m), j = (int)(t / d.m);  // 0-based
+    const double c = g[t];
+    const double nsum = (i > 0 ? g[t - 1] : 0.0) + (i < d.m - 1 ? g[t + 1] : 0.0) +
+                        (j > 0 ? g[t - d.m] : 0.0) + (j < d.p - 1 ? g[t + d.m] : 0.0);
+    double v = c + a * (nsum - 4.0 * c);
+    v = v + f1 * N1[t];
+    v = v + f2 * N2[t];
+    double eb = 0.0;
+    if (j == 0) eb += bc[d.Bo + 1 + i];
+    if (j == d.p - 1) eb += bc[d.To + 1 + i];
+    if (i == 0) eb += bc[d.Lo + 1 + j];
+    if (i == d.m - 1) eb += bc[d.Ro + 1 + j];
+    return v + eb;
+}
+
+__global__ void k_rhs(Dom d, const double *__restrict__ g, const double *__restrict__ N1,
+                      const double *__restrict__ N2, const double *__restrict__ bc,
+                      double *__restrict__ out, double h, double dt, double Re, double f1,
+                      double f2) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;  // level
+    const double hc = h * (double)(1L << b);
+    const double a = dt / (2 * Re * (hc * hc));
+    const long o = b * d.ng;
+    out[o + t] = rhs_at(d, g + o, N1 + o, N2 + o, bc + (long)b * d.nbc, a, f1, f2, t);
+}
+
+void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
+                const double *bcacc, double *out, int L, double h, double dt, double Re,
+                double f1, double f2, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(d.ng, 256), L), dim3(256), 0, s, d, g, N1, N2, bcacc, out,
+                       h, dt, Re, f1, f2);
+}
+
 __global__ void k_fill(double *p, long n, double v) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -429,6 +473,15 @@ __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, cons
                                          A.gbc_scale, ux * hc, uy * hc, hc, t);
 }
 
+__global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *off, int n) {
+    GRP_PROLOGUE(RhsA, A.d.ng)
+    if (t >= d.ng) return;
+    const double hc = A.h * (double)(1L << b);
+    const double a = A.dt / (2 * A.Re * (hc * hc));
+    const long o = b * d.ng;
+    A.out[o + t] = rhs_at(d, A.g + o, A.N1 + o, A.N2 + o, A.bc + (long)b * d.nbc, a, A.f1, A.f2, t);
+}
+
 __global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const BcPrefA A = args[p];
@@ -457,6 +510,7 @@ GRP_LAUNCH(launch_grp_get_bc, kg_get_bc, GetBcA)
 GRP_LAUNCH(launch_grp_copy_apply_bc, kg_copy_apply_bc, CopyBcA)
 GRP_LAUNCH(launch_grp_nonlinear, kg_nonlinear, NonlinA)
 GRP_LAUNCH(launch_grp_bc_prefix, kg_bc_prefix, BcPrefA)
+GRP_LAUNCH(launch_grp_rhs, kg_rhs, RhsA)
 GRP_LAUNCH(launch_grp_fill, kg_fill, FillA)
 
 }  // namespace ibh

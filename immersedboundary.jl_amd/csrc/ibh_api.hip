@@ -152,12 +152,12 @@ GemmArgs gargs() {
     return g;
 }
 
+}  // namespace
+
 // --------------------------------------------------------------------------- transforms
 // LaplacianInv apply (fluid-ops.jl:42-56) on `nbatch` fields: out = S((S·in·S)⊙scale)S where
 // the 16·s factor of two RODFT00 passes lives in the scale table.  The last contraction's
 // epilogue is `epi` (EPI_STORE, EPI_SUBFROM with D = out, or EPI_RHS with the extras in rhs).
-}  // namespace
-
 // The four contractions are TN GEMMs (both operands k-fastest): the left ones read S_m
 // through its symmetry and write their result transposed, so the right ones read it
 // k-fastest too.  Through the odd/even split each costs half the MFMA work of the full one.
@@ -260,7 +260,7 @@ void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, cons
 // L levels of ng (column stride L·ng); Q: per column nq (level 0 only, curl_all = false) or
 // L·nq; bc: per column L·nbc.  Coarse centres of G are overwritten as in the reference.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
-               const int *skip) {
+               const int *skip, int prec = -1) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng, nq = d.nq;
@@ -271,7 +271,8 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     double *bcL = bc + (long)(L - 1) * d.nbc;
     launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
-    lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip);
+    lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
+           prec);
     if (curl_all || L == 1)
         launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0), sQ,
                        nc, skip, c->s);
@@ -279,7 +280,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         double *bcl = bc + (long)lev * d.nbc;
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
         launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
-        lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip);
+        lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, prec);
         if (curl_all || lev == 0)
             launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ, nc,
                            skip, c->s);
@@ -292,12 +293,12 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
-    launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->s);
+    launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
     launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
     for (int col = 0; col < nc; ++col)
         if (L > 1) launch_fill(c->gB + col * L * ng + ng, (L - 1) * ng, 0.0, c->s);
-    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip);
-    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip);
+    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, IBH_FP64);
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, IBH_FP64);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -328,6 +329,10 @@ int ensure_B_work(ibh_ctx *c, int nc) {
     ALLOC(c->qLB, (size_t)nc * d.nq);
     ALLOC(c->bcB, (size_t)nc * L * d.nbc);
     ALLOC(c->zB, (size_t)nc * 2 * c->nb + 1);
+    if (nc > c->etcap) {
+        ALLOC(c->etscale, nc);
+        c->etcap = nc;
+    }
     if ((long)nc * d.ng > c->Tcap) {
         ALLOC(c->T1, (size_t)nc * d.ng);
         ALLOC(c->T2, (size_t)nc * d.ng);
@@ -494,7 +499,9 @@ void seg_trial(ibh_ctx *c) {
     launch_nonlinear(d, c->q, d.nq, c->gam, ng, c->bcraw, d.nbc, 0.25, c->nl[0], ng, 0.0, 0.0, c->h,
                      0, L, c->s, 0, c->uv);
     RhsExtras rx{c->nl[0], c->nl[1], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
-    lapinv(c, c->gam, ng, c->gs, ng, c->sA, ng, L, EPI_RHS, &rx, nullptr);
+    // A_ℓΓ by its exact stencil, fused with the right-hand side terms; then Ainv_ℓ
+    launch_rhs(d, c->gam, rx.N1, rx.N2, rx.bcacc, c->gs, L, c->h, c->dt, c->Re, rx.f1, rx.f2,
+               c->s);
     lapinv(c, c->gs, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
     trial_swap(c);
     vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr);
@@ -541,9 +548,10 @@ void seg_post(ibh_ctx *c) {
     // ---- CircProjecter (timestepping.jl:156-171): Γ = Γs; Γ₁ -= Ainv₁ C' Eᵀ Fb
     post_swap(c);
     if (nf) {
-        launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->s);
+        launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s);
         launch_rot(d, c->qtmp, d.nq, c->g1, ng, 1, nullptr, c->s);
-        lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr);
+        lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
+               IBH_FP64);
     }
     // ---- vort2flux!(qty) (fluid-ops.jl:438)
     vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr);
@@ -774,6 +782,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->fac, L + 1));
     CK(dalloc(c, &c->qtmp, nq));
     CK(dalloc(c, &c->g1, ng));
+    CK(dalloc(c, &c->etscale, 1));
+    c->etcap = 1;
     CK(dalloc(c, &c->T1, L * ng));
     CK(dalloc(c, &c->T2, L * ng));
     c->Tcap = L * ng;
@@ -1100,7 +1110,7 @@ int ibh_op_ET(ibh_ctx *c, const double *fb, double *q) {
     if (!c) return IBH_EINVAL;
     DevBuf B{c};
     double *df = B.up(fb, 2 * c->nb), *dq = B.get(c->d.nq);
-    launch_ET(c->d, c->reg, df, 0, dq, 0, 1, nullptr, c->s);
+    launch_ET(c->d, c->reg, df, 0, dq, 0, 1, nullptr, c->etscale, c->s);
     int rc = down(c, q, dq, c->d.nq);
     if (rc) return rc;
     return check_flags(c);

@@ -92,6 +92,11 @@ void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, l
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
                       hipStream_t s);
 void launch_fill(double *p, long n, double v, hipStream_t s);
+// trial right-hand side of L levels: out = A_ℓ·g + f1·N1 + f2·N2 + edges(bcacc), A_ℓ as the
+// exact 5-point stencil I − a_ℓ(−L), a_ℓ = dt/(2 Re (h·2^ℓ)²)
+void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
+                const double *bcacc, double *out, int L, double h, double dt, double Re,
+                double f1, double f2, hipStream_t s);
 void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+1)(k+1)/n), (n-1)²
 // So[a][k] = sin(π(a+1)(2k+1)/n), Se[a][k] = sin(2π(a+1)(k+1)/n) (0 on the last row/column),
 // both h×h (h = n/2) stored [k + h·a]
@@ -112,9 +117,9 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s,
               const double *uv = nullptr, double uvscale = 1.0);
-// q = Eᵀ fb (q zeroed by the kernel pair)
+// q = Eᵀ fb (zero, scale, fixed-point scatter, conversion); scale: nbatch doubles of work
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
-               int nbatch, const int *skip, hipStream_t s);
+               int nbatch, const int *skip, double *scale, hipStream_t s);
 void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
                  hipStream_t s);
 long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
@@ -180,9 +185,12 @@ struct NonlinA {
     double gbc_scale; double *out; long sout; double h; int lev0; const double *uv; int nbatch;
 };
 struct BcPrefA { int nbc; const double *bcraw; double *acc; const double *fac; int L; };
+struct RhsA {
+    Dom d; const double *g, *N1, *N2, *bc; double *out; double h, dt, Re, f1, f2; int nbatch;
+};
 struct FillA { double *p; long n; double v; };
 struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
-struct ETA { Dom d; RegDev r; const double *fb; double *q; };
+struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; };
 struct GemvA { int n; const double *A; const double *x; double *ws; double *y; };
 struct StressA {
     int nbodies; const int *boff, *npts; int nb; const double *Fb, *ds; double h, dt;
@@ -204,10 +212,13 @@ void launch_grp_get_bc(const Grp &g, hipStream_t s);
 void launch_grp_copy_apply_bc(const Grp &g, hipStream_t s);
 void launch_grp_nonlinear(const Grp &g, hipStream_t s);
 void launch_grp_bc_prefix(const Grp &g, hipStream_t s);   // blocks: ceil(nbc/256)
+void launch_grp_rhs(const Grp &g, hipStream_t s);         // blocks: ceil(nΓ/256)·levels
 void launch_grp_fill(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
 void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
 void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)
+void launch_grp_ET_scale(const Grp &g, hipStream_t s);    // ETA; blocks: 1
 void launch_grp_ET(const Grp &g, hipStream_t s);          // ETA; blocks: ceil(2nb·169/256)
+void launch_grp_ET_fix(const Grp &g, hipStream_t s);      // ETA; blocks: ceil(nq/256)
 void launch_grp_gemv_part(const Grp &g, hipStream_t s);   // blocks: ceil(n/256)·ceil(n/64)
 void launch_grp_gemv_sum(const Grp &g, hipStream_t s);    // blocks: ceil(n/256)
 void launch_grp_stress(const Grp &g, hipStream_t s);      // blocks: nbodies

@@ -118,7 +118,7 @@ def load():
     L.ibh_batch_sync.argtypes = [vp]
     L.ibh_batch_launches_per_step.argtypes = [vp]
     L.ibh_batch_profile_enable.argtypes = [vp, C.c_int32]
-    L.ibh_batch_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64), dp]
+    L.ibh_batch_profile_read.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
     L.ibh_batch_tune_report.argtypes = [vp, C.c_char_p, C.c_int32]
     _lib = L
     return L

@@ -379,9 +379,10 @@ class Batch:
     def profile(self, on: bool):
         self._check(self.lib.ibh_batch_profile_enable(self.h, int(on)))
 
-    def profile_read(self):
+    def profile_read(self, kind: int = 0):
         ms, n, fl = C.c_double(), C.c_int64(), C.c_double()
-        self._check(self.lib.ibh_batch_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(fl)))
+        self._check(self.lib.ibh_batch_profile_read(self.h, kind, C.byref(ms), C.byref(n),
+                                                    C.byref(fl)))
         return ms.value, n.value, fl.value
 
     def tune_report(self) -> list:

@@ -180,7 +180,9 @@ int ibh_batch_sync(ibh_batch *batch);
 /* grouped launches per batch step (one HIP graph replays them) */
 int ibh_batch_launches_per_step(ibh_batch *batch);
 int ibh_batch_profile_enable(ibh_batch *batch, int32_t on);
-int ibh_batch_profile_read(ibh_batch *batch, double *ms, int64_t *launches, double *flops);
+/* kind as in ibh_profile_read: the grouped contraction launches of that precision */
+int ibh_batch_profile_read(ibh_batch *batch, int32_t kind, double *ms, int64_t *launches,
+                           double *flops);
 int ibh_batch_tune_report(ibh_batch *batch, char *buf, int32_t len);
 
 /* ---- profiling (HIP events around each launch of the sine-transform kernels) ----------- */
