@@ -29,7 +29,7 @@ namespace {
 
 enum Op {
     OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
-    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS
+    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS, OP_RESID
 };
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -119,11 +119,45 @@ void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int
     }
 }
 
+// FP64-residual refinement of one finest-level solve per listed case (x -= Solve(M x − rhs),
+// the correction in the batch precision), as many sweeps as each case's `refine`.
+struct RefineItem {
+    int i;
+    double *x;
+    const double *rhs;
+    double a;             // < 0: Poisson, else the implicit CN operator I + a(−L)
+    const double *scale;  // eigenvalue table of the solve
+};
+
+void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineItem> &items) {
+    int maxref = 0;
+    for (const RefineItem &it : items) maxref = std::max(maxref, b->cs[it.i]->refine);
+    for (int k = 0; k < maxref; ++k) {
+        std::vector<ResidA> ra;
+        std::vector<int> bl;
+        std::vector<LapInvPlan> plans;
+        for (const RefineItem &it : items) {
+            ibh_ctx *c = b->cs[it.i];
+            if (c->refine <= k) continue;
+            const Dom &d = c->d;
+            ra.push_back({d, it.x, it.rhs, it.a, c->g1, 1});
+            bl.push_back(nblk(d.ng, GRP_BS));
+            LapInvPlan P;
+            lapinv_plan(c, c->g1, d.ng, it.x, d.ng, it.scale, 0, 1, EPI_SUBFROM, nullptr, nullptr,
+                        &P);
+            plans.push_back(P);
+        }
+        add(v, OP_RESID, ra, bl);
+        add_lapinv(v, plans, b->prec);
+    }
+}
+
 // Vort2Flux (fluid-ops.jl:440-472) of every case: coarsify up, then the Poisson solves from
-// each case's coarsest level down; stage j of the descent is level L-1-j of a case.
+// each case's coarsest level down; stage j of the descent is level L-1-j of a case.  With
+// `refine`, each case's finest-level solve is refined before its curl.
 void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
                    const std::vector<double *> &Psi, const std::vector<double *> &Q,
-                   bool curl_all) {
+                   bool curl_all, bool refine = false) {
     const int n = (int)b->cs.size();
     for (int j = 1; j < b->Lmax; ++j) {
         std::vector<CoarsA> a;
@@ -143,6 +177,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         std::vector<LapInvPlan> plans;
         std::vector<CurlA> cu;
         std::vector<int> cb;
+        std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             const Dom &d = c->d;
@@ -154,6 +189,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             lapinv_plan(c, G[i] + (L - 1) * d.ng, d.ng, Psi[i] + (L - 1) * d.ng, d.ng, c->sD, 0, 1,
                         EPI_STORE, nullptr, nullptr, &P);
             plans.push_back(P);
+            if (refine && L == 1) rf.push_back({i, Psi[i], G[i], -1.0, c->sD});
             if (curl_all || L == 1) {
                 cu.push_back({d, Psi[i] + (L - 1) * d.ng, 0, bcL, 0,
                               Q[i] + (curl_all ? (L - 1) * d.nq : 0), 0, 1});
@@ -162,6 +198,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         }
         add(v, OP_FILL, f, fb);
         add_lapinv(v, plans, b->prec);
+        add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
     }
     for (int j = 1; j < b->Lmax; ++j) {
@@ -170,6 +207,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         std::vector<int> b1, b2, cb;
         std::vector<LapInvPlan> plans;
         std::vector<CurlA> cu;
+        std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             if (c->L <= j) continue;
@@ -184,6 +222,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             lapinv_plan(c, c->gtmp, d.ng, Psi[i] + lev * d.ng, d.ng, c->sD, 0, 1, EPI_STORE,
                         nullptr, nullptr, &P);
             plans.push_back(P);
+            if (refine && lev == 0) rf.push_back({i, Psi[i], c->gtmp, -1.0, c->sD});
             if (curl_all || lev == 0) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
@@ -193,6 +232,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         add(v, OP_GETBC, gb, b1);
         add(v, OP_COPYBC, cp, b2);
         add_lapinv(v, plans, b->prec);
+        add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
     }
 }
@@ -233,6 +273,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         std::vector<RhsA> ra;
         std::vector<int> bl;
         std::vector<LapInvPlan> pI;
+        std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             const Dom &d = c->d;
@@ -240,18 +281,20 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
                          b->uv, c->L});
             bl.push_back(nblk(d.ng, GRP_BS) * c->L);
             // A_ℓΓ as its exact stencil fused with the right-hand side, then Ainv_ℓ
-            ra.push_back({d, gam[i], n0[i], n1[i], c->accbc, gs[i], c->h, c->dt, c->Re,
+            ra.push_back({d, gam[i], n0[i], n1[i], c->accbc, c->rhsb, c->h, c->dt, c->Re,
                           c->dt * c->beta[0], c->dt * c->beta[1], c->L});
             LapInvPlan P2;
-            lapinv_plan(c, gs[i], d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
+            lapinv_plan(c, c->rhsb, d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
                         nullptr, &P2);
             pI.push_back(P2);
+            rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv});
         }
         add(v, OP_NONLIN, a, bl);
         add(v, OP_RHS, ra, bl);
         add_lapinv(v, pI, b->prec);
+        add_refine(b, v, rf);
     }
-    add_vort2flux(b, v, gs, psi, qs, false);
+    add_vort2flux(b, v, gs, psi, qs, false, true);
     // ---- BoundaryForcer (timestepping.jl:106-120): Fb = Binv·E(qs + q0)
     {
         std::vector<EA> e;
@@ -304,7 +347,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         add(v, OP_ROT, ro, b4);
         add_lapinv(v, plans, IBH_FP64);  // the projection stays FP64
     }
-    add_vort2flux(b, v, gs, psi, q, true);
+    add_vort2flux(b, v, gs, psi, q, true, true);
 }
 
 // copy every stage's tables into one device arena
@@ -390,6 +433,7 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant) {
     case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
     case OP_STRESS: launch_grp_stress(g, s); break;
     case OP_RHS: launch_grp_rhs(g, s); break;
+    case OP_RESID: launch_grp_resid(g, s); break;
     }
 }
 

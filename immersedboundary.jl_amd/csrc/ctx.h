@@ -60,6 +60,8 @@ struct ibh_ctx {
     double *gtmp = nullptr, *fac = nullptr, *qtmp = nullptr, *g1 = nullptr;
     double *T1 = nullptr, *T2 = nullptr;
     long Tcap = 0;  // elements in T1/T2
+    double *rhsb = nullptr;  // trial right-hand side, L levels
+    int refine = 0;          // FP64-residual refinement sweeps of the finest-level trial solves
     // B-product work (batched over nc columns)
     int ncB = 0;
     double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,

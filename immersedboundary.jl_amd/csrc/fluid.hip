@@ -348,6 +348,30 @@ void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N
                        h, dt, Re, f1, f2);
 }
 
+// FP64 residual of a finest-level solve for iterative refinement: r = M·x − rhs with the
+// 5-point operator that the sine transform inverts exactly (zero values beyond the edges):
+// M = −L (Poisson, a < 0) or M = I + a(−L) (the implicit CN operator, a ≥ 0).
+__device__ inline double resid_at(const Dom &d, const double *__restrict__ x,
+                                  const double *__restrict__ rhs, double a, long t) {
+    const int i = (int)(t % d.m), j = (int)(t / d.m);
+    const double c = x[t];
+    const double nsum = (i > 0 ? x[t - 1] : 0.0) + (i < d.m - 1 ? x[t + 1] : 0.0) +
+                        (j > 0 ? x[t - d.m] : 0.0) + (j < d.p - 1 ? x[t + d.m] : 0.0);
+    const double lx = 4.0 * c - nsum;  // (−L)x
+    return (a < 0 ? lx : c + a * lx) - rhs[t];
+}
+
+__global__ void k_resid(Dom d, const double *__restrict__ x, const double *__restrict__ rhs,
+                        double a, double *__restrict__ r) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < d.ng) r[t] = resid_at(d, x, rhs, a, t);
+}
+
+void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double *r,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_resid, dim3(nblk(d.ng, 256)), dim3(256), 0, s, d, x, rhs, a, r);
+}
+
 __global__ void k_fill(double *p, long n, double v) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -482,6 +506,12 @@ __global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *of
     A.out[o + t] = rhs_at(d, A.g + o, A.N1 + o, A.N2 + o, A.bc + (long)b * d.nbc, a, A.f1, A.f2, t);
 }
 
+__global__ __launch_bounds__(GRP_BS) void kg_resid(const ResidA *args, const int *off, int n) {
+    GRP_PROLOGUE(ResidA, A.d.ng)
+    if (t >= d.ng) return;
+    A.r[t] = resid_at(d, A.x, A.rhs, A.a, t);
+}
+
 __global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const BcPrefA A = args[p];
@@ -511,6 +541,7 @@ GRP_LAUNCH(launch_grp_copy_apply_bc, kg_copy_apply_bc, CopyBcA)
 GRP_LAUNCH(launch_grp_nonlinear, kg_nonlinear, NonlinA)
 GRP_LAUNCH(launch_grp_bc_prefix, kg_bc_prefix, BcPrefA)
 GRP_LAUNCH(launch_grp_rhs, kg_rhs, RhsA)
+GRP_LAUNCH(launch_grp_resid, kg_resid, ResidA)
 GRP_LAUNCH(launch_grp_fill, kg_fill, FillA)
 
 }  // namespace ibh

@@ -14,6 +14,8 @@
 #include <tuple>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/ibh.h"
 #include "ctx.h"
 #include "ibh_internal.h"
@@ -259,8 +261,18 @@ void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, cons
 // Vort2Flux (fluid-ops.jl:440-472) on nc independent multilevel fields.  G, Psi: per column
 // L levels of ng (column stride L·ng); Q: per column nq (level 0 only, curl_all = false) or
 // L·nq; bc: per column L·nbc.  Coarse centres of G are overwritten as in the reference.
+// `refine` sweeps of FP64-residual refinement of the finest-level Poisson solve (single column):
+// ψ₁ -= Δinv(−Lψ₁ − rhs), the correction in the transform precision.
+void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const double *scale,
+                   int sweeps, int prec) {
+    for (int k = 0; k < sweeps; ++k) {
+        launch_resid(c->d, x, rhs, a, c->g1, c->s);
+        lapinv(c, c->g1, c->d.ng, x, c->d.ng, scale, 0, 1, EPI_SUBFROM, nullptr, nullptr, prec);
+    }
+}
+
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
-               const int *skip, int prec = -1) {
+               const int *skip, int prec = -1, int refine = 0) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng, nq = d.nq;
@@ -273,6 +285,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
     lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
            prec);
+    if (L == 1 && refine) refine_sweeps(c, Psi, G, -1.0, c->sD, refine, prec);
     if (curl_all || L == 1)
         launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0), sQ,
                        nc, skip, c->s);
@@ -281,6 +294,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
         launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
         lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, prec);
+        if (lev == 0 && refine) refine_sweeps(c, Psi, c->gtmp, -1.0, c->sD, refine, prec);
         if (curl_all || lev == 0)
             launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ, nc,
                            skip, c->s);
@@ -371,8 +385,46 @@ int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt) {
 }
 
 // Inv = inverse(A) by Gauss–Jordan on [A | I]; A is destroyed
+void dfree(ibh_ctx *c, void *p);
+
+// Inv = inverse(A) through LAPACK's getrf + getri as rocSOLVER implements them (the routines
+// behind inv(Bmat), cnab/ops.jl:96): blocked, MFMA-backed, for systems where Gauss–Jordan's
+// one HBM pass per pivot is too slow.  A is destroyed.
+int invert_lapack(ibh_ctx *c, double *A, double *Inv) {
+    const int nf = 2 * c->nb;
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+        return fail(c, IBH_EDEVICE, "rocblas_create_handle failed");
+    rocblas_set_stream(h, c->s);
+    rocblas_int *piv = nullptr, *info = nullptr;
+    int rc = dalloc(c, &piv, (size_t)nf);
+    if (!rc) rc = dalloc(c, &info, 2);
+    rocblas_status st = rocblas_status_success;
+    if (!rc) st = rocsolver_dgetrf(h, nf, nf, A, nf, piv, info);
+    if (!rc && st == rocblas_status_success) st = rocsolver_dgetri(h, nf, A, nf, piv, info + 1);
+    int hinfo[2] = {0, 0};
+    if (!rc && st == rocblas_status_success) {
+        launch_copy(Inv, A, (long)nf * nf, c->s);
+        if (hipMemcpyAsync(hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+            hipStreamSynchronize(c->s) != hipSuccess)
+            rc = fail(c, IBH_EDEVICE, "rocSOLVER inverse: device error");
+    }
+    rocblas_destroy_handle(h);
+    dfree(c, piv);
+    dfree(c, info);
+    if (rc) return rc;
+    if (st != rocblas_status_success) return fail(c, IBH_EDEVICE, "rocSOLVER getrf/getri failed");
+    if (hinfo[0] || hinfo[1]) return fail(c, IBH_ESINGULAR, "singular B matrix");
+    return IBH_OK;
+}
+
+// Inv = inverse(A) by Gauss–Jordan with partial pivoting (up to 2048 unknowns) or rocSOLVER's
+// LU (larger systems, or IBH_DENSE_INV=lapack); A is destroyed
 int invert(ibh_ctx *c, double *A, double *Inv) {
     const int nf = 2 * c->nb;
+    bool lapack = nf > 2048;
+    if (const char *e = std::getenv("IBH_DENSE_INV")) lapack = std::strcmp(e, "lapack") == 0;
+    if (lapack) return invert_lapack(c, A, Inv);
     hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
                        c->s, Inv, nf, 0, nf);
     if (!c->gjws) ALLOC(c->gjws, (size_t)2 * (nf + 4) + 8);
@@ -500,11 +552,17 @@ void seg_trial(ibh_ctx *c) {
                      0, L, c->s, 0, c->uv);
     RhsExtras rx{c->nl[0], c->nl[1], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
     // A_ℓΓ by its exact stencil, fused with the right-hand side terms; then Ainv_ℓ
-    launch_rhs(d, c->gam, rx.N1, rx.N2, rx.bcacc, c->gs, L, c->h, c->dt, c->Re, rx.f1, rx.f2,
+    launch_rhs(d, c->gam, rx.N1, rx.N2, rx.bcacc, c->rhsb, L, c->h, c->dt, c->Re, rx.f1, rx.f2,
                c->s);
-    lapinv(c, c->gs, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
+    lapinv(c, c->rhsb, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
+    // low-precision transforms: the finest level's two solves are refined against their FP64
+    // residuals (the forces see the trial flux at the body through Binv, which amplifies
+    // high-frequency noise there; coarser levels reach it only through smooth boundary values)
+    if (c->refine)
+        refine_sweeps(c, c->gs, c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, c->refine,
+                      c->precision);
     trial_swap(c);
-    vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr);
+    vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr, -1, c->refine);
     // ---- BoundaryForcer right-hand side (timestepping.jl:106-139): E(qs + q0) [- v·h]
     const int nf = 2 * c->nb;
     if (nf) {
@@ -553,8 +611,9 @@ void seg_post(ibh_ctx *c) {
         lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
                IBH_FP64);
     }
-    // ---- vort2flux!(qty) (fluid-ops.jl:438)
-    vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr);
+    // ---- vort2flux!(qty) (fluid-ops.jl:438); the finest level is refined like the trial's (its
+    // flux feeds the next step's nonlinear term next to the bodies)
+    vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr, -1, c->refine);
 }
 
 // status word: [CG done, CG iterations, error flags]
@@ -730,6 +789,10 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     c->use_pcg = D.force_solver == IBH_FORCE_PCG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
     c->use_cg = D.force_solver == IBH_FORCE_CG || c->use_pcg;
     c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : 50;
+    if (D.precision != IBH_FP64)
+        c->refine = D.refine_sweeps > 0 ? D.refine_sweeps
+                    : D.refine_sweeps < 0 ? 0 : (D.precision == IBH_FP32 ? 1 : 2);
+    if (const char *rf = std::getenv("IBH_REFINE")) c->refine = std::max(0, std::atoi(rf));
     {
         // unknowns of moving bodies, as contiguous runs in [fx | fy]
         for (int comp = 0; comp < 2; ++comp)
@@ -784,6 +847,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->g1, ng));
     CK(dalloc(c, &c->etscale, 1));
     c->etcap = 1;
+    CK(dalloc(c, &c->rhsb, L * ng));
     CK(dalloc(c, &c->T1, L * ng));
     CK(dalloc(c, &c->T2, L * ng));
     c->Tcap = L * ng;

@@ -97,6 +97,9 @@ void launch_fill(double *p, long n, double v, hipStream_t s);
 void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
                 const double *bcacc, double *out, int L, double h, double dt, double Re,
                 double f1, double f2, hipStream_t s);
+// r = M·x − rhs (FP64) with M = −L (a < 0) or I + a(−L), zero values beyond the edges
+void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double *r,
+                  hipStream_t s);
 void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+1)(k+1)/n), (n-1)²
 // So[a][k] = sin(π(a+1)(2k+1)/n), Se[a][k] = sin(2π(a+1)(k+1)/n) (0 on the last row/column),
 // both h×h (h = n/2) stored [k + h·a]
@@ -188,6 +191,7 @@ struct BcPrefA { int nbc; const double *bcraw; double *acc; const double *fac; i
 struct RhsA {
     Dom d; const double *g, *N1, *N2, *bc; double *out; double h, dt, Re, f1, f2; int nbatch;
 };
+struct ResidA { Dom d; const double *x, *rhs; double a; double *r; int nbatch; };
 struct FillA { double *p; long n; double v; };
 struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
 struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; };
@@ -213,6 +217,7 @@ void launch_grp_copy_apply_bc(const Grp &g, hipStream_t s);
 void launch_grp_nonlinear(const Grp &g, hipStream_t s);
 void launch_grp_bc_prefix(const Grp &g, hipStream_t s);   // blocks: ceil(nbc/256)
 void launch_grp_rhs(const Grp &g, hipStream_t s);         // blocks: ceil(nΓ/256)·levels
+void launch_grp_resid(const Grp &g, hipStream_t s);       // blocks: ceil(nΓ/256)
 void launch_grp_fill(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
 void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
 void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)

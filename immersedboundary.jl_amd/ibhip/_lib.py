@@ -55,7 +55,7 @@ class CaseDesc(C.Structure):
         ("xb", C.POINTER(C.c_double)), ("ds", C.POINTER(C.c_double)),
         ("precision", C.c_int32), ("force_solver", C.c_int32),
         ("cg_maxiter", C.c_int32), ("cg_reltol", C.c_double),
-        ("pc_refresh", C.c_int32),
+        ("pc_refresh", C.c_int32), ("refine_sweeps", C.c_int32),
     ]
 
 

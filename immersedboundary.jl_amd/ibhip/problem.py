@@ -218,7 +218,9 @@ class SchemeCNAB:
     otherwise; "dense"; "cg": the reference's unpreconditioned cg!; "pcg": cg! with the dense
     inverse of B as preconditioner, rebuilt for moving bodies every `pc_refresh` steps).
     cg_maxiter / cg_reltol are the reference's cg! settings (cnab/ops.jl:110); every CG
-    variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.
+    variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.  `refine_sweeps` (FP32 /
+    bf16x3 only): FP64-residual refinement sweeps of the two finest-level trial solves; 0 picks
+    1 for FP32 and 2 for bf16x3, a negative value turns refinement off.
     """
 
     dt: float
@@ -228,6 +230,7 @@ class SchemeCNAB:
     cg_reltol: float = 1e-12
     pc_refresh: int = 50
     beta: tuple = (1.5, -0.5)
+    refine_sweeps: int = 0
 
     def __post_init__(self):
         if self.precision not in PRECISIONS:

@@ -86,6 +86,10 @@ typedef struct ibh_case_desc {
     int32_t cg_maxiter;     /* ≤0 → 5000 (cnab/ops.jl:110)                                */
     double cg_reltol;       /* ≤0 → 1e-12 (cnab/ops.jl:110)                               */
     int32_t pc_refresh;     /* PCG: steps between preconditioner rebuilds (≤0 → 50)       */
+    int32_t refine_sweeps;  /* FP32/BF16X3: iterative-refinement sweeps of the two finest-
+                               level trial solves (Ainv₁ and the level-1 Poisson solve), each
+                               an FP64 5-point residual plus a low-precision correction;
+                               0 → 1 for FP32 and 2 for BF16X3, < 0 → none.  FP64: unused. */
 } ibh_case_desc;
 
 typedef struct ibh_ctx ibh_ctx;

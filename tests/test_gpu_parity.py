@@ -218,6 +218,46 @@ def test_dst2_mixed_precision(ibhip, precision, tol):
     assert err < tol, err
 
 
+def test_dense_inverse_lapack_path_matches_oracle(ibhip, monkeypatch):
+    # systems above 2048 unknowns are inverted with rocSOLVER's getrf/getri; forced here on a
+    # small case and held to the same 1e-10 as the Gauss–Jordan path
+    monkeypatch.setenv("IBH_DENSE_INV", "lapack")
+    prob = cylinder_case(n=64, nlev=3)
+    state, orc, hist = run_pair(ibhip, prob, 6)
+    assert_state_close(state.case, orc, 1e-10)
+
+
+def cd_history(prob, nsteps):
+    import ibhip as m
+
+    st = m.initstate(prob)
+    sv = m.initsolver(prob, st)
+    out = []
+    for k in range(nsteps):
+        m.advance_(st, sv, k * prob.scheme.dt)
+        out.append(st.case.CD()[0])
+    st.case.close()
+    return np.array(out)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mixed_refinement_tightens_forces(ibhip, precision):
+    # the FP64-residual refinement of the finest-level solves removes most of the
+    # low-precision noise that Binv amplifies into the forces
+    from ibhip.problem import SchemeCNAB
+
+    base = cylinder_case(n=128, nlev=3)
+    ref = cd_history(base, 12)
+
+    def run(sweeps):
+        p = cylinder_case(n=128, nlev=3, precision=precision)
+        p.scheme = SchemeCNAB(p.scheme.dt, precision=precision, refine_sweeps=sweeps)
+        return np.abs(cd_history(p, 12)[1:] - ref[1:]).max() / np.abs(ref[1:]).max()
+
+    e_off, e_on = run(-1), run(0)
+    assert e_on < e_off / 5, (e_off, e_on)
+
+
 def test_gemm_tile_variants_identical(ibhip, monkeypatch):
     # every tile variant sums k in the same order, so the autotuner's choice cannot change
     # the numbers
