@@ -80,6 +80,9 @@ struct ibh_ctx {
     // PCG preconditioner: dense inverse of the assembled B (moving columns refreshed)
     bool use_pcg = false, pc_valid = false;
     int pc_refresh = 50, steps_since_pc = 0;
+    bool pc_adaptive = false;            // rebuild by cost (pc_refresh ≤ 0 in the description)
+    double pc_t_refresh = 0, pc_t_iter = 0, pc_extra = 0;  // seconds
+    int pc_base = 1 << 30;               // fewest iterations since the last rebuild
     double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
     int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
     std::vector<int> J;                      // unknowns of moving bodies
@@ -99,6 +102,7 @@ struct ibh_ctx {
     // GEMM autotuner: chosen variant per shape, scratch output, log of the choices
     std::map<ibh::TuneKey, int> tuned;
     int tune_fixed = -1;
+    bool tune_verbose = false;  // log every candidate's time (IBH_TUNE_VERBOSE)
     double *tune_buf = nullptr;
     size_t tune_cap = 0;
     std::vector<std::string> tune_log;

@@ -9,7 +9,8 @@
 // Tiles are staged in LDS as [row][k].  The output is written column-major (x fastest) or
 // transposed, by choosing which staged tile feeds the MFMA's A operand, so that lane%16
 // always walks the contiguous index of C.  Block tile BM×BN, K tile BK, WM×WN waves; K tiles
-// are double-buffered in LDS with the next tile prefetched into registers.
+// are staged in one or two LDS buffers with the next one or two tiles prefetched into
+// registers.
 //
 // Precision policies:
 //   FP64   v_mfma_f64_16x16x4f64,  D row = lane/16 + 4r
@@ -32,7 +33,9 @@ struct PF64 {
     typedef double E;    // LDS element
     typedef d4 Acc;
     static constexpr int KS = 4;      // K per MFMA
-    static constexpr int PAD = 4;     // LDS row padding (elements)
+    // LDS row padding (elements): an odd row stride in doubles; measured with rocprofv3 on the
+    // 1023² contraction, padding 1 has 13× fewer bank-conflict cycles than 4 (8 is worse still)
+    static constexpr int PAD = 1;
     __device__ static Acc zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
     __device__ static int drow(int lq, int r) { return lq + 4 * r; }
 };
@@ -292,8 +295,12 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
     const int wm = wave % WM, wn = wave / WM;
     const int l16 = lane & 15, lq = lane >> 4;
 
-    __shared__ Tile<P, BM, BK> AsO[NB], AsE[NB];
-    __shared__ Tile<P, BN, BK> BsO[NB], BsE[NB];
+    // NB = 1: one LDS buffer; 2: two buffers, the next K tile prefetched into registers;
+    // 3: two buffers and two register stages, so global loads are issued two K tiles ahead
+    constexpr int NBUF = NB == 1 ? 1 : 2;
+    constexpr int NREG = NB == 3 ? 2 : 1;
+    __shared__ Tile<P, BM, BK> AsO[NBUF], AsE[NBUF];
+    __shared__ Tile<P, BN, BK> BsO[NBUF], BsE[NBUF];
 
     typename P::Acc accO[FN][FM], accE[FN][FM];
 #pragma unroll
@@ -301,53 +308,46 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
 #pragma unroll
         for (int b = 0; b < FM; ++b) accO[a][b] = accE[a][b] = P::zero();
 
-    double rso[LS], rse[LS], rd[LD];
-    auto gload = [&](int k0) {
+    double rso[NREG][LS], rse[NREG][LS], rd[NREG][LD];
+    auto gload = [&](int k0, int st) {
 #pragma unroll
         for (int u = 0; u < LS; ++u) {
             const int e = tid + NT * u, r = r0s + e / BK, k = k0 + e % BK;
             const bool in = r < h && k < h;
-            rso[u] = in ? g.So[k + (long)h * r] : 0.0;
-            rse[u] = in ? g.Se[k + (long)h * r] : 0.0;
+            rso[st][u] = in ? g.So[k + (long)h * r] : 0.0;
+            rse[st][u] = in ? g.Se[k + (long)h * r] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < LD; ++u) {
             const int e = tid + NT * u, r = r0d + e / (2 * BK), q = 2 * k0 + e % (2 * BK);
-            rd[u] = (r < nrd && q < Kd) ? X[q + ldx * (long)r] : 0.0;
+            rd[st][u] = (r < nrd && q < Kd) ? X[q + ldx * (long)r] : 0.0;
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int st) {
 #pragma unroll
         for (int u = 0; u < LS; ++u) {
             const int e = tid + NT * u;
             if (SIDE == 0) {
-                tput(AsO[buf], e / BK, e % BK, rso[u]);
-                tput(AsE[buf], e / BK, e % BK, rse[u]);
+                tput(AsO[buf], e / BK, e % BK, rso[st][u]);
+                tput(AsE[buf], e / BK, e % BK, rse[st][u]);
             } else {
-                tput(BsO[buf], e / BK, e % BK, rso[u]);
-                tput(BsE[buf], e / BK, e % BK, rse[u]);
+                tput(BsO[buf], e / BK, e % BK, rso[st][u]);
+                tput(BsE[buf], e / BK, e % BK, rse[st][u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < LD; ++u) {
             const int e = tid + NT * u, r = e / (2 * BK), q = e % (2 * BK);
             if (SIDE == 0) {
-                if (q & 1) tput(BsE[buf], r, q >> 1, rd[u]);
-                else tput(BsO[buf], r, q >> 1, rd[u]);
+                if (q & 1) tput(BsE[buf], r, q >> 1, rd[st][u]);
+                else tput(BsO[buf], r, q >> 1, rd[st][u]);
             } else {
-                if (q & 1) tput(AsE[buf], r, q >> 1, rd[u]);
-                else tput(AsO[buf], r, q >> 1, rd[u]);
+                if (q & 1) tput(AsE[buf], r, q >> 1, rd[st][u]);
+                else tput(AsO[buf], r, q >> 1, rd[st][u]);
             }
         }
     };
-
-    const int nk = (h + BK - 1) / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = NB == 2 ? (kt & 1) : 0;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < BK / KS; ++kk) {
             Frag<P> fa[FM], fb[FN];
@@ -374,14 +374,45 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
                     else accE[j][i] = mma(fb[j], fa[i], accE[j][i]);
                 }
         }
-        if (NB == 2) {
-            if (kt + 1 < nk) sstore(cur ^ 1);
+    };
+
+    const int nk = (h + BK - 1) / BK;
+    if (NB == 3) {
+        // tile kt computes from buffer kt&1 while tile kt+1 waits in one register stage and
+        // the loads of tile kt+2 land in the other; the loop is unrolled by two so that the
+        // register stages are compile-time indices
+        gload(0, 0);
+        sstore(0, 0);
+        if (nk > 1) gload(BK, 0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            if (kt + 2 < nk) gload((kt + 2) * BK, NREG - 1);
+            compute(0);
+            if (kt + 1 < nk) sstore(1, 0);
             __syncthreads();
-        } else {  // one LDS buffer: wait for every wave to finish reading before refilling
+            if (kt + 1 >= nk) break;
+            if (kt + 3 < nk) gload((kt + 3) * BK, 0);
+            compute(1);
+            if (kt + 2 < nk) sstore(0, NREG - 1);
             __syncthreads();
-            if (kt + 1 < nk) {
-                sstore(0);
+        }
+    } else {
+        gload(0, 0);
+        sstore(0, 0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int cur = NB == 2 ? (kt & 1) : 0;
+            if (kt + 1 < nk) gload((kt + 1) * BK, 0);
+            compute(cur);
+            if (NB == 2) {
+                if (kt + 1 < nk) sstore(cur ^ 1, 0);
                 __syncthreads();
+            } else {  // one LDS buffer: wait for every wave to finish reading before refilling
+                __syncthreads();
+                if (kt + 1 < nk) {
+                    sstore(0, 0);
+                    __syncthreads();
+                }
             }
         }
     }
@@ -544,16 +575,21 @@ static const VariantEO EF64[] = {
     EOV(PF64, 32, 32, 16, 2, 2, 1), EOV(PF64, 64, 64, 8, 2, 2, 1),  EOV(PF64, 32, 64, 8, 2, 2, 1),
     EOV(PF64, 64, 64, 16, 2, 4, 1), EOV(PF64, 64, 64, 16, 4, 2, 1), EOV(PF64, 128, 64, 16, 4, 2, 1),
     EOV(PF64, 64, 128, 16, 2, 4, 1),
+    // two-stage register prefetch (loads issued two K tiles ahead)
+    EOV(PF64, 32, 64, 32, 2, 2, 3), EOV(PF64, 64, 32, 32, 2, 2, 3), EOV(PF64, 32, 64, 16, 2, 2, 3),
+    EOV(PF64, 64, 64, 16, 2, 2, 3), EOV(PF64, 32, 32, 32, 2, 2, 3),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
     EOV(PF32, 64, 64, 32, 2, 2, 2), EOV(PF32, 64, 64, 16, 2, 2, 1), EOV(PF32, 32, 64, 16, 2, 2, 1),
     EOV(PF32, 64, 32, 16, 2, 2, 1), EOV(PF32, 64, 64, 32, 2, 2, 1),
+    EOV(PF32, 32, 64, 32, 2, 2, 3), EOV(PF32, 64, 64, 32, 2, 2, 3),
 };
 static const VariantEO EBF3[] = {
     EOV(PBF3, 64, 64, 32, 2, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2, 2),
     EOV(PBF3, 32, 64, 32, 1, 2, 2), EOV(PBF3, 64, 64, 32, 2, 2, 1), EOV(PBF3, 128, 64, 32, 2, 2, 1),
     EOV(PBF3, 64, 128, 32, 2, 2, 1), EOV(PBF3, 128, 128, 32, 2, 2, 1),
+    EOV(PBF3, 64, 64, 32, 2, 2, 3),
 };
 
 static const VariantEO *table_eo(int prec, int *n) {
@@ -603,15 +639,15 @@ struct VariantGEO {
 static const VariantGEO GF64[] = {
     GEOV(PF64, 64, 64, 16, 2, 2, 1), GEOV(PF64, 32, 64, 16, 2, 2, 1), GEOV(PF64, 64, 32, 16, 2, 2, 1),
     GEOV(PF64, 32, 64, 32, 2, 2, 2), GEOV(PF64, 64, 32, 32, 2, 2, 2), GEOV(PF64, 64, 64, 8, 2, 2, 1),
-    GEOV(PF64, 32, 32, 16, 2, 2, 1),
+    GEOV(PF64, 32, 32, 16, 2, 2, 1), GEOV(PF64, 64, 64, 16, 2, 2, 3), GEOV(PF64, 32, 64, 32, 2, 2, 3),
 };
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),
-    GEOV(PF32, 64, 32, 16, 2, 2, 1),
+    GEOV(PF32, 64, 32, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 3),
 };
 static const VariantGEO GBF3[] = {
     GEOV(PBF3, 64, 64, 32, 2, 2, 1), GEOV(PBF3, 128, 64, 32, 2, 2, 1), GEOV(PBF3, 64, 128, 32, 2, 2, 1),
-    GEOV(PBF3, 64, 64, 32, 2, 2, 2),
+    GEOV(PBF3, 64, 64, 32, 2, 2, 2), GEOV(PBF3, 64, 64, 32, 2, 2, 3),
 };
 
 static const VariantGEO *table_geo(int prec, int *n) {

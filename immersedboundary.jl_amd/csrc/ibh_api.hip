@@ -4,6 +4,8 @@
 // advance! (cnab/timestepping.jl:253-290); the independent per-level work of GetTrialState
 // (timestepping.jl:28-79) is batched over levels in single launches, and the Schur-complement
 // products B·z (cnab/ops.jl:125-143) are batched over columns for the dense-inverse setup.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +29,10 @@ namespace {
 int fail(ibh_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 #define HIPCK(expr)                                                                   \
@@ -113,6 +119,10 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
             best_ms = ms;
             best = v;
         }
+        if (c->tune_verbose)
+            c->tune_log.push_back(std::string("  candidate ") +
+                                  (kind == 0 ? gemm_variant_name(prec, v) : gemm_eo_variant_name(prec, v)) +
+                                  " " + std::to_string(ms / 3 * 1e3) + "us");
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -682,10 +692,23 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     int rc;
     if (refresh) {  // eager: the preconditioner needs this step's regularization
         seg_trial(c);
+        HIPCK(hipStreamSynchronize(c->s));
+        const bool first = !c->pc_valid;
+        const double t0 = now_s();
         if ((rc = refresh_pc(c))) return rc;
+        // the adaptive rule weighs later refreshes (moving bodies' columns only)
+        const double nf = 2.0 * c->nb;
+        c->pc_t_refresh = (now_s() - t0) * (first ? c->J.size() / std::max(nf, 1.0) : 1.0);
+        c->pc_extra = 0.0;
+        c->pc_base = 1 << 30;
         if ((rc = ensure_B_work(c, 1))) return rc;
+        const double t1 = now_s();
         seg_solve_begin(c);
         if (c->use_cg) seg_cg_iters(c, k1);
+        int s0[3];
+        if ((rc = read_status(c, s0))) return rc;
+        // one B-product per iteration plus the initial residual
+        c->pc_t_iter = (now_s() - t1) / (std::min(s0[1], k1) + 1);
     } else {
         if (c->use_cg && (rc = ensure_B_work(c, 1))) return rc;
         rc = run_seg(c, &c->gTrial[par], g, [&] {
@@ -705,6 +728,14 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
         c->cg_last = st[1];
         if (c->use_pcg) {
             ++c->steps_since_pc;
+            if (c->pc_adaptive) {
+                // rent or buy: rebuild once the iterations spent above the post-rebuild
+                // minimum have cost as much as the last rebuild (at most twice the best
+                // fixed interval's cost for a steadily drifting B)
+                c->pc_base = std::min(c->pc_base, st[1]);
+                c->pc_extra += (st[1] - c->pc_base) * c->pc_t_iter;
+                if (c->pc_extra >= c->pc_t_refresh) c->steps_since_pc = c->pc_refresh;
+            }
             // a preconditioner that has drifted far from B is rebuilt at the next step
             if (st[1] > 24) c->steps_since_pc = c->pc_refresh;
         }
@@ -785,10 +816,12 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     c->nb = nb;
     c->precision = D.precision;
     if (const char *fv = std::getenv("IBH_GEMM_VARIANT")) c->tune_fixed = std::atoi(fv);
+    if (const char *tv = std::getenv("IBH_TUNE_VERBOSE")) c->tune_verbose = std::atoi(tv) != 0;
     c->force_solver = D.force_solver;
     c->use_pcg = D.force_solver == IBH_FORCE_PCG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
     c->use_cg = D.force_solver == IBH_FORCE_CG || c->use_pcg;
-    c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : 50;
+    c->pc_adaptive = D.pc_refresh <= 0;
+    c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : (1 << 30);
     if (D.precision != IBH_FP64)
         c->refine = D.refine_sweeps > 0 ? D.refine_sweeps
                     : D.refine_sweeps < 0 ? 0 : (D.precision == IBH_FP32 ? 1 : 2);

@@ -216,7 +216,8 @@ class SchemeCNAB:
     `precision` of the bulk sine transforms ("fp64", "fp32", "bf16x3") and the surface-force
     solver ("auto": the precomputed dense inverse for bodies fixed in the grid, PCG
     otherwise; "dense"; "cg": the reference's unpreconditioned cg!; "pcg": cg! with the dense
-    inverse of B as preconditioner, rebuilt for moving bodies every `pc_refresh` steps).
+    inverse of B as preconditioner, rebuilt for moving bodies every `pc_refresh` steps, or
+    when its drift has cost as much as a rebuild for pc_refresh = 0).
     cg_maxiter / cg_reltol are the reference's cg! settings (cnab/ops.jl:110); every CG
     variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.  `refine_sweeps` (FP32 /
     bf16x3 only): FP64-residual refinement sweeps of the two finest-level trial solves; 0 picks
@@ -228,7 +229,7 @@ class SchemeCNAB:
     force_solver: str = "auto"
     cg_maxiter: int = 5000
     cg_reltol: float = 1e-12
-    pc_refresh: int = 50
+    pc_refresh: int = 0
     beta: tuple = (1.5, -0.5)
     refine_sweeps: int = 0
 

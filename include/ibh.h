@@ -46,7 +46,7 @@ enum ibh_force_solver {
     IBH_FORCE_DENSE = 1, /* Binv_linearmap dense branch, cnab/ops.jl:79-102                  */
     IBH_FORCE_CG = 2,    /* Binv_linearmap CG branch, cnab/ops.jl:104-114                    */
     IBH_FORCE_PCG = 3    /* the CG branch with a preconditioner: the dense inverse of B,     */
-                         /* rebuilt for the moving bodies' columns every pc_refresh steps;  */
+                         /* rebuilt for the moving bodies' columns (see pc_refresh);        */
                          /* same stopping rule (‖b−Bx‖ ≤ reltol·‖b−Bx₀‖), far fewer products */
 };
 
@@ -85,7 +85,9 @@ typedef struct ibh_case_desc {
     int32_t force_solver;   /* enum ibh_force_solver                                      */
     int32_t cg_maxiter;     /* ≤0 → 5000 (cnab/ops.jl:110)                                */
     double cg_reltol;       /* ≤0 → 1e-12 (cnab/ops.jl:110)                               */
-    int32_t pc_refresh;     /* PCG: steps between preconditioner rebuilds (≤0 → 50)       */
+    int32_t pc_refresh;     /* PCG: steps between preconditioner rebuilds; ≤0: rebuild when
+                               the iterations spent above the post-rebuild minimum have cost
+                               as much as the last rebuild (timed on the device)          */
     int32_t refine_sweeps;  /* FP32/BF16X3: iterative-refinement sweeps of the two finest-
                                level trial solves (Ainv₁ and the level-1 Poisson solve), each
                                an FP64 5-point residual plus a low-precision correction;
