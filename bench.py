@@ -14,6 +14,12 @@ launch time.  `roofline.traffic` is the per-launch HBM byte count measured for t
 by rocprofv3 counter runs (profiles/contraction_traffic.json), or null when none was measured.
 The CPU baseline runs the FP64 oracle (a C restatement of the reference, OpenMP) on a bounded
 sample of the same workload.
+
+The default run also measures the metric's other side on the batched sweep (`ellipse_batch`:
+64 independent cases of 128²–512² cells advanced together): the mixed-precision batched step
+(FP32 transforms, FP64 forces) and the FP64 batched step, with the mixed run's lift/drag error
+against FP64, under config.mixed_precision_batched (skip with --no-batch).  `--workload
+ellipse_batch` or `cylinder_array` (2048², 16 cylinders) bench those configurations alone.
 """
 import argparse
 import json
@@ -164,18 +170,18 @@ def batch_lift_drag_error(ibhip, make, nsteps):
             "step1_CD_rel": float((np.abs(cd1[0] - cd0[0]) / np.abs(cd0[0])).max())}
 
 
-def run_batch(args, torch, ibhip):
+def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accuracy=True):
     """The batched form: the ellipse sweep (64 independent cases of 128²–512² cells, 2–5
     levels) advanced together, one grouped launch per substep, the step replayed from a HIP
     graph.  A step is one advance! of every case; cell-steps/s counts all cases."""
     from ibhip.workloads import ellipse_batch
 
-    make = lambda **kw: ellipse_batch(**{"precision": args.precision, **kw})  # noqa: E731
+    make = lambda **kw: ellipse_batch(**{"precision": precision, **kw})  # noqa: E731
     probs = make()
     cells = sum(p.fluid.nx * p.fluid.ny * p.fluid.nlevels for p in probs)
     npts = sum(p.bodies.npoints for p in probs)
     log(f"bench: ellipse_batch {len(probs)} cases, {cells} cells/step, {npts} points, "
-        f"{args.precision}")
+        f"{precision}")
     t0 = time.perf_counter()
     states = [ibhip.initstate(p) for p in probs]
     for p, st in zip(probs, states):
@@ -184,17 +190,17 @@ def run_batch(args, torch, ibhip):
     batch.sync()
     t_setup = time.perf_counter() - t0
     log(f"bench: setup {t_setup:.2f} s")
-    warm = max(args.warmup, 3)  # step 1 tunes, steps 2 and 3 capture the two parity graphs
+    warm = max(warmup, 3)  # step 1 tunes, steps 2 and 3 capture the two parity graphs
     for _ in range(warm):
         batch.advance()
     batch.sync()
     marker = torch.zeros(1, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(steps):
         batch.advance()
         if (s + 1) % 25 == 0:
-            log(f"bench: {s + 1}/{args.steps} steps")
+            log(f"bench: {s + 1}/{steps} steps")
     batch.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -202,67 +208,104 @@ def run_batch(args, torch, ibhip):
     torch.cuda.synchronize()
     batch.profile(True)
     t1 = time.perf_counter()
-    for _ in range(args.profile_steps):
+    for _ in range(profile_steps):
         batch.advance()
     batch.sync()
     prof_elapsed = time.perf_counter() - t1
-    ms, launches, flops = batch.profile_read({"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision])
+    ms, launches, flops = batch.profile_read({"fp64": 0, "fp32": 1, "bf16x3": 2}[precision])
     batch.profile(False)
     launches_per_step = batch.launches_per_step()
     tune = batch.tune_report()
     batch.close()
     for st in states:
         st.case.close()
-    accuracy = None
-    if args.precision != "fp64":
-        accuracy = batch_lift_drag_error(ibhip, make, 20)
-        log(f"bench: lift/drag error vs FP64 {accuracy}")
-    value = cells * args.steps / elapsed
+    acc = None
+    if accuracy and precision != "fp64":
+        acc = batch_lift_drag_error(ibhip, make, 20)
+        log(f"bench: lift/drag error vs FP64 {acc}")
     avg_launch_s = (ms / 1e3) / max(launches, 1)
     achieved = (flops / max(launches, 1)) / avg_launch_s if launches else 0.0
-    peak = PEAK[args.precision]
-    out = {
-        "metric": "cell-steps/s: FP64 vs mixed-precision batched, at ≤1e-6 rel lift/drag err",
-        "value": value,
-        "unit": "cell-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": warm,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 transforms / f64 forces",
-                  "bf16x3": "bf16x3 transforms / f64 forces"}[args.precision],
-        "data": "synthetic",
-        "config": {"workload": (f"ellipse_batch: {len(probs)} independent cases, grids "
-                                f"128²–512², 2–5 levels, one fixed ellipse each ({npts} points "
-                                f"in all), Re 50–400, dt = 0.2h; {cells} cells per step"),
-                   "cases": len(probs), "cells_per_step": cells, "body_points": npts,
-                   "precision": args.precision, "setup_s": t_setup,
-                   "grouped_launches_per_step": launches_per_step,
-                   "lift_drag_error_vs_fp64": accuracy,
-                   "autotuned_contractions": tune},
+    return {
+        "probs": probs, "cells": cells, "npts": npts, "warmup": warm, "elapsed": elapsed,
+        "value": cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+        "setup_s": t_setup, "launches_per_step": launches_per_step, "tune": tune,
+        "accuracy": acc,
         "roofline": {
             "bound": "mfma",
             "achieved": achieved / 1e12,
-            "peak": peak / 1e12,
+            "peak": PEAK[precision] / 1e12,
             "unit": "TFLOP/s",
-            "frac": achieved / peak,
+            "frac": achieved / PEAK[precision],
             "traffic": None,
-            "kernel": f"kg_gemm_eo<{args.precision}> (grouped sine-transform contraction)",
+            "kernel": f"kg_gemm_eo<{precision}> (grouped sine-transform contraction)",
             "avg_launch_us": avg_launch_s * 1e6,
             "launches": launches,
             "gemm_share_of_profiled_steps": (ms / 1e3) / prof_elapsed,
             "timing": "HIP events around each grouped contraction launch on the batch stream, "
                       "over a profiled pass of eager steps right after the timed (graph) region",
         },
+    }
+
+
+DTYPE = {"fp64": "f64", "fp32": "f32 transforms / f64 forces",
+         "bf16x3": "bf16x3 transforms / f64 forces"}
+
+
+def run_batch(args, torch, ibhip):
+    m = measure_batch(torch, ibhip, args.precision, args.steps, args.warmup, args.profile_steps)
+    out = {
+        "metric": "cell-steps/s: FP64 vs mixed-precision batched, at ≤1e-6 rel lift/drag err",
+        "value": m["value"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": m["warmup"],
+        "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic",
+        "config": {"workload": (f"ellipse_batch: {len(m['probs'])} independent cases, grids "
+                                f"128²–512², 2–5 levels, one fixed ellipse each ({m['npts']} "
+                                f"points in all), Re 50–400, dt = 0.2h; {m['cells']} cells per "
+                                f"step"),
+                   "cases": len(m["probs"]), "cells_per_step": m["cells"],
+                   "body_points": m["npts"], "precision": args.precision,
+                   "setup_s": m["setup_s"], "grouped_launches_per_step": m["launches_per_step"],
+                   "lift_drag_error_vs_fp64": m["accuracy"],
+                   "autotuned_contractions": m["tune"]},
+        "roofline": m["roofline"],
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = batch_cpu_baseline(probs)
+        out["cpu_baseline"] = batch_cpu_baseline(m["probs"])
         log("bench: cpu baseline done")
     print(json.dumps(out))
+
+
+def batched_summary(torch, ibhip, args, plain_value):
+    """The metric's comparison: the mixed-precision (FP32 transforms, FP64 forces) batched
+    step and the FP64 batched step on the 64-case sweep, beside the plain FP64 single-case
+    step measured above."""
+    m32 = measure_batch(torch, ibhip, "fp32", args.batch_steps, 3, 2)
+    m64 = measure_batch(torch, ibhip, "fp64", args.batch_steps, 3, 2, accuracy=False)
+    return {
+        "workload": (f"ellipse_batch: {len(m32['probs'])} cases, 128²–512², 2–5 levels, "
+                     f"{m32['cells']} cells per step"),
+        "steps": args.batch_steps,
+        "fp32_cell_steps_per_s": m32["value"],
+        "fp64_cell_steps_per_s": m64["value"],
+        "fp32_ms_per_step": m32["ms_per_step"],
+        "fp64_ms_per_step": m64["ms_per_step"],
+        "fp32_vs_fp64_batched": m32["value"] / m64["value"],
+        "fp32_batched_vs_plain_fp64_step": m32["value"] / plain_value,
+        "fp32_lift_drag_error_vs_fp64": m32["accuracy"],
+        "fp32_contraction_roofline": {k: m32["roofline"][k] for k in
+                                      ("achieved", "peak", "frac", "avg_launch_us")},
+        "fp64_contraction_roofline": {k: m64["roofline"][k] for k in
+                                      ("achieved", "peak", "frac", "avg_launch_us")},
+    }
 
 
 def main():
@@ -277,6 +320,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=5,
                     help="steps of the profiled (eager) pass that times the contraction kernel")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched (64-case) FP32/FP64 comparison of the default run")
+    ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one elementwise copy of 2^25 doubles first (256 MiB read + 256 MiB "
                          "written) so that profiler byte counters can be scaled")
@@ -420,6 +466,9 @@ def main():
         log(f"bench: reference cg! iterations on step 1: {n_ref}")
         out["cpu_baseline"] = cpu_baseline(make, n_ref)
         log("bench: cpu baseline done")
+    if not args.no_batch and args.workload == "cylinder_plate" and args.precision == "fp64":
+        out["config"]["mixed_precision_batched"] = batched_summary(torch, ibhip, args, value)
+        log("bench: batched comparison done")
     print(json.dumps(out))
 
 
