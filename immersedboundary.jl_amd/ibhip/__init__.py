@@ -12,7 +12,7 @@ from .problem import (  # noqa: F401
 from .cnab import (  # noqa: F401
     Batch, BodyStress, DeviceCase, Quantities, Solver, State, StateCallback, StateResult,
     StopIteration, advance_, at_indices, at_times, each_timestep, initsolver, initstate,
-    quantities, solve, solve_,
+    load_state, quantities, save_state, solve, solve_,
 )
 from ._lib import IbhError, load as load_library  # noqa: F401
 
@@ -21,5 +21,6 @@ __all__ = [
     "RelativeFrame", "RigidBody", "SchemeCNAB", "StreamFcnFluid", "UniformGrid", "circle",
     "timesteps", "Batch", "BodyStress", "DeviceCase", "Quantities", "Solver", "State", "StateCallback",
     "StateResult", "StopIteration", "advance_", "at_indices", "at_times", "each_timestep",
-    "initsolver", "initstate", "quantities", "solve", "solve_", "IbhError", "load_library",
+    "initsolver", "initstate", "load_state", "quantities", "save_state", "solve", "solve_",
+    "IbhError", "load_library",
 ]

@@ -326,6 +326,39 @@ def advance_(state: State, solver: Solver, t: float) -> None:
     state.t = float(t)
 
 
+SNAPSHOT_FIELDS = (("gamma", _lib.F_GAMMA), ("psi", _lib.F_PSI), ("q", _lib.F_Q),
+                   ("nonlin1", _lib.F_NONLIN1), ("nonlin2", _lib.F_NONLIN2), ("Fb", _lib.F_FB))
+
+
+def save_state(state: State, path) -> None:
+    """Write a snapshot of a CNAB state to an .npz file: Γ, ψ, q (one column per level, the
+    reference's matrices, streamfcn/states.jl:9-17), nonlin[1:2] in cycle order
+    (cnab/states.jl:1-6), Fb, q0 and t, plus the grid shape for checking on load."""
+    c = state.case
+    np.savez(path, gamma=c.gamma(), psi=c.psi(), q=c.q(), q0=c.q0(), nonlin1=c.nonlin(0),
+             nonlin2=c.nonlin(1), Fb=c.Fb(), t=np.float64(state.t),
+             shape=np.array([c.nx, c.ny, c.nlev, c.nb], dtype=np.int64))
+
+
+def load_state(state: State, path) -> None:
+    """Restore a snapshot written by save_state into a state of the same problem shape (the
+    device fields are overwritten; the solver's regularization and force solver are kept)."""
+    c = state.case
+    with np.load(path) as d:
+        shape = tuple(int(v) for v in d["shape"])
+        if shape != (c.nx, c.ny, c.nlev, c.nb):
+            raise ValueError(f"snapshot of shape {shape} does not fit a case of shape "
+                             f"{(c.nx, c.ny, c.nlev, c.nb)}")
+        for name, which in SNAPSHOT_FIELDS:
+            a = np.asarray(d[name], dtype=np.float64)
+            if name == "Fb":
+                if c.nb:
+                    c.set_field(which, a)
+                continue
+            c.set_field(which, a.T.reshape(-1))  # levels stacked, as on the device
+        state.t = float(d["t"])
+
+
 class Batch:
     """Independent cases advanced together: every substep of advance! is one grouped launch
     over all cases (each with its own grid, levels, Re, dt and bodies) and the whole step is
