@@ -37,6 +37,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, precision):
+    """HBM bytes per contraction launch measured by rocprofv3 for this workload, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "contraction_traffic.json")) as f:
+            return json.load(f).get(f"{workload}/{precision}", {}).get("bytes_per_launch")
+    except OSError:
+        return None
+
+
 def cpu_baseline(make_problem, n_ref_iters, cap=8):
     """Time the oracle on one step of the workload with cg! capped at `cap` iterations and on
     two lone B-products; extrapolate the step to `n_ref_iters` iterations of the reference's
@@ -248,7 +257,7 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
             "peak": PEAK[precision] / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK[precision],
-            "traffic": None,
+            "traffic": measured_traffic("ellipse_batch", precision),
             "kernel": f"kg_gemm_eo<{precision}> (grouped sine-transform contraction)",
             "avg_launch_us": avg_launch_s * 1e6,
             "launches": launches,
@@ -419,13 +428,7 @@ def main():
     avg_launch_s = (ms / 1e3) / max(launches, 1)
     achieved = (flops / max(launches, 1)) / avg_launch_s if launches else 0.0
     peak = PEAK["fp64"] if args.precision == "fp64" else PEAK[args.precision]
-    traffic = None
-    try:
-        with open(os.path.join(ROOT, "profiles", "contraction_traffic.json")) as f:
-            traffic = json.load(f).get(f"{args.workload}/{args.precision}", {}).get(
-                "bytes_per_launch")
-    except OSError:
-        pass
+    traffic = measured_traffic(args.workload, args.precision)
     roofline = {
         "bound": "mfma",
         "achieved": achieved / 1e12,
