@@ -81,6 +81,7 @@ struct ibh_ctx {
     bool use_pcg = false, pc_valid = false;
     int pc_refresh = 50, steps_since_pc = 0;
     bool pc_adaptive = false;            // rebuild by cost (pc_refresh ≤ 0 in the description)
+    int pc_prec = 1;                     // transform precision of the preconditioner's columns
     double pc_t_refresh = 0, pc_t_iter = 0, pc_extra = 0;  // seconds
     int pc_base = 1 << 30;               // fewest iterations since the last rebuild
     double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;

@@ -312,8 +312,10 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
 }
 
 // x = B·z for nc columns (B_Times, cnab/ops.jl:125-143): Γ = 0; Γ₁ = Ainv₁ C' Eᵀ z;
-// vort2flux; x = E q₁.  z, x: column stride sz, sx.
-void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip) {
+// vort2flux; x = E q₁.  z, x: column stride sz, sx.  The transforms run in FP64 (the
+// Schur-complement side of the step) unless `prec` asks otherwise (preconditioner columns).
+void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,
+             int prec = IBH_FP64) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
@@ -321,8 +323,8 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
     for (int col = 0; col < nc; ++col)
         if (L > 1) launch_fill(c->gB + col * L * ng + ng, (L - 1) * ng, 0.0, c->s);
-    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, IBH_FP64);
-    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, IBH_FP64);
+    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -376,7 +378,7 @@ __global__ void k_unit_cols(double *Z, int nf, int j0, int nc) {
 }
 
 // Bm[:, j0:j0+cnt] = B·e_j (unit-vector products in column batches, cnab/ops.jl:88-94)
-int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt) {
+int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64) {
     const int nf = 2 * c->nb;
     // column batch sized to keep the B-product work under ~12 GB
     const Dom &d = c->d;
@@ -389,7 +391,7 @@ int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt) {
         const int n = std::min(nc, j0 + cnt - j);
         hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * n + 255) / 256)), dim3(256), 0,
                            c->s, c->zB, nf, j, n);
-        B_times(c, c->zB, nf, Bm + (long)nf * j, nf, n, nullptr);
+        B_times(c, c->zB, nf, Bm + (long)nf * j, nf, n, nullptr, prec);
     }
     return IBH_OK;
 }
@@ -512,11 +514,11 @@ int refresh_pc(ibh_ctx *c) {
         HIPCK(hipStreamSynchronize(c->s));
     }
     if (!c->pc_valid) {
-        rc = B_columns(c, c->Bfull, 0, nf);
+        rc = B_columns(c, c->Bfull, 0, nf, c->pc_prec);
         if (rc) return rc;
     } else {
         for (auto &r : c->Jrange) {
-            rc = B_columns(c, c->Bfull, r.first, r.second);
+            rc = B_columns(c, c->Bfull, r.first, r.second, c->pc_prec);
             if (rc) return rc;
         }
         launch_mirror_rows(nf, c->Bfull, c->dJ, (int)c->J.size(), c->dinJ, c->s);
@@ -821,6 +823,10 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     c->use_pcg = D.force_solver == IBH_FORCE_PCG || (D.force_solver == IBH_FORCE_AUTO && c->anymove);
     c->use_cg = D.force_solver == IBH_FORCE_CG || c->use_pcg;
     c->pc_adaptive = D.pc_refresh <= 0;
+    // the PCG preconditioner only approximates B⁻¹, so its columns use FP32 transforms (PCG
+    // still applies the FP64 B); IBH_PC_PREC=0/1/2 picks another precision
+    c->pc_prec = IBH_FP32;
+    if (const char *pp = std::getenv("IBH_PC_PREC")) c->pc_prec = std::atoi(pp);
     c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : (1 << 30);
     if (D.precision != IBH_FP64)
         c->refine = D.refine_sweeps > 0 ? D.refine_sweeps
