@@ -119,14 +119,17 @@ void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int
     }
 }
 
-// FP64-residual refinement of one finest-level solve per listed case (x -= Solve(M x − rhs),
-// the correction in the batch precision), as many sweeps as each case's `refine`.
+// FP64-residual refinement of the solves on nlev consecutive levels per listed case
+// (x -= Solve(M x − rhs), the correction in the batch precision), as many sweeps as each
+// case's `refine`.
 struct RefineItem {
     int i;
     double *x;
     const double *rhs;
-    double a;             // < 0: Poisson, else the implicit CN operator I + a(−L)
-    const double *scale;  // eigenvalue table of the solve
+    double a;             // < 0: Poisson, else the implicit CN operator I + (a/4^ℓ)(−L)
+    const double *scale;  // eigenvalue tables of the solves (level stride sscale)
+    long sscale;
+    int nlev;
 };
 
 void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineItem> &items) {
@@ -140,11 +143,11 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
             ibh_ctx *c = b->cs[it.i];
             if (c->refine <= k) continue;
             const Dom &d = c->d;
-            ra.push_back({d, it.x, it.rhs, it.a, c->g1, 1});
-            bl.push_back(nblk(d.ng, GRP_BS));
+            ra.push_back({d, it.x, it.rhs, it.a, it.a < 0 ? 1.0 : 0.25, c->rres, it.nlev});
+            bl.push_back(nblk(d.ng, GRP_BS) * it.nlev);
             LapInvPlan P;
-            lapinv_plan(c, c->g1, d.ng, it.x, d.ng, it.scale, 0, 1, EPI_SUBFROM, nullptr, nullptr,
-                        &P);
+            lapinv_plan(c, c->rres, d.ng, it.x, d.ng, it.scale, it.sscale, it.nlev, EPI_SUBFROM,
+                        nullptr, nullptr, &P);
             plans.push_back(P);
         }
         add(v, OP_RESID, ra, bl);
@@ -154,7 +157,7 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
 
 // Vort2Flux (fluid-ops.jl:440-472) of every case: coarsify up, then the Poisson solves from
 // each case's coarsest level down; stage j of the descent is level L-1-j of a case.  With
-// `refine`, each case's finest-level solve is refined before its curl.
+// `refine`, the solves on each case's finest refine_levels levels are refined before their curl.
 void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
                    const std::vector<double *> &Psi, const std::vector<double *> &Q,
                    bool curl_all, bool refine = false) {
@@ -189,7 +192,9 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             lapinv_plan(c, G[i] + (L - 1) * d.ng, d.ng, Psi[i] + (L - 1) * d.ng, d.ng, c->sD, 0, 1,
                         EPI_STORE, nullptr, nullptr, &P);
             plans.push_back(P);
-            if (refine && L == 1) rf.push_back({i, Psi[i], G[i], -1.0, c->sD});
+            if (refine && L - 1 < c->refine_levels)
+                rf.push_back({i, Psi[i] + (L - 1) * d.ng, G[i] + (L - 1) * d.ng, -1.0, c->sD, 0,
+                              1});
             if (curl_all || L == 1) {
                 cu.push_back({d, Psi[i] + (L - 1) * d.ng, 0, bcL, 0,
                               Q[i] + (curl_all ? (L - 1) * d.nq : 0), 0, 1});
@@ -222,7 +227,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             lapinv_plan(c, c->gtmp, d.ng, Psi[i] + lev * d.ng, d.ng, c->sD, 0, 1, EPI_STORE,
                         nullptr, nullptr, &P);
             plans.push_back(P);
-            if (refine && lev == 0) rf.push_back({i, Psi[i], c->gtmp, -1.0, c->sD});
+            if (refine && lev < c->refine_levels)
+                rf.push_back({i, Psi[i] + lev * d.ng, c->gtmp, -1.0, c->sD, 0, 1});
             if (curl_all || lev == 0) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
@@ -287,7 +293,8 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             lapinv_plan(c, c->rhsb, d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
                         nullptr, &P2);
             pI.push_back(P2);
-            rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv});
+            rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, d.ng,
+                          std::min(c->refine_levels, c->L)});
         }
         add(v, OP_NONLIN, a, bl);
         add(v, OP_RHS, ra, bl);

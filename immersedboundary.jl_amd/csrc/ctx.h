@@ -61,7 +61,9 @@ struct ibh_ctx {
     double *T1 = nullptr, *T2 = nullptr;
     long Tcap = 0;  // elements in T1/T2
     double *rhsb = nullptr;  // trial right-hand side, L levels
-    int refine = 0;          // FP64-residual refinement sweeps of the finest-level trial solves
+    int refine = 0;          // FP64-residual refinement sweeps of the low-precision solves
+    int refine_levels = 1;   // ... on the finest refine_levels levels
+    double *rres = nullptr;  // refinement residuals, L levels
     // B-product work (batched over nc columns)
     int ncB = 0;
     double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,

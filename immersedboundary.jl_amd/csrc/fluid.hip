@@ -361,15 +361,23 @@ __device__ inline double resid_at(const Dom &d, const double *__restrict__ x,
     return (a < 0 ? lx : c + a * lx) - rhs[t];
 }
 
-__global__ void k_resid(Dom d, const double *__restrict__ x, const double *__restrict__ rhs,
-                        double a, double *__restrict__ r) {
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t < d.ng) r[t] = resid_at(d, x, rhs, a, t);
+// operator coefficient of level b: a·ar^b (a < 0, the Poisson marker, is kept as is)
+__device__ inline double level_coef(double a, double ar, int b) {
+    for (int k = 0; k < b; ++k) a *= ar;
+    return a;
 }
 
-void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double *r,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_resid, dim3(nblk(d.ng, 256)), dim3(256), 0, s, d, x, rhs, a, r);
+__global__ void k_resid(Dom d, const double *__restrict__ x, const double *__restrict__ rhs,
+                        double a, double ar, double *__restrict__ r) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long o = (long)blockIdx.y * d.ng;
+    if (t < d.ng) r[o + t] = resid_at(d, x + o, rhs + o, level_coef(a, ar, blockIdx.y), t);
+}
+
+void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double ar,
+                  double *r, int nlev, hipStream_t s) {
+    hipLaunchKernelGGL(k_resid, dim3(nblk(d.ng, 256), nlev), dim3(256), 0, s, d, x, rhs,
+                       a < 0 ? -1.0 : a, a < 0 ? 1.0 : ar, r);
 }
 
 __global__ void k_fill(double *p, long n, double v) {
@@ -509,7 +517,8 @@ __global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *of
 __global__ __launch_bounds__(GRP_BS) void kg_resid(const ResidA *args, const int *off, int n) {
     GRP_PROLOGUE(ResidA, A.d.ng)
     if (t >= d.ng) return;
-    A.r[t] = resid_at(d, A.x, A.rhs, A.a, t);
+    const long o = (long)b * d.ng;
+    A.r[o + t] = resid_at(d, A.x + o, A.rhs + o, level_coef(A.a, A.ar, b), t);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {

@@ -271,13 +271,15 @@ void lapinv(ibh_ctx *c, const double *in, long sin, double *out, long sout, cons
 // Vort2Flux (fluid-ops.jl:440-472) on nc independent multilevel fields.  G, Psi: per column
 // L levels of ng (column stride L·ng); Q: per column nq (level 0 only, curl_all = false) or
 // L·nq; bc: per column L·nbc.  Coarse centres of G are overwritten as in the reference.
-// `refine` sweeps of FP64-residual refinement of the finest-level Poisson solve (single column):
-// ψ₁ -= Δinv(−Lψ₁ − rhs), the correction in the transform precision.
+// `sweeps` sweeps of FP64-residual refinement of the solves M_ℓ x_ℓ = rhs_ℓ on nlev consecutive
+// levels (single column): x -= Solve(M x − rhs), the correction in the transform precision.
+// M = −L (a < 0, scale table sD) or I + a_ℓ(−L) with a_ℓ = a/4^ℓ (per-level tables, stride sscale).
 void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const double *scale,
-                   int sweeps, int prec) {
+                   long sscale, int nlev, int sweeps, int prec) {
+    const long ng = c->d.ng;
     for (int k = 0; k < sweeps; ++k) {
-        launch_resid(c->d, x, rhs, a, c->g1, c->s);
-        lapinv(c, c->g1, c->d.ng, x, c->d.ng, scale, 0, 1, EPI_SUBFROM, nullptr, nullptr, prec);
+        launch_resid(c->d, x, rhs, a, 0.25, c->rres, nlev, c->s);
+        lapinv(c, c->rres, ng, x, ng, scale, sscale, nlev, EPI_SUBFROM, nullptr, nullptr, prec);
     }
 }
 
@@ -295,7 +297,8 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
     lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
            prec);
-    if (L == 1 && refine) refine_sweeps(c, Psi, G, -1.0, c->sD, refine, prec);
+    if (refine && L - 1 < c->refine_levels)
+        refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
     if (curl_all || L == 1)
         launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0), sQ,
                        nc, skip, c->s);
@@ -304,7 +307,8 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
         launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
         lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, prec);
-        if (lev == 0 && refine) refine_sweeps(c, Psi, c->gtmp, -1.0, c->sD, refine, prec);
+        if (refine && lev < c->refine_levels)
+            refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine, prec);
         if (curl_all || lev == 0)
             launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ, nc,
                            skip, c->s);
@@ -571,8 +575,8 @@ void seg_trial(ibh_ctx *c) {
     // residuals (the forces see the trial flux at the body through Binv, which amplifies
     // high-frequency noise there; coarser levels reach it only through smooth boundary values)
     if (c->refine)
-        refine_sweeps(c, c->gs, c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, c->refine,
-                      c->precision);
+        refine_sweeps(c, c->gs, c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, ng,
+                      std::min(c->refine_levels, L), c->refine, c->precision);
     trial_swap(c);
     vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr, -1, c->refine);
     // ---- BoundaryForcer right-hand side (timestepping.jl:106-139): E(qs + q0) [- v·h]
@@ -832,6 +836,9 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         c->refine = D.refine_sweeps > 0 ? D.refine_sweeps
                     : D.refine_sweeps < 0 ? 0 : (D.precision == IBH_FP32 ? 1 : 2);
     if (const char *rf = std::getenv("IBH_REFINE")) c->refine = std::max(0, std::atoi(rf));
+    c->refine_levels = D.refine_levels > 0 ? std::min(D.refine_levels, c->L) : c->L;
+    if (const char *rl = std::getenv("IBH_REFINE_LEVELS"))
+        c->refine_levels = std::min(std::max(1, std::atoi(rl)), c->L);
     {
         // unknowns of moving bodies, as contiguous runs in [fx | fy]
         for (int comp = 0; comp < 2; ++comp)
@@ -887,6 +894,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->etscale, 1));
     c->etcap = 1;
     CK(dalloc(c, &c->rhsb, L * ng));
+    if (c->refine) CK(dalloc(c, &c->rres, L * ng));
     CK(dalloc(c, &c->T1, L * ng));
     CK(dalloc(c, &c->T2, L * ng));
     c->Tcap = L * ng;

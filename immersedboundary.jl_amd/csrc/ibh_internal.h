@@ -97,9 +97,10 @@ void launch_fill(double *p, long n, double v, hipStream_t s);
 void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
                 const double *bcacc, double *out, int L, double h, double dt, double Re,
                 double f1, double f2, hipStream_t s);
-// r = M·x − rhs (FP64) with M = −L (a < 0) or I + a(−L), zero values beyond the edges
-void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double *r,
-                  hipStream_t s);
+// r = M·x − rhs (FP64) with M = −L (a < 0) or I + a(−L), zero values beyond the edges, on
+// nlev consecutive levels (stride ng); level b uses a·ar^b (ar = 1/4 for the CN operator)
+void launch_resid(const Dom &d, const double *x, const double *rhs, double a, double ar,
+                  double *r, int nlev, hipStream_t s);
 void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+1)(k+1)/n), (n-1)²
 // So[a][k] = sin(π(a+1)(2k+1)/n), Se[a][k] = sin(2π(a+1)(k+1)/n) (0 on the last row/column),
 // both h×h (h = n/2) stored [k + h·a]
@@ -191,7 +192,7 @@ struct BcPrefA { int nbc; const double *bcraw; double *acc; const double *fac; i
 struct RhsA {
     Dom d; const double *g, *N1, *N2, *bc; double *out; double h, dt, Re, f1, f2; int nbatch;
 };
-struct ResidA { Dom d; const double *x, *rhs; double a; double *r; int nbatch; };
+struct ResidA { Dom d; const double *x, *rhs; double a, ar; double *r; int nbatch; };
 struct FillA { double *p; long n; double v; };
 struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
 struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; };

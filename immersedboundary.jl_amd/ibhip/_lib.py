@@ -56,6 +56,7 @@ class CaseDesc(C.Structure):
         ("precision", C.c_int32), ("force_solver", C.c_int32),
         ("cg_maxiter", C.c_int32), ("cg_reltol", C.c_double),
         ("pc_refresh", C.c_int32), ("refine_sweeps", C.c_int32),
+        ("refine_levels", C.c_int32),
     ]
 
 

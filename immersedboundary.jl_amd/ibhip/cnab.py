@@ -59,6 +59,7 @@ class DeviceCase:
         d.cg_reltol = sch.cg_reltol
         d.pc_refresh = sch.pc_refresh
         d.refine_sweeps = sch.refine_sweeps
+        d.refine_levels = sch.refine_levels
         self._desc = d
         h = C.c_void_p()
         rc = L.ibh_create(C.byref(d), C.byref(h))

@@ -220,8 +220,9 @@ class SchemeCNAB:
     when its drift has cost as much as a rebuild for pc_refresh = 0).
     cg_maxiter / cg_reltol are the reference's cg! settings (cnab/ops.jl:110); every CG
     variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.  `refine_sweeps` (FP32 /
-    bf16x3 only): FP64-residual refinement sweeps of the two finest-level trial solves; 0 picks
-    1 for FP32 and 2 for bf16x3, a negative value turns refinement off.
+    bf16x3 only): FP64-residual refinement sweeps of every low-precision solve (trial Ainv_ℓ,
+    both Vort2Flux Poisson solves) on the finest `refine_levels` levels (0: every level);
+    refine_sweeps = 0 picks 1 for FP32 and 2 for bf16x3, a negative value turns refinement off.
     """
 
     dt: float
@@ -232,6 +233,7 @@ class SchemeCNAB:
     pc_refresh: int = 0
     beta: tuple = (1.5, -0.5)
     refine_sweeps: int = 0
+    refine_levels: int = 0
 
     def __post_init__(self):
         if self.precision not in PRECISIONS:

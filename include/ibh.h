@@ -88,10 +88,13 @@ typedef struct ibh_case_desc {
     int32_t pc_refresh;     /* PCG: steps between preconditioner rebuilds; ≤0: rebuild when
                                the iterations spent above the post-rebuild minimum have cost
                                as much as the last rebuild (timed on the device)          */
-    int32_t refine_sweeps;  /* FP32/BF16X3: iterative-refinement sweeps of the two finest-
-                               level trial solves (Ainv₁ and the level-1 Poisson solve), each
-                               an FP64 5-point residual plus a low-precision correction;
+    int32_t refine_sweeps;  /* FP32/BF16X3: iterative-refinement sweeps of every solve in
+                               the transform precision on the levels refine_levels selects
+                               (trial Ainv_ℓ and both Vort2Flux Poisson solves), each an FP64
+                               5-point residual plus a low-precision correction;
                                0 → 1 for FP32 and 2 for BF16X3, < 0 → none.  FP64: unused. */
+    int32_t refine_levels;  /* FP32/BF16X3: refine the solves of the finest refine_levels
+                               levels; ≤ 0 → every level.  FP64: unused.                   */
 } ibh_case_desc;
 
 typedef struct ibh_ctx ibh_ctx;
