@@ -64,6 +64,7 @@ struct ibh_batch {
     std::vector<double *> gam0, gs0, nl00, nl10;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool use_graphs = true, tuned = false;
+    int tune_fixed = -1;  // grouped contraction variant forced for every stage (IBH_GRP_VARIANT)
     std::vector<std::string> tune_log;
     ibh::Prof prof;
     std::string err;
@@ -481,7 +482,9 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
     for (size_t k = 0; k < v.size(); ++k) {
         Stage &S = v[k];
         if (S.op == OP_GEMM && S.variant < 0) {
-            if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
+            if (b->tune_fixed >= 0 && b->tune_fixed < gemm_eo_grp_variant_count(S.prec))
+                S.variant = b->tune_fixed;
+            else if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
             else if (may_tune) {
                 int rc = tune_stage(b, S);
                 if (rc) return rc;
@@ -595,6 +598,7 @@ int ibh_batch_create(int32_t n, ibh_ctx *const *cases, ibh_batch **out) {
     }
     b->prec = cases[0]->precision;
     if (const char *ge = std::getenv("IBH_GRAPH")) b->use_graphs = std::atoi(ge) != 0;
+    if (const char *gv = std::getenv("IBH_GRP_VARIANT")) b->tune_fixed = std::atoi(gv);
     if (hipDeviceSynchronize() != hipSuccess || hipStreamCreate(&b->s) != hipSuccess ||
         hipMalloc(&b->uv, 2 * sizeof(double)) != hipSuccess) {
         ibh_batch_destroy(b);

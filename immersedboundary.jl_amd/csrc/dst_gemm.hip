@@ -270,9 +270,15 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
 // X as the B operand); SIDE 1: C = X·S (pairs on the N index, data as the A operand).  The
 // data operand is read as 2·BK contiguous k values and split by parity on the LDS store; the
 // butterfly runs in the epilogue.
-template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB>
+// WS = 2 splits the parts over the waves: WM×WN waves compute O and as many compute E from the
+// same LDS tiles (twice the resident waves for latency hiding when the grid is small); the E
+// waves hand their accumulators to the O waves through LDS for the butterfly.  Each part's
+// sums run in the same order either way.
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB,
+          int WS = 1>
 __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int bz) {
-    constexpr int NT = WM * WN * 64;
+    constexpr int NW = WM * WN;              // waves per part
+    constexpr int NT = NW * 64 * WS;
     constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
     constexpr int KS = P::KS;
@@ -292,7 +298,8 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
     const int r0s = SIDE == 0 ? m0 : n0, r0d = SIDE == 0 ? n0 : m0;
     const int nrd = SIDE == 0 ? N : M;       // valid data rows
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % WM, wn = wave / WM;
+    const int part = WS == 2 ? wave / NW : 0;  // 0: O (or both parts), 1: E
+    const int wl = wave % NW, wm = wl % WM, wn = wl / WM;
     const int l16 = lane & 15, lq = lane >> 4;
 
     // NB = 1: one LDS buffer; 2: two buffers, the next K tile prefetched into registers;
@@ -350,6 +357,26 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
         }
     };
     auto compute = [&](int cur) {
+        if (WS == 2) {  // one part per wave: its accumulators live in accO
+            const Tile<P, BM, BK> &Ta = part ? AsE[cur] : AsO[cur];
+            const Tile<P, BN, BK> &Tb = part ? BsE[cur] : BsO[cur];
+#pragma unroll
+            for (int kk = 0; kk < BK / KS; ++kk) {
+                Frag<P> fa[FM], fb[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) fload(fa[i], Ta, wm * TM + i * 16 + l16, kk, lq);
+#pragma unroll
+                for (int j = 0; j < FN; ++j) fload(fb[j], Tb, wn * TN + j * 16 + l16, kk, lq);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int i = 0; i < FM; ++i) {
+                        if (TOUT) accO[j][i] = mma(fa[i], fb[j], accO[j][i]);
+                        else accO[j][i] = mma(fb[j], fa[i], accO[j][i]);
+                    }
+            }
+            return;
+        }
 #pragma unroll
         for (int kk = 0; kk < BK / KS; ++kk) {
             Frag<P> fa[FM], fb[FN];
@@ -419,6 +446,24 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
         }
     }
 
+    // WS = 2: the E waves pass their accumulators to the O waves (same wave-local index and
+    // lane), then leave.  Every wave of the group reaches this barrier exactly once.
+    constexpr int NX = WS == 2 ? NW * FN * FM * 4 * 64 : 1;
+    __shared__ double xch[NX];
+    if (WS == 2) {
+        if (part == 1) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        xch[(((wl * FN + j) * FM + i) * 4 + r) * 64 + lane] = (double)accO[j][i][r];
+        }
+        __syncthreads();
+        if (part == 1) return;
+    }
+
     const double *D = g.D ? g.D + bz * g.sD : nullptr;
 #pragma unroll
     for (int j = 0; j < FN; ++j)
@@ -437,7 +482,9 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
                 const int pi = SIDE == 0 ? m0 + mi : n0 + ni;   // pair index a''
                 const int di = SIDE == 0 ? n0 + ni : m0 + mi;   // data row index
                 if (pi >= h || di >= nrd) continue;
-                const double o = (double)accO[j][i][r], e = (double)accE[j][i][r];
+                const double o = (double)accO[j][i][r];
+                const double e = WS == 2 ? xch[(((wl * FN + j) * FM + i) * 4 + r) * 64 + lane]
+                                         : (double)accE[j][i][r];
                 for (int side2 = 0; side2 < 2; ++side2) {
                     const int pr = side2 == 0 ? pi : Kd - 1 - pi;
                     if (side2 == 1 && pr == pi) break;
@@ -470,20 +517,22 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
             }
 }
 
-template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB = 2>
-__global__ __launch_bounds__(WM *WN * 64) void k_gemm_eo(GemmArgs g) {
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB = 2,
+          int WS = 1>
+__global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo(GemmArgs g) {
     if (g.skip && *g.skip) return;
     int bxt, byt, bz;
     xcd_tile(bxt, byt, bz);
-    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB>(g, bxt, byt, bz);
+    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, bxt, byt, bz);
 }
 
 // Grouped form: one launch over the contractions of many cases (each its own sizes, sine
 // matrices and buffers).  The linear workgroup id is remapped XCD-contiguously over the whole
 // grid, then located in the tile-offset table; the tile walks x fastest inside its problem.
-template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB>
-__global__ __launch_bounds__(WM *WN * 64) void kg_gemm_eo(const GemmArgs *gs, const int *off,
-                                                          int n) {
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB,
+          int WS = 1>
+__global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo(const GemmArgs *gs, const int *off,
+                                                               int n) {
     const int total = gridDim.x, q = total / 8, r = total % 8;
     const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
     const int T = xcd * q + min(xcd, r) + slot;
@@ -492,8 +541,8 @@ __global__ __launch_bounds__(WM *WN * 64) void kg_gemm_eo(const GemmArgs *gs, co
     const int local = T - off[p];
     const int rows = SIDE == 0 ? g.h : g.M, cols = SIDE == 0 ? g.N : g.h;
     const int gx = (rows + BM - 1) / BM, gy = (cols + BN - 1) / BN;
-    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB>(g, local % gx, (local / gx) % gy,
-                                                        local / (gx * gy));
+    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, local % gx, (local / gx) % gy,
+                                                            local / (gx * gy));
 }
 
 // ------------------------------------------------------------------------ variant tables
@@ -561,14 +610,17 @@ struct VariantEO {
     KFn fn[2][4];
     const char *name;
 };
-#define EOV(P, BM, BN, BK, WM, WN, NB)                                                       \
-    {BM, BN, WM * WN * 64,                                                                   \
-     {{k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB>, nullptr, nullptr, nullptr}, \
-      {k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB>,                            \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB>,                            \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB>,                          \
-       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB>}},                            \
-     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB}
+#define EOVW(P, BM, BN, BK, WM, WN, NB, WS, SUF)                                             \
+    {BM, BN, WM * WN * 64 * WS,                                                                  \
+     {{k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB, WS>, nullptr, nullptr, nullptr}, \
+      {k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB, WS>,                            \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB, WS>,                            \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB, WS>,                          \
+       k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB, WS>}},                            \
+     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF}
+#define EOV(P, BM, BN, BK, WM, WN, NB) EOVW(P, BM, BN, BK, WM, WN, NB, 1, "")
+// odd/even parts on separate waves (WS = 2)
+#define EOVS(P, BM, BN, BK, WM, WN, NB) EOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
 
 static const VariantEO EF64[] = {
     EOV(PF64, 64, 64, 16, 2, 2, 2), EOV(PF64, 32, 64, 16, 1, 2, 2), EOV(PF64, 64, 32, 16, 2, 1, 2),
@@ -583,12 +635,17 @@ static const VariantEO EF64[] = {
     // 64-deep K tiles: four times the MFMA work per tile's loads, stores and barriers
     EOV(PF64, 32, 32, 64, 2, 2, 1), EOV(PF64, 32, 64, 64, 2, 2, 1), EOV(PF64, 64, 32, 64, 2, 2, 1),
     EOV(PF64, 32, 32, 64, 2, 2, 2), EOV(PF64, 64, 64, 32, 2, 2, 1),
+    // odd and even parts on separate waves: twice the resident waves for small grids
+    EOVS(PF64, 32, 32, 16, 2, 2, 1), EOVS(PF64, 32, 32, 16, 2, 2, 2), EOVS(PF64, 32, 32, 32, 2, 2, 3),
+    EOVS(PF64, 32, 32, 32, 2, 2, 1), EOVS(PF64, 32, 64, 16, 2, 2, 1), EOVS(PF64, 64, 32, 16, 2, 2, 1),
+    EOVS(PF64, 16, 32, 16, 1, 2, 1), EOVS(PF64, 32, 16, 16, 2, 1, 1),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
     EOV(PF32, 64, 64, 32, 2, 2, 2), EOV(PF32, 64, 64, 16, 2, 2, 1), EOV(PF32, 32, 64, 16, 2, 2, 1),
     EOV(PF32, 64, 32, 16, 2, 2, 1), EOV(PF32, 64, 64, 32, 2, 2, 1),
     EOV(PF32, 32, 64, 32, 2, 2, 3), EOV(PF32, 64, 64, 32, 2, 2, 3),
+    EOVS(PF32, 32, 64, 16, 2, 2, 1), EOVS(PF32, 64, 32, 16, 2, 2, 1),
 };
 static const VariantEO EBF3[] = {
     EOV(PBF3, 64, 64, 32, 2, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2, 2),
@@ -632,20 +689,23 @@ struct VariantGEO {
     KGFn fn[2][4];
     const char *name;
 };
-#define GEOV(P, BM, BN, BK, WM, WN, NB)                                                       \
-    {BM, BN, WM * WN * 64,                                                                    \
-     {{kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB>, nullptr, nullptr, nullptr}, \
-      {kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB>,                            \
-       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB>,                            \
-       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB>,                          \
-       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB>}},                            \
-     "grouped eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB}
+#define GEOVW(P, BM, BN, BK, WM, WN, NB, WS, SUF)                                             \
+    {BM, BN, WM * WN * 64 * WS,                                                                   \
+     {{kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, NB, WS>, nullptr, nullptr, nullptr}, \
+      {kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, NB, WS>,                            \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB, WS>,                            \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB, WS>,                          \
+       kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB, WS>}},                            \
+     "grouped eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF}
+#define GEOV(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 1, "")
+#define GEOVS(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
 
 static const VariantGEO GF64[] = {
     GEOV(PF64, 64, 64, 16, 2, 2, 1), GEOV(PF64, 32, 64, 16, 2, 2, 1), GEOV(PF64, 64, 32, 16, 2, 2, 1),
     GEOV(PF64, 32, 64, 32, 2, 2, 2), GEOV(PF64, 64, 32, 32, 2, 2, 2), GEOV(PF64, 64, 64, 8, 2, 2, 1),
     GEOV(PF64, 32, 32, 16, 2, 2, 1), GEOV(PF64, 64, 64, 16, 2, 2, 3), GEOV(PF64, 32, 64, 32, 2, 2, 3),
     GEOV(PF64, 32, 32, 64, 2, 2, 1), GEOV(PF64, 32, 64, 64, 2, 2, 1),
+    GEOVS(PF64, 32, 32, 16, 2, 2, 1), GEOVS(PF64, 32, 64, 16, 2, 2, 1),
 };
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),

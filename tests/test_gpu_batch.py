@@ -139,6 +139,31 @@ def test_batch_graph_replay_matches_eager(monkeypatch):
             assert rel(a, b) < 1e-12
 
 
+def test_batch_grouped_variants_identical(monkeypatch):
+    # every grouped tile variant (forced for all stages) sums k in the same order, so the
+    # tuner's choice cannot change a batch's numbers
+    import ibhip as m
+
+    runs = []
+    for v in range(13):  # every FP64 grouped variant
+        monkeypatch.setenv("IBH_GRP_VARIANT", str(v))
+        probs = mixed_problems("fp64")[:4]
+        states = [m.initstate(p) for p in probs]
+        for p, st in zip(probs, states):
+            m.initsolver(p, st)
+        batch = m.Batch(states)
+        for k in range(2):
+            batch.advance()
+        batch.sync()
+        runs.append([st.case.gamma() for st in states] + [st.case.Fb() for st in states])
+        batch.close()
+        for st in states:
+            st.case.close()
+    for r in runs[1:]:
+        for a, b in zip(r, runs[0]):
+            assert np.array_equal(a, b)
+
+
 def test_batch_rejects_moving_bodies_and_outside_steps():
     import ibhip as m
 
