@@ -391,6 +391,7 @@ def main():
     log(f"bench: warmup done ({args.warmup} steps)")
 
     iters = []
+    st0 = case.solver_stats()
     marker = torch.zeros(1, device="cuda")  # a fill kernel that marks the timed window in traces
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -403,6 +404,7 @@ def main():
     case.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    st1 = case.solver_stats()
     marker.zero_()
     torch.cuda.synchronize()
     # per-launch timing of the contraction kernels: the library records HIP events around
@@ -445,6 +447,14 @@ def main():
     }
     cg = {"force_solver": "pcg" if prob.any_moving else "dense",
           "iters_mean": sum(iters) / max(len(iters), 1), "iters_max": max(iters or [0])}
+    if prob.any_moving:
+        # preconditioner rebuilds inside the timed steps (moving bodies' columns + inverse)
+        cg.update({"pc_rebuilds_timed": st1["pc_rebuilds"] - st0["pc_rebuilds"],
+                   "pc_rebuild_ms_per_step": 1e3 * (st1["pc_rebuild_s"] - st0["pc_rebuild_s"])
+                   / args.steps,
+                   "pc_columns_ms_total": 1e3 * (st1["pc_columns_s"] - st0["pc_columns_s"]),
+                   "pc_inverse_ms_total": 1e3 * (st1["pc_inverse_s"] - st0["pc_inverse_s"]),
+                   "iteration_ms": 1e3 * st1["iteration_s"]})
     out = {
         "metric": "cell-steps/s: FP64 vs mixed-precision batched, at ≤1e-6 rel lift/drag err",
         "value": value,

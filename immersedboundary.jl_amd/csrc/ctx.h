@@ -84,8 +84,11 @@ struct ibh_ctx {
     int pc_refresh = 50, steps_since_pc = 0;
     bool pc_adaptive = false;            // rebuild by cost (pc_refresh ≤ 0 in the description)
     int pc_prec = 1;                     // transform precision of the preconditioner's columns
+    int pc_levels = 1;                   // Vort2Flux levels of the moving columns (≤ 0: all)
     double pc_t_refresh = 0, pc_t_iter = 0, pc_extra = 0;  // seconds
     int pc_base = 1 << 30;               // fewest iterations since the last rebuild
+    int pc_rebuilds = 0;
+    double pc_time_total = 0, pc_time_cols = 0, pc_time_inv = 0;  // seconds, all rebuilds
     double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
     int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
     std::vector<int> J;                      // unknowns of moving bodies

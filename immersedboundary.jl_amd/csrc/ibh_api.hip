@@ -283,12 +283,14 @@ void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const dou
     }
 }
 
+// `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
+// values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
-               const int *skip, int prec = -1, int refine = 0) {
+               const int *skip, int prec = -1, int refine = 0, int nlev = -1) {
     const Dom &d = c->d;
-    const int L = c->L;
+    const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
     const long ng = d.ng, nq = d.nq;
-    const long sG = L * ng, sQ = curl_all ? L * nq : nq, sB = (long)L * d.nbc;
+    const long sG = Ls * ng, sQ = curl_all ? Ls * nq : nq, sB = (long)Ls * d.nbc;
     for (int lev = 1; lev < L; ++lev)
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
@@ -317,18 +319,19 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
 
 // x = B·z for nc columns (B_Times, cnab/ops.jl:125-143): Γ = 0; Γ₁ = Ainv₁ C' Eᵀ z;
 // vort2flux; x = E q₁.  z, x: column stride sz, sx.  The transforms run in FP64 (the
-// Schur-complement side of the step) unless `prec` asks otherwise (preconditioner columns).
+// Schur-complement side of the step) unless `prec` asks otherwise, and Vort2Flux runs on every
+// level unless `nlev` asks for fewer (both for the preconditioner's columns).
 void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,
-             int prec = IBH_FP64) {
+             int prec = IBH_FP64, int nlev = -1) {
     const Dom &d = c->d;
-    const int L = c->L;
+    const int L = c->L, Lv = nlev > 0 ? std::min(nlev, L) : L;
     const long ng = d.ng;
     launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
     launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
     for (int col = 0; col < nc; ++col)
-        if (L > 1) launch_fill(c->gB + col * L * ng + ng, (L - 1) * ng, 0.0, c->s);
+        if (Lv > 1) launch_fill(c->gB + col * L * ng + ng, (Lv - 1) * ng, 0.0, c->s);
     lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
-    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec);
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -382,7 +385,7 @@ __global__ void k_unit_cols(double *Z, int nf, int j0, int nc) {
 }
 
 // Bm[:, j0:j0+cnt] = B·e_j (unit-vector products in column batches, cnab/ops.jl:88-94)
-int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64) {
+int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64, int nlev = -1) {
     const int nf = 2 * c->nb;
     // column batch sized to keep the B-product work under ~12 GB
     const Dom &d = c->d;
@@ -395,12 +398,11 @@ int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64) {
         const int n = std::min(nc, j0 + cnt - j);
         hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * n + 255) / 256)), dim3(256), 0,
                            c->s, c->zB, nf, j, n);
-        B_times(c, c->zB, nf, Bm + (long)nf * j, nf, n, nullptr, prec);
+        B_times(c, c->zB, nf, Bm + (long)nf * j, nf, n, nullptr, prec, nlev);
     }
     return IBH_OK;
 }
 
-// Inv = inverse(A) by Gauss–Jordan on [A | I]; A is destroyed
 void dfree(ibh_ctx *c, void *p);
 
 // Inv = inverse(A) through LAPACK's getrf + getri as rocSOLVER implements them (the routines
@@ -434,11 +436,12 @@ int invert_lapack(ibh_ctx *c, double *A, double *Inv) {
     return IBH_OK;
 }
 
-// Inv = inverse(A) by Gauss–Jordan with partial pivoting (up to 2048 unknowns) or rocSOLVER's
-// LU (larger systems, or IBH_DENSE_INV=lapack); A is destroyed
+// Inv = inverse(A) by Gauss–Jordan with partial pivoting (up to 768 unknowns) or rocSOLVER's
+// LU (larger systems, or IBH_DENSE_INV=lapack|gj to force one); A is destroyed
 int invert(ibh_ctx *c, double *A, double *Inv) {
     const int nf = 2 * c->nb;
-    bool lapack = nf > 2048;
+    // measured on gfx950: Gauss–Jordan inverts 1656² in 39 ms, rocSOLVER getrf+getri in 16 ms
+    bool lapack = nf > 768;
     if (const char *e = std::getenv("IBH_DENSE_INV")) lapack = std::strcmp(e, "lapack") == 0;
     if (lapack) return invert_lapack(c, A, Inv);
     hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
@@ -517,19 +520,30 @@ int refresh_pc(ibh_ctx *c) {
         HIPCK(hipMemcpyAsync(c->dinJ, inJ.data(), sizeof(int) * nf, hipMemcpyHostToDevice, c->s));
         HIPCK(hipStreamSynchronize(c->s));
     }
+    const double t0 = now_s();
+    // fewer levels only where rebuilds recur (moving bodies); a fixed set is built once, exact
+    const int lv = c->J.empty() ? -1 : c->pc_levels;
     if (!c->pc_valid) {
-        rc = B_columns(c, c->Bfull, 0, nf, c->pc_prec);
+        rc = B_columns(c, c->Bfull, 0, nf, c->pc_prec, lv);
         if (rc) return rc;
     } else {
         for (auto &r : c->Jrange) {
-            rc = B_columns(c, c->Bfull, r.first, r.second, c->pc_prec);
+            rc = B_columns(c, c->Bfull, r.first, r.second, c->pc_prec, lv);
             if (rc) return rc;
         }
         launch_mirror_rows(nf, c->Bfull, c->dJ, (int)c->J.size(), c->dinJ, c->s);
     }
     launch_copy(c->Bwork, c->Bfull, (long)nf * nf, c->s);
+    HIPCK(hipStreamSynchronize(c->s));
+    const double t1 = now_s();
     rc = invert(c, c->Bwork, c->Pc);
     if (rc) return rc;
+    HIPCK(hipStreamSynchronize(c->s));
+    const double t2 = now_s();
+    ++c->pc_rebuilds;
+    c->pc_time_cols += t1 - t0;
+    c->pc_time_inv += t2 - t1;
+    c->pc_time_total += t2 - t0;
     c->pc_valid = true;
     c->steps_since_pc = 0;
     return IBH_OK;
@@ -744,6 +758,12 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
             }
             // a preconditioner that has drifted far from B is rebuilt at the next step
             if (st[1] > 24) c->steps_since_pc = c->pc_refresh;
+            // a fresh one that is already slow lacks coarse-level coupling: add a level
+            if (refresh && !c->J.empty() && st[1] > 8 && c->pc_levels > 0 &&
+                c->pc_levels < c->L) {
+                ++c->pc_levels;
+                c->steps_since_pc = c->pc_refresh;
+            }
         }
     }
     if ((rc = run_seg(c, &c->gPost[par], g, [&] { seg_post(c); }, post_swap))) return rc;
@@ -831,6 +851,14 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     // still applies the FP64 B); IBH_PC_PREC=0/1/2 picks another precision
     c->pc_prec = IBH_FP32;
     if (const char *pp = std::getenv("IBH_PC_PREC")) c->pc_prec = std::atoi(pp);
+    // ... and, for moving bodies, a Vort2Flux over the finest level only: the coarser levels
+    // change B by ~1e-5 relative (measured on the 1024², 5-level plate case: the inverse of
+    // the one-level B takes 4-5 PCG iterations on the five-level B where its exact inverse
+    // takes 2, the two-level one 3), at a third of the transforms per column.  A fresh
+    // preconditioner that needs more than 8 iterations adds a level (see step).
+    // IBH_PC_LEVELS sets the starting count (≤ 0: every level).
+    c->pc_levels = 1;
+    if (const char *pl = std::getenv("IBH_PC_LEVELS")) c->pc_levels = std::atoi(pl);
     c->pc_refresh = D.pc_refresh > 0 ? D.pc_refresh : (1 << 30);
     if (D.precision != IBH_FP64)
         c->refine = D.refine_sweeps > 0 ? D.refine_sweeps
@@ -1076,6 +1104,14 @@ int ibh_dims(ibh_ctx *c, int32_t *o) {
 double ibh_time(ibh_ctx *c) { return c ? c->t : 0.0; }
 
 int ibh_cg_iterations(ibh_ctx *c) { return c ? c->cg_last : -1; }
+
+int ibh_solver_stats(ibh_ctx *c, double *out6) {
+    if (!c || !out6) return IBH_EINVAL;
+    const double v[6] = {(double)c->pc_rebuilds, c->pc_time_total, c->pc_time_cols,
+                         c->pc_time_inv, c->pc_t_iter, c->pc_t_refresh};
+    for (int k = 0; k < 6; ++k) out6[k] = v[k];
+    return IBH_OK;
+}
 
 // --------------------------------------------------------------------------- operators
 namespace {

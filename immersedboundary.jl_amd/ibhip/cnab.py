@@ -148,6 +148,14 @@ class DeviceCase:
     def cg_iters(self) -> int:
         return self.lib.ibh_cg_iterations(self.h)
 
+    def solver_stats(self) -> dict:
+        """PCG preconditioner rebuilds: count, seconds in total / column assembly / inversion,
+        and the per-iteration and per-rebuild costs the adaptive rebuild rule weighs."""
+        out = np.zeros(6)
+        self._check(self.lib.ibh_solver_stats(self.h, dptr(out)))
+        return {"pc_rebuilds": int(out[0]), "pc_rebuild_s": out[1], "pc_columns_s": out[2],
+                "pc_inverse_s": out[3], "iteration_s": out[4], "rebuild_s_weighed": out[5]}
+
     def time(self) -> float:
         return self.lib.ibh_time(self.h)
 

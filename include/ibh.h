@@ -135,6 +135,10 @@ int ibh_dims(ibh_ctx *ctx, int32_t *out8);
 double ibh_time(ibh_ctx *ctx);
 /* iterations of the last CG force solve (IterativeSolvers cg!, cnab/ops.jl:110) */
 int ibh_cg_iterations(ibh_ctx *ctx);
+/* PCG preconditioner statistics (no reference counterpart; the reference has no
+ * preconditioner): out6 = {rebuilds, total rebuild s, column assembly s, inversion s,
+ * last measured seconds per iteration, last rebuild s as the adaptive rule weighs it} */
+int ibh_solver_stats(ibh_ctx *ctx, double *out6);
 
 /* ---- operator-level entry points (host arrays in/out; each runs the device kernels) ---- */
 /* LaplacianInv apply (fluid-ops.jl:42-56): mode 0 Δinv (Λ), 1 A_ℓ (Λexpl), 2 Ainv_ℓ (Λimpl),
