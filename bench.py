@@ -233,7 +233,9 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
         batch.advance()
     batch.sync()
     prof_elapsed = time.perf_counter() - t1
-    ms, launches, flops = batch.profile_read({"fp64": 0, "fp32": 1, "bf16x3": 2}[precision])
+    kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[precision]
+    ms, launches, flops = batch.profile_read(kind)
+    ev_ms, ev_launches, _ = batch.profile_read(kind, events=True)
     batch.profile(False)
     launches_per_step = batch.launches_per_step()
     tune = batch.tune_report()
@@ -260,10 +262,13 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
             "traffic": measured_traffic("ellipse_batch", precision),
             "kernel": f"kg_gemm_eo<{precision}> (grouped sine-transform contraction)",
             "avg_launch_us": avg_launch_s * 1e6,
+            "event_avg_launch_us": 1e3 * ev_ms / max(ev_launches, 1),
             "launches": launches,
             "gemm_share_of_profiled_steps": (ms / 1e3) / prof_elapsed,
-            "timing": "HIP events around each grouped contraction launch on the batch stream, "
-                      "over a profiled pass of eager steps right after the timed (graph) region",
+            "timing": "device wall clock of each grouped contraction launch (first workgroup "
+                      "start to last workgroup end) over a profiled pass of eager steps right "
+                      "after the timed (graph) region; event_avg_launch_us: HIP events around "
+                      "each launch on the batch stream",
         },
     }
 
@@ -417,7 +422,9 @@ def main():
         k += 1
     case.sync()
     prof_elapsed = time.perf_counter() - t1
-    ms, launches, flops = case.profile_read({"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision])
+    kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision]
+    ms, launches, flops = case.profile_read(kind)
+    ev_ms, ev_launches, _ = case.profile_read(kind, events=True)
     case.profile(False)
     log(f"bench: profiled pass {args.profile_steps} steps")
     accuracy = None
@@ -438,12 +445,18 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / peak,
         "traffic": traffic,
-        "kernel": f"k_gemm<{args.precision}> (sine-transform contraction)",
+        "kernel": f"k_gemm_eo<{args.precision}> (split sine-transform contraction)",
         "avg_launch_us": avg_launch_s * 1e6,
         "launches": launches,
+        "event_avg_launch_us": 1e3 * ev_ms / max(ev_launches, 1),
+        "event_launches": ev_launches,
         "gemm_share_of_profiled_steps": (ms / 1e3) / prof_elapsed,
-        "timing": "HIP events around each contraction launch on the library stream, over a "
-                  "profiled pass of eager steps right after the timed (graph) region",
+        "timing": "device wall clock of each contraction launch (first workgroup start to last "
+                  "workgroup end, as rocprofv3's kernel trace measures it) over a profiled pass "
+                  "of eager steps right after the timed (graph) region; launches skipped on the "
+                  "device after the CG converged are left out of time, count and flops. "
+                  "event_avg_launch_us: HIP events around every launch on the library stream "
+                  "(adds the dispatch gap, counts the skipped launches)",
     }
     cg = {"force_solver": "pcg" if prob.any_moving else "dense",
           "iters_mean": sum(iters) / max(len(iters), 1), "iters_max": max(iters or [0])}

@@ -415,14 +415,14 @@ int upload(ibh_batch *b) {
     return IBH_OK;
 }
 
-void launch_stage(ibh_batch *b, const Stage &S, int variant) {
+void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long *ts = nullptr) {
     Grp g{S.d_args, S.d_off, S.n, S.total.empty() ? 0 : S.total[0]};
     hipStream_t s = b->s;
     switch (S.op) {
     case OP_GEMM:
         g.off = S.d_off + (size_t)variant * (S.n + 1);
         g.blocks = S.total[variant];
-        launch_gemm_eo_grp(g, S.prec, variant, S.side, S.epi, s);
+        launch_gemm_eo_grp(g, S.prec, variant, S.side, S.epi, s, ts);
         break;
     case OP_CURL: launch_grp_curl(g, s); break;
     case OP_ROT: launch_grp_rot(g, s); break;
@@ -499,8 +499,9 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
                     hipEventCreate(&e);
                     P.pool.push_back(e);
                 }
+            unsigned long long *ts = prof_slot(P);
             hipEventRecord(P.pool[P.used], b->s);
-            launch_stage(b, S, S.variant);
+            launch_stage(b, S, S.variant, ts);
             hipEventRecord(P.pool[P.used + 1], b->s);
             P.used += 2;
             P.rec.push_back({S.prec, S.flops});
@@ -623,6 +624,7 @@ void ibh_batch_destroy(ibh_batch *b) {
     if (b->arena) hipFree(b->arena);
     if (b->uv) hipFree(b->uv);
     for (hipEvent_t e : b->prof.pool) hipEventDestroy(e);
+    if (b->prof.ts) hipFree(b->prof.ts);
     if (b->s) hipStreamDestroy(b->s);
     delete b;
 }
@@ -665,6 +667,8 @@ int ibh_batch_profile_enable(ibh_batch *b, int32_t on) {
     b->prof.on = on != 0;
     b->prof.used = 0;
     b->prof.rec.clear();
+    if (b->prof.on && prof_arm(b->prof, b->s))
+        return bfail(b, IBH_EDEVICE, "allocating the profiling timestamps failed");
     return IBH_OK;
 }
 
@@ -672,20 +676,16 @@ int ibh_batch_profile_read(ibh_batch *b, int32_t kind, double *ms, int64_t *laun
                            double *flops) {
     if (!b) return IBH_EINVAL;
     BCK(hipStreamSynchronize(b->s));
-    double tot = 0, fl = 0;
-    int64_t n = 0;
-    for (size_t k = 0; k < b->prof.rec.size(); ++k) {
-        if (b->prof.rec[k].first != kind) continue;
-        ++n;
-        float e = 0;
-        BCK(hipEventElapsedTime(&e, b->prof.pool[2 * k], b->prof.pool[2 * k + 1]));
-        tot += e;
-        fl += b->prof.rec[k].second;
-    }
-    if (ms) *ms = tot;
-    if (launches) *launches = n;
-    if (flops) *flops = fl;
-    return IBH_OK;
+    const int rc = prof_device_sum(b->prof, kind, b->s, ms, launches, flops);
+    return rc ? bfail(b, rc, "reading the profiling timestamps failed") : IBH_OK;
+}
+
+int ibh_batch_profile_read_events(ibh_batch *b, int32_t kind, double *ms, int64_t *launches,
+                                  double *flops) {
+    if (!b) return IBH_EINVAL;
+    BCK(hipStreamSynchronize(b->s));
+    const int rc = prof_event_sum(b->prof, kind, ms, launches, flops);
+    return rc ? bfail(b, rc, "reading the profiling events failed") : IBH_OK;
 }
 
 int ibh_batch_tune_report(ibh_batch *b, char *buf, int32_t len) {

@@ -27,7 +27,19 @@ struct Prof {
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<std::pair<int, double>> rec;  // (kind, flops) per recorded launch
+    unsigned long long *ts = nullptr;  // device wall clock: PROF_SLOTS records of PROF_REC
 };
+
+// Profiling of the contraction launches.  prof_arm (at profile enable) allocates and presets
+// every timestamp slot at once, so that nothing runs between the profiled launches;
+// prof_slot returns the slot of the record about to be pushed (null past the capacity);
+// prof_device_sum adds up the device-timed kernel time, launches and flops of one precision,
+// leaving out launches that their device flag skipped; prof_event_sum does the same with the
+// HIP events around each launch (which also count the dispatch gap and the skipped launches).
+int prof_arm(Prof &P, hipStream_t s);
+unsigned long long *prof_slot(Prof &P);
+int prof_device_sum(const Prof &P, int kind, hipStream_t s, double *ms, int64_t *n, double *fl);
+int prof_event_sum(const Prof &P, int kind, double *ms, int64_t *n, double *fl);
 
 }  // namespace ibh
 

@@ -129,6 +129,22 @@ __device__ inline void xcd_tile(int &x, int &y, int &z) {
     z = T / (gx * gy);
 }
 
+// ------------------------------------------------------------------------ profiling
+// Kernel duration as the profiler defines it (first workgroup start to last workgroup end), on
+// the constant-rate device wall clock, only when a profiling record is passed: workgroup 0
+// (linear id `lin`) stores the start; every workgroup folds its end into a sub-slot with a
+// non-returning vector atomic.
+__device__ inline void ts_mark(unsigned long long *ts, bool end, unsigned lin) {
+    if (ts && threadIdx.x == 0) {
+        const unsigned long long t = (unsigned long long)wall_clock64();
+        if (end) atomicMax(ts + 16 + 16 * (lin & 15), t);
+        else if (lin == 0) ts[0] = t;
+    }
+}
+__device__ inline unsigned lin_block() {
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
 // ------------------------------------------------------------------------ kernel
 template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT>
 __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
@@ -141,6 +157,7 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
     static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "load split");
 
     if (g.skip && *g.skip) return;
+    ts_mark(g.ts, false, lin_block());
     int bxt, byt, bz;
     xcd_tile(bxt, byt, bz);
     const double *__restrict__ A = g.A + bz * g.sA;  // A(m, k) at A[k + lda·m]
@@ -259,6 +276,7 @@ __global__ __launch_bounds__(WM *WN * 64) void k_gemm(GemmArgs g) {
                     C[ci] = v + e;
                 }
             }
+    ts_mark(g.ts, true, lin_block());
 }
 
 // ------------------------------------------------------------------------ even/odd split
@@ -521,9 +539,11 @@ template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, i
           int WS = 1>
 __global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo(GemmArgs g) {
     if (g.skip && *g.skip) return;
+    ts_mark(g.ts, false, lin_block());
     int bxt, byt, bz;
     xcd_tile(bxt, byt, bz);
     eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, bxt, byt, bz);
+    ts_mark(g.ts, true, lin_block());
 }
 
 // Grouped form: one launch over the contractions of many cases (each its own sizes, sine
@@ -532,7 +552,8 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo(GemmArgs g) {
 template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int NB,
           int WS = 1>
 __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo(const GemmArgs *gs, const int *off,
-                                                               int n) {
+                                                               int n, unsigned long long *ts) {
+    ts_mark(ts, false, blockIdx.x);
     const int total = gridDim.x, q = total / 8, r = total % 8;
     const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
     const int T = xcd * q + min(xcd, r) + slot;
@@ -543,6 +564,7 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo(const GemmArgs *g
     const int gx = (rows + BM - 1) / BM, gy = (cols + BN - 1) / BN;
     eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, local % gx, (local / gx) % gy,
                                                             local / (gx * gy));
+    ts_mark(ts, true, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------ variant tables
@@ -683,7 +705,7 @@ void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi,
 }
 
 // grouped split variants: [side][epi]
-typedef void (*KGFn)(const GemmArgs *, const int *, int);
+typedef void (*KGFn)(const GemmArgs *, const int *, int, unsigned long long *);
 struct VariantGEO {
     int bm, bn, nthreads;
     KGFn fn[2][4];
@@ -743,13 +765,14 @@ long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side) {
     return (long)((gm + V.bm - 1) / V.bm) * ((gn + V.bn - 1) / V.bn) * g.batch;
 }
 
-void launch_gemm_eo_grp(const Grp &gr, int prec, int variant, int side, int epi, hipStream_t s) {
+void launch_gemm_eo_grp(const Grp &gr, int prec, int variant, int side, int epi, hipStream_t s,
+                        unsigned long long *ts) {
     if (gr.n == 0 || gr.blocks == 0) return;
     int n;
     const VariantGEO *t = table_geo(prec, &n);
     const VariantGEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
     hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], dim3(gr.blocks), dim3(V.nthreads), 0, s,
-                       static_cast<const GemmArgs *>(gr.args), gr.off, gr.n);
+                       static_cast<const GemmArgs *>(gr.args), gr.off, gr.n, ts);
 }
 
 }  // namespace ibh

@@ -61,6 +61,81 @@ int dalloc(ibh_ctx *c, T **p, size_t n) {
     } while (0)
 
 // --------------------------------------------------------------------------- profiling
+}  // namespace
+
+namespace ibh {
+
+int prof_arm(Prof &P, hipStream_t s) {
+    const size_t bytes = (size_t)PROF_SLOTS * PROF_REC * sizeof(unsigned long long);
+    if (!P.ts && hipMalloc(&P.ts, bytes) != hipSuccess) {
+        P.ts = nullptr;
+        return IBH_EDEVICE;
+    }
+    // enabling profiling runs outside any graph capture, so a plain memset is fine here
+    if (hipMemsetAsync(P.ts, 0, bytes, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return IBH_EDEVICE;
+    return IBH_OK;
+}
+
+unsigned long long *prof_slot(Prof &P) {
+    const size_t k = P.rec.size();
+    return (P.ts && k < (size_t)PROF_SLOTS) ? P.ts + k * PROF_REC : nullptr;
+}
+
+int prof_device_sum(const Prof &P, int kind, hipStream_t s, double *ms, int64_t *n, double *fl) {
+    const size_t nr = P.ts ? std::min(P.rec.size(), (size_t)PROF_SLOTS) : 0;
+    std::vector<unsigned long long> h(nr * PROF_REC + 1);
+    if (nr) {
+        if (hipMemcpyAsync(h.data(), P.ts, nr * PROF_REC * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return IBH_EDEVICE;
+    }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        return IBH_EDEVICE;
+    double tot = 0, f = 0;
+    int64_t cnt = 0;
+    for (size_t k = 0; k < nr; ++k) {
+        if (P.rec[k].first != kind) continue;
+        const unsigned long long *r = h.data() + k * PROF_REC;
+        unsigned long long b = 0;
+        for (int j = 0; j < 16; ++j) b = std::max(b, r[16 + 16 * j]);
+        const unsigned long long a = r[0];
+        if (a == 0 || b == 0 || b < a) continue;  // skipped on the device
+        tot += (double)(b - a) / khz;                  // ticks / kHz = ms
+        f += P.rec[k].second;
+        ++cnt;
+    }
+    if (ms) *ms = tot;
+    if (n) *n = cnt;
+    if (fl) *fl = f;
+    return IBH_OK;
+}
+
+int prof_event_sum(const Prof &P, int kind, double *ms, int64_t *n, double *fl) {
+    double tot = 0, f = 0;
+    int64_t cnt = 0;
+    for (size_t k = 0; k < P.rec.size(); ++k) {
+        if (P.rec[k].first != kind) continue;
+        float e = 0;
+        if (hipEventElapsedTime(&e, P.pool[2 * k], P.pool[2 * k + 1]) != hipSuccess)
+            return IBH_EDEVICE;
+        tot += e;
+        f += P.rec[k].second;
+        ++cnt;
+    }
+    if (ms) *ms = tot;
+    if (n) *n = cnt;
+    if (fl) *fl = f;
+    return IBH_OK;
+}
+
+}  // namespace ibh
+
+namespace {
+
 void prof_begin(ibh_ctx *c, hipEvent_t *e0) {
     Prof &P = c->prof;
     if (P.used + 2 > P.pool.size()) {
@@ -145,6 +220,7 @@ double gemm_flops(const GemmArgs &g, int kind) {
 void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
     const int v = tune(c, g, prec, kind, tout);
     if (c->prof.on) {
+        g.ts = prof_slot(c->prof);
         hipEvent_t e0;
         prof_begin(c, &e0);
         launch_kind(g, prec, kind, v, epi, tout, c->s);
@@ -995,6 +1071,7 @@ void ibh_destroy(ibh_ctx *c) {
     for (void *p : c->allocs) hipFree(p);
     if (c->tune_buf) hipFree(c->tune_buf);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
+    if (c->prof.ts) hipFree(c->prof.ts);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }
@@ -1288,26 +1365,24 @@ int ibh_profile_enable(ibh_ctx *c, int32_t on) {
     c->prof.on = on != 0;
     c->prof.used = 0;
     c->prof.rec.clear();
+    if (c->prof.on && ibh::prof_arm(c->prof, c->s))
+        return fail(c, IBH_EDEVICE, "allocating the profiling timestamps failed");
     return IBH_OK;
 }
 
 int ibh_profile_read(ibh_ctx *c, int32_t kind, double *ms, int64_t *launches, double *flops) {
     if (!c) return IBH_EINVAL;
     HIPCK(hipStreamSynchronize(c->s));
-    double tot = 0, fl = 0;
-    int64_t n = 0;
-    for (size_t k = 0; k < c->prof.rec.size(); ++k) {
-        if (c->prof.rec[k].first != kind) continue;
-        float e = 0;
-        HIPCK(hipEventElapsedTime(&e, c->prof.pool[2 * k], c->prof.pool[2 * k + 1]));
-        tot += e;
-        fl += c->prof.rec[k].second;
-        ++n;
-    }
-    if (ms) *ms = tot;
-    if (launches) *launches = n;
-    if (flops) *flops = fl;
-    return IBH_OK;
+    const int rc = ibh::prof_device_sum(c->prof, kind, c->s, ms, launches, flops);
+    return rc ? fail(c, rc, "reading the profiling timestamps failed") : IBH_OK;
+}
+
+int ibh_profile_read_events(ibh_ctx *c, int32_t kind, double *ms, int64_t *launches,
+                            double *flops) {
+    if (!c) return IBH_EINVAL;
+    HIPCK(hipStreamSynchronize(c->s));
+    const int rc = ibh::prof_event_sum(c->prof, kind, ms, launches, flops);
+    return rc ? fail(c, rc, "reading the profiling events failed") : IBH_OK;
 }
 
 }  // extern "C"

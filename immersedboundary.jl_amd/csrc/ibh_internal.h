@@ -52,7 +52,15 @@ struct GemmArgs {
     const double *So, *Se;
     int h;
     int batch;
+    // profiling only: a PROF_REC record of device wall-clock ticks, zeroed beforehand: the
+    // start of workgroup 0 (the first dispatched) at ts[0], each workgroup's end folded by
+    // atomicMax into one of 16 sub-slots (ts[16 + 16·j], 128 B apart, so that the end
+    // atomics do not pile onto one address); a launch skipped by its flag writes nothing.
+    // Null when not profiling.
+    unsigned long long *ts;
 };
+constexpr int PROF_SLOTS = 8192;  // profiled launches per profiling session
+constexpr int PROF_REC = 272;     // u64 per profiled launch
 
 // prec: 0 FP64, 1 FP32, 2 bf16x3; variant indexes the tile table of that precision;
 // tout writes C transposed (C(m, n) at C[n + ldc·m], EPI_STORE only).
@@ -232,7 +240,8 @@ void launch_grp_stress(const Grp &g, hipStream_t s);      // blocks: nbodies
 // grouped even/odd split contraction: every problem's GemmArgs in device memory; problem i
 // owns tiles [off[i], off[i+1]) with tiles = ceil(rows/BM)·ceil(cols/BN)·batch for the
 // variant's tile shape (grouped_eo_tiles).
-void launch_gemm_eo_grp(const Grp &g, int prec, int variant, int side, int epi, hipStream_t s);
+void launch_gemm_eo_grp(const Grp &g, int prec, int variant, int side, int epi, hipStream_t s,
+                        unsigned long long *ts = nullptr);
 int gemm_eo_grp_variant_count(int prec);
 const char *gemm_eo_grp_variant_name(int prec, int v);
 long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side);

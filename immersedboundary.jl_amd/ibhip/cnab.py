@@ -239,11 +239,14 @@ class DeviceCase:
     def profile(self, on: bool):
         self._check(self.lib.ibh_profile_enable(self.h, int(on)))
 
-    def profile_read(self, kind: int = 0):
+    def profile_read(self, kind: int = 0, events: bool = False):
+        """(ms, launches, flops) of the profiled contraction launches of one precision: device
+        kernel time of the launches that ran, or (events=True) HIP-event time of all."""
         ms = C.c_double()
         n = C.c_int64()
         fl = C.c_double()
-        self._check(self.lib.ibh_profile_read(self.h, kind, C.byref(ms), C.byref(n), C.byref(fl)))
+        fn = self.lib.ibh_profile_read_events if events else self.lib.ibh_profile_read
+        self._check(fn(self.h, kind, C.byref(ms), C.byref(n), C.byref(fl)))
         return ms.value, n.value, fl.value
 
 
@@ -422,10 +425,10 @@ class Batch:
     def profile(self, on: bool):
         self._check(self.lib.ibh_batch_profile_enable(self.h, int(on)))
 
-    def profile_read(self, kind: int = 0):
+    def profile_read(self, kind: int = 0, events: bool = False):
         ms, n, fl = C.c_double(), C.c_int64(), C.c_double()
-        self._check(self.lib.ibh_batch_profile_read(self.h, kind, C.byref(ms), C.byref(n),
-                                                    C.byref(fl)))
+        fn = self.lib.ibh_batch_profile_read_events if events else self.lib.ibh_batch_profile_read
+        self._check(fn(self.h, kind, C.byref(ms), C.byref(n), C.byref(fl)))
         return ms.value, n.value, fl.value
 
     def tune_report(self) -> list:

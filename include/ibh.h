@@ -196,16 +196,25 @@ int ibh_batch_profile_enable(ibh_batch *batch, int32_t on);
 /* kind as in ibh_profile_read: the grouped contraction launches of that precision */
 int ibh_batch_profile_read(ibh_batch *batch, int32_t kind, double *ms, int64_t *launches,
                            double *flops);
+int ibh_batch_profile_read_events(ibh_batch *batch, int32_t kind, double *ms, int64_t *launches,
+                                  double *flops);
 int ibh_batch_tune_report(ibh_batch *batch, char *buf, int32_t len);
 
-/* ---- profiling (HIP events around each launch of the sine-transform kernels) ----------- */
+/* ---- profiling of the sine-transform contraction launches (no reference counterpart) --- */
 int ibh_profile_enable(ibh_ctx *ctx, int32_t on);
 /* The GEMM autotuner's choices so far (one line per contraction shape: tile variant, sizes,
  * measured time), as text.  No reference counterpart. */
 int ibh_tune_report(ibh_ctx *ctx, char *buf, int32_t len);
 /* kind 0: FP64 transform GEMMs, 1: FP32, 2: bf16x3.  Returns the summed kernel time (ms),
- * launch count and algorithmic flops since the last enable. */
+ * launch count and algorithmic flops since the last enable.  Kernel time is the device wall
+ * clock from the first workgroup's start to the last one's end, as a profiler's kernel trace
+ * measures it; launches that a device flag skipped (B-products after the CG converged) are
+ * left out of all three figures. */
 int ibh_profile_read(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches, double *flops);
+/* The same from HIP events recorded around every launch: includes the dispatch gap and the
+ * skipped launches. */
+int ibh_profile_read_events(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches,
+                            double *flops);
 
 #ifdef __cplusplus
 }
