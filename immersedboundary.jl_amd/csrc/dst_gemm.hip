@@ -321,9 +321,10 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
     const int l16 = lane & 15, lq = lane >> 4;
 
     // NB = 1: one LDS buffer; 2: two buffers, the next K tile prefetched into registers;
-    // 3: two buffers and two register stages, so global loads are issued two K tiles ahead
+    // NB = R + 1 ≥ 3: two buffers and R register stages, so global loads are issued R K tiles
+    // ahead
     constexpr int NBUF = NB == 1 ? 1 : 2;
-    constexpr int NREG = NB == 3 ? 2 : 1;
+    constexpr int NREG = NB >= 3 ? NB - 1 : 1;
     __shared__ Tile<P, BM, BK> AsO[NBUF], AsE[NBUF];
     __shared__ Tile<P, BN, BK> BsO[NBUF], BsE[NBUF];
 
@@ -424,24 +425,27 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
     };
 
     const int nk = (h + BK - 1) / BK;
-    if (NB == 3) {
-        // tile kt computes from buffer kt&1 while tile kt+1 waits in one register stage and
-        // the loads of tile kt+2 land in the other; the loop is unrolled by two so that the
-        // register stages are compile-time indices
+    if (NB >= 3) {
+        // tile t computes from LDS buffer t&1 while tiles t+1 .. t+R-1 wait in register
+        // stages and the loads of tile t+R land in the stage tile t left; the loop is
+        // unrolled by R so that the register stage of every tile is a compile-time index
+        constexpr int R = NREG;
         gload(0, 0);
         sstore(0, 0);
-        if (nk > 1) gload(BK, 0);
+#pragma unroll
+        for (int u = 1; u < R; ++u)
+            if (u < nk) gload(u * BK, u);
         __syncthreads();
-        for (int kt = 0; kt < nk; kt += 2) {
-            if (kt + 2 < nk) gload((kt + 2) * BK, NREG - 1);
-            compute(0);
-            if (kt + 1 < nk) sstore(1, 0);
-            __syncthreads();
-            if (kt + 1 >= nk) break;
-            if (kt + 3 < nk) gload((kt + 3) * BK, 0);
-            compute(1);
-            if (kt + 2 < nk) sstore(0, NREG - 1);
-            __syncthreads();
+        for (int kt = 0; kt < nk; kt += R) {
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int t = kt + u;
+                if (t >= nk) break;
+                if (t + R < nk) gload((t + R) * BK, u);
+                compute(t & 1);
+                if (t + 1 < nk) sstore((t + 1) & 1, (u + 1) % R);
+                __syncthreads();
+            }
         }
     } else {
         gload(0, 0);
@@ -661,6 +665,9 @@ static const VariantEO EF64[] = {
     EOVS(PF64, 32, 32, 16, 2, 2, 1), EOVS(PF64, 32, 32, 16, 2, 2, 2), EOVS(PF64, 32, 32, 32, 2, 2, 3),
     EOVS(PF64, 32, 32, 32, 2, 2, 1), EOVS(PF64, 32, 64, 16, 2, 2, 1), EOVS(PF64, 64, 32, 16, 2, 2, 1),
     EOVS(PF64, 16, 32, 16, 1, 2, 1), EOVS(PF64, 32, 16, 16, 2, 1, 1),
+    // three and four register stages (loads issued three / four K tiles ahead)
+    EOVS(PF64, 32, 32, 32, 2, 2, 4), EOVS(PF64, 32, 32, 32, 2, 2, 5), EOVS(PF64, 32, 32, 16, 2, 2, 4),
+    EOVS(PF64, 32, 32, 16, 2, 2, 5), EOVS(PF64, 32, 64, 16, 2, 2, 4), EOVS(PF64, 64, 32, 16, 2, 2, 4),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
@@ -728,6 +735,7 @@ static const VariantGEO GF64[] = {
     GEOV(PF64, 32, 32, 16, 2, 2, 1), GEOV(PF64, 64, 64, 16, 2, 2, 3), GEOV(PF64, 32, 64, 32, 2, 2, 3),
     GEOV(PF64, 32, 32, 64, 2, 2, 1), GEOV(PF64, 32, 64, 64, 2, 2, 1),
     GEOVS(PF64, 32, 32, 16, 2, 2, 1), GEOVS(PF64, 32, 64, 16, 2, 2, 1),
+    GEOVS(PF64, 32, 32, 32, 2, 2, 4), GEOVS(PF64, 32, 64, 16, 2, 2, 4),
 };
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),

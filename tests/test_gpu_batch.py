@@ -145,7 +145,7 @@ def test_batch_grouped_variants_identical(monkeypatch):
     import ibhip as m
 
     runs = []
-    for v in range(13):  # every FP64 grouped variant
+    for v in range(15):  # every FP64 grouped variant
         monkeypatch.setenv("IBH_GRP_VARIANT", str(v))
         probs = mixed_problems("fp64")[:4]
         states = [m.initstate(p) for p in probs]
