@@ -424,36 +424,41 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
         }
     };
 
-    const int nk = (h + BK - 1) / BK;
-    if (NB >= 3) {
+    // K tiles [kt0, kt0 + nk) hold the pairs to sum (all of them unless klo/khi narrow it)
+    const int nkall = (h + BK - 1) / BK;
+    const int kt0 = g.khi > 0 ? g.klo / BK : 0;
+    const int nk = g.khi > 0 ? min(nkall, (g.khi + BK - 1) / BK) - kt0 : nkall;
+    if (nk <= 0) {
+        // nothing to sum: the accumulators stay zero
+    } else if (NB >= 3) {
         // tile t computes from LDS buffer t&1 while tiles t+1 .. t+R-1 wait in register
         // stages and the loads of tile t+R land in the stage tile t left; the loop is
         // unrolled by R so that the register stage of every tile is a compile-time index
         constexpr int R = NREG;
-        gload(0, 0);
+        gload(kt0 * BK, 0);
         sstore(0, 0);
 #pragma unroll
         for (int u = 1; u < R; ++u)
-            if (u < nk) gload(u * BK, u);
+            if (u < nk) gload((kt0 + u) * BK, u);
         __syncthreads();
         for (int kt = 0; kt < nk; kt += R) {
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const int t = kt + u;
                 if (t >= nk) break;
-                if (t + R < nk) gload((t + R) * BK, u);
+                if (t + R < nk) gload((kt0 + t + R) * BK, u);
                 compute(t & 1);
                 if (t + 1 < nk) sstore((t + 1) & 1, (u + 1) % R);
                 __syncthreads();
             }
         }
     } else {
-        gload(0, 0);
+        gload(kt0 * BK, 0);
         sstore(0, 0);
         __syncthreads();
         for (int kt = 0; kt < nk; ++kt) {
             const int cur = NB == 2 ? (kt & 1) : 0;
-            if (kt + 1 < nk) gload((kt + 1) * BK, 0);
+            if (kt + 1 < nk) gload((kt0 + kt + 1) * BK, 0);
             compute(cur);
             if (NB == 2) {
                 if (kt + 1 < nk) sstore(cur ^ 1, 0);

@@ -385,6 +385,81 @@ __global__ void k_fill(double *p, long n, double v) {
     if (t < n) p[t] = v;
 }
 
+// ---- boundary terms of a Poisson solve in the sine basis (used where the interior data is
+// sparse and its forward transform skips the zero tiles).  apply_bc! (fluid-ops.jl:249-275)
+// adds edge vectors on the first/last rows and columns of the right-hand side: cB at j = 1,
+// cT at j = p, cL at i = 1, cR at i = m.  Their transform S_m·E·S_p is rank 4:
+//   (S_m cB) ⊗ S_p[0,:] + (S_m cT) ⊗ S_p[p-1,:] + S_m[:,0] ⊗ (S_p cL) + S_m[:,m-1] ⊗ (S_p cR)
+// (S symmetric).  k_edge_gemv forms W = [S_m cB | S_m cT | S_p cL | S_p cR] (2m + 2p per
+// column), one workgroup per output entry; k_edge_add adds the rank-4 term, times the
+// eigenvalue scale D, to the scaled spectral array F = (S_m X S_p) ⊙ D.
+__global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restrict__ Sm,
+                                                   const double *__restrict__ Sp,
+                                                   const double *__restrict__ bc, long sbc,
+                                                   double *__restrict__ W, long sW,
+                                                   const int *skip) {
+    SKIP_GUARD
+    __shared__ double sh[2][4];
+    const int x = blockIdx.x, b = blockIdx.y;
+    const double *r = bc + b * sbc;
+    const bool xs = x < d.m;                    // S_m rows first, then S_p rows
+    const int row = xs ? x : x - d.m, n = xs ? d.m : d.p;
+    const double *S = (xs ? Sm : Sp) + (long)row * n;
+    const int o1 = xs ? d.Bo : d.Lo, o2 = xs ? d.To : d.Ro;  // 0-based: entry k at o + 1 + k
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const double sv = S[k];
+        s1 += sv * r[o1 + 1 + k];
+        s2 += sv * r[o2 + 1 + k];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = s1;
+        sh[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double *Wb = W + b * sW;
+        const double t1 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        const double t2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+        if (xs) {
+            Wb[row] = t1;
+            Wb[d.m + row] = t2;
+        } else {
+            Wb[2 * d.m + row] = t1;
+            Wb[2 * d.m + d.p + row] = t2;
+        }
+    }
+}
+
+__global__ void k_edge_add(Dom d, const double *__restrict__ Sm, const double *__restrict__ Sp,
+                           const double *__restrict__ W, long sW, const double *__restrict__ D,
+                           double *__restrict__ F, long sF, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    const int a = (int)(t % d.m), c = (int)(t / d.m);
+    const double *Wb = W + b * sW;
+    const double e = Wb[a] * Sp[c] + Wb[d.m + a] * Sp[(long)(d.p - 1) * d.p + c] +
+                     Sm[a] * Wb[2 * d.m + c] + Sm[(long)(d.m - 1) * d.m + a] * Wb[2 * d.m + d.p + c];
+    F[b * sF + t] += D[t] * e;
+}
+
+void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, const double *bc,
+                          long sbc, double *W, const double *D, double *F, long sF, int nbatch,
+                          const int *skip, hipStream_t s) {
+    const long sW = 2L * (d.m + d.p);
+    hipLaunchKernelGGL(k_edge_gemv, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp, bc, sbc,
+                       W, sW, skip);
+    hipLaunchKernelGGL(k_edge_add, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, Sm, Sp, W,
+                       sW, D, F, sF, skip);
+}
+
 void launch_fill(double *p, long n, double v, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);

@@ -209,12 +209,13 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     return c->tuned[key] = best;
 }
 
-// MFMA flops a launch executes: 2MNK for the full contraction, 2·2·h·h·(data rows) for the
-// split one (two h×h parts)
+// Algorithmic flops of a launch: 2MNK for the full contraction, 2·2·h·k·(data rows) for the
+// split one (two h×k parts; k = h pairs, or the khi - klo pairs a narrowed launch sums)
 double gemm_flops(const GemmArgs &g, int kind) {
     if (kind == 0) return 2.0 * g.M * (double)g.N * g.K * g.batch;
     const double rows = kind == 1 ? g.N : g.M;
-    return 4.0 * g.h * (double)g.h * rows * g.batch;
+    const double kp = g.khi > 0 ? (double)(g.khi - g.klo) : (double)g.h;
+    return 4.0 * g.h * kp * rows * g.batch;
 }
 
 void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
@@ -359,22 +360,53 @@ void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const dou
     }
 }
 
+// Poisson solve ψ = Δinv(G + edges(bc)) on level `lev` of a B-product, where G is zero
+// outside the rows/columns [bx_lo, bx_hi) × [by_lo, by_hi) of that level: the two forward
+// contractions sum only those K tiles, and the boundary terms enter in the sine basis as a
+// rank-4 update of the scaled spectrum (bc null: zero boundary values).  Up to roundoff this
+// is the LaplacianInv apply of G + edges (fluid-ops.jl:42-56, 249-275).
+void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi, int nc, int lev,
+                   const double *bc, long sbc, const int *skip, int prec) {
+    LapInvPlan P;
+    lapinv_plan(c, G, sG, Psi, sPsi, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+    P.g[0].klo = c->bx_lo[lev] / 2;
+    P.g[0].khi = (c->bx_hi[lev] + 1) / 2;
+    P.g[1].klo = c->by_lo[lev] / 2;
+    P.g[1].khi = (c->by_hi[lev] + 1) / 2;
+    gemm(c, P.g[0], P.epi[0], prec, true, P.kind[0]);
+    gemm(c, P.g[1], P.epi[1], prec, false, P.kind[1]);
+    if (bc)
+        launch_edge_spectral(c->d, c->Sm, c->Sp, bc, sbc, c->edgeW, c->sD, c->T2, c->d.ng, nc,
+                             skip, c->s);
+    gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
+    gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
+}
+
 // `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
+// `sparse` (B-products): levels ≥ 1 hold only the coarsified image of level 1, so their
+// solves go through lapinv_sparse.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
-               const int *skip, int prec = -1, int refine = 0, int nlev = -1) {
+               const int *skip, int prec = -1, int refine = 0, int nlev = -1,
+               bool sparse = false) {
     const Dom &d = c->d;
     const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
     const long ng = d.ng, nq = d.nq;
     const long sG = Ls * ng, sQ = curl_all ? Ls * nq : nq, sB = (long)Ls * d.nbc;
+    if (prec < 0) prec = c->precision;
+    sparse = sparse && c->use_eo && c->sparse_B && refine == 0;
     for (int lev = 1; lev < L; ++lev)
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
     double *bcL = bc + (long)(L - 1) * d.nbc;
     launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
-    lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
-           prec);
+    if (sparse && L > 1)
+        lapinv_sparse(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, nc, L - 1, nullptr, 0,
+                      skip, prec);
+    else
+        lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr,
+               skip, prec);
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
     if (curl_all || L == 1)
@@ -383,8 +415,13 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     for (int lev = L - 2; lev >= 0; --lev) {
         double *bcl = bc + (long)lev * d.nbc;
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
-        launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
-        lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, prec);
+        if (sparse && lev >= 1) {
+            lapinv_sparse(c, G + lev * ng, sG, Psi + lev * ng, sG, nc, lev, bcl, sB, skip, prec);
+        } else {
+            launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
+            lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
+                   prec);
+        }
         if (refine && lev < c->refine_levels)
             refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine, prec);
         if (curl_all || lev == 0)
@@ -407,7 +444,7 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     for (int col = 0; col < nc; ++col)
         if (Lv > 1) launch_fill(c->gB + col * L * ng + ng, (Lv - 1) * ng, 0.0, c->s);
     lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
-    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv);
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv, true);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -438,6 +475,7 @@ int ensure_B_work(ibh_ctx *c, int nc) {
     ALLOC(c->qLB, (size_t)nc * d.nq);
     ALLOC(c->bcB, (size_t)nc * L * d.nbc);
     ALLOC(c->zB, (size_t)nc * 2 * c->nb + 1);
+    ALLOC(c->edgeW, (size_t)nc * 2 * (d.m + d.p));
     if (nc > c->etcap) {
         ALLOC(c->etscale, nc);
         c->etcap = nc;
@@ -941,6 +979,27 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
                     : D.refine_sweeps < 0 ? 0 : (D.precision == IBH_FP32 ? 1 : 2);
     if (const char *rf = std::getenv("IBH_REFINE")) c->refine = std::max(0, std::atoi(rf));
     c->refine_levels = D.refine_levels > 0 ? std::min(D.refine_levels, c->L) : c->L;
+    {
+        // the region of each level that a B-product's Vort2Flux can make nonzero: level 0 in
+        // full, level ℓ the coarsify! image (fluid-ops.jl:184-202) of level ℓ-1 — coarse ic
+        // (1-based) reads fine i-1..i+1 with i = 2(ic - n/4) and is written for
+        // ic in [n/4+1, 3n/4-1] — widened by one cell; 0-based [lo, hi)
+        auto coarse = [](int n, int lo, int hi, int *clo, int *chi) {
+            const int a = std::max(n / 4 + 1, n / 4 + (lo + 1) / 2) - 1;
+            const int b = std::min(3 * n / 4 - 1, n / 4 + (hi + 1) / 2) + 1;
+            *clo = std::max(1, a) - 1;
+            *chi = std::min(n - 1, b);
+        };
+        c->bx_lo.assign(c->L, 0);
+        c->bx_hi.assign(c->L, D.nx - 1);
+        c->by_lo.assign(c->L, 0);
+        c->by_hi.assign(c->L, D.ny - 1);
+        for (int l = 1; l < c->L; ++l) {
+            coarse(D.nx, c->bx_lo[l - 1], c->bx_hi[l - 1], &c->bx_lo[l], &c->bx_hi[l]);
+            coarse(D.ny, c->by_lo[l - 1], c->by_hi[l - 1], &c->by_lo[l], &c->by_hi[l]);
+        }
+        if (const char *sb = std::getenv("IBH_SPARSE_B")) c->sparse_B = std::atoi(sb) != 0;
+    }
     if (const char *rl = std::getenv("IBH_REFINE_LEVELS"))
         c->refine_levels = std::min(std::max(1, std::atoi(rl)), c->L);
     {

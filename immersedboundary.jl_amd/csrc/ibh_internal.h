@@ -52,6 +52,9 @@ struct GemmArgs {
     const double *So, *Se;
     int h;
     int batch;
+    // even/odd split: only pair indices [klo, khi) of the data's contiguous dimension are
+    // summed (the data is zero elsewhere); khi = 0 sums all of them
+    int klo, khi;
     // profiling only: a PROF_REC record of device wall-clock ticks, zeroed beforehand: the
     // start of workgroup 0 (the first dispatched) at ts[0], each workgroup's end folded by
     // atomicMax into one of 16 sub-slots (ts[16 + 16·j], 128 B apart, so that the end
@@ -100,6 +103,12 @@ void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, l
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
                       hipStream_t s);
 void launch_fill(double *p, long n, double v, hipStream_t s);
+// F[b] += D ⊙ (S_m E_b S_p) for the edge field E_b of the bc array bc[b] (apply_bc! with fac 1):
+// the rank-4 sine transform of a Poisson right-hand side's boundary terms.  W: work of
+// 2(m + p) doubles per batch entry; D: the m×p scale table (shared by the batch).
+void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, const double *bc,
+                          long sbc, double *W, const double *D, double *F, long sF, int nbatch,
+                          const int *skip, hipStream_t s);
 // trial right-hand side of L levels: out = A_ℓ·g + f1·N1 + f2·N2 + edges(bcacc), A_ℓ as the
 // exact 5-point stencil I − a_ℓ(−L), a_ℓ = dt/(2 Re (h·2^ℓ)²)
 void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
