@@ -79,11 +79,12 @@ struct ibh_ctx {
     // B-product work (batched over nc columns)
     int ncB = 0;
     double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,
-           *bcB = nullptr, *g1B = nullptr, *edgeW = nullptr;
+           *bcB = nullptr, *g1B = nullptr, *edgeW = nullptr, *lineW = nullptr;
     // In a B-product, level ℓ ≥ 1 of Γ is zero outside the region that the coarsify! chain
     // writes: 0-based index ranges [lo, hi) per level, in x (i) and y (j)
     std::vector<int> bx_lo, bx_hi, by_lo, by_hi;
     bool sparse_B = true;   // skip those zero tiles in the B-products' forward transforms
+    bool lines_B = true;    // ... and form coarse-level ψ only on the lines get_bc! reads
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
     int *d_boff = nullptr, *d_npts = nullptr;

@@ -460,6 +460,136 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
                        sW, D, F, sF, skip);
 }
 
+// ---- ψ = S_m F S_p on four lines only: the rows i = nx/4, 3nx/4 and the columns j = ny/4,
+// 3ny/4 (1-based), the values get_bc! reads (fluid-ops.jl:210-239).  Coarse levels of a
+// B-product need nothing else, so this replaces the two inverse contractions there.  Four
+// deterministic passes (no atomics): column partial sums v_J = F·S_p[:, J] over LN_CH chunks of
+// the column index, their sum, the row sums w_I = S_m[I, :]·F (one wave per column), and the
+// line values S_m·v_J and w_I·S_p (one wave per output).  Work per batch entry:
+// 2·LN_CH·m + 2m + 2p doubles.
+constexpr int LN_CH = 16;
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__global__ void k_lines_colpart(Dom d, const double *__restrict__ F, long sF,
+                                const double *__restrict__ Sp, double *__restrict__ W, long sW,
+                                const int *skip) {
+    SKIP_GUARD
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y, bt = blockIdx.z;
+    if (a >= d.m) return;
+    const double *Fb = F + bt * sF;
+    const int J1 = d.ny / 4 - 1, J2 = 3 * d.ny / 4 - 1;
+    const int b0 = (int)((long)ch * d.p / LN_CH), b1 = (int)((long)(ch + 1) * d.p / LN_CH);
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        const double f = Fb[a + (long)d.m * b];
+        s1 += f * Sp[(long)J1 * d.p + b];
+        s2 += f * Sp[(long)J2 * d.p + b];
+    }
+    double *Wb = W + bt * sW;
+    Wb[(2 * ch) * (long)d.m + a] = s1;
+    Wb[(2 * ch + 1) * (long)d.m + a] = s2;
+}
+
+__global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int *skip) {
+    SKIP_GUARD
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, bt = blockIdx.y;
+    if (a >= d.m) return;
+    double *Wb = W + bt * sW;
+    double s1 = 0.0, s2 = 0.0;
+    for (int ch = 0; ch < LN_CH; ++ch) {
+        s1 += Wb[(2 * ch) * (long)d.m + a];
+        s2 += Wb[(2 * ch + 1) * (long)d.m + a];
+    }
+    Wb[2L * LN_CH * d.m + a] = s1;          // v_J1
+    Wb[(2L * LN_CH + 1) * d.m + a] = s2;    // v_J2
+}
+
+__global__ void k_lines_row(Dom d, const double *__restrict__ F, long sF,
+                            const double *__restrict__ Sm, double *__restrict__ W, long sW,
+                            const int *skip) {
+    SKIP_GUARD
+    const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    if (b >= d.p) return;
+    const double *Fc = F + bt * sF + (long)d.m * b;
+    const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
+    double s1 = 0.0, s2 = 0.0;
+    for (int a = lane; a < d.m; a += 64) {
+        const double f = Fc[a];
+        s1 += Sm[(long)I1 * d.m + a] * f;
+        s2 += Sm[(long)I2 * d.m + a] * f;
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
+        Wr[b] = s1;              // w_I1
+        Wr[d.p + b] = s2;        // w_I2
+    }
+}
+
+__global__ void k_lines_out(Dom d, const double *__restrict__ Sm, const double *__restrict__ Sp,
+                            const double *__restrict__ W, long sW, double *__restrict__ psi,
+                            long spsi, const int *skip) {
+    SKIP_GUARD
+    const int o = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    if (o >= d.m + d.p) return;
+    const double *Wb = W + bt * sW;
+    double *ps = psi + bt * spsi;
+    double s1 = 0.0, s2 = 0.0;
+    if (o < d.m) {  // ψ(o, J) = Σ_a S_m[o, a] v_J[a]
+        const double *v1 = Wb + 2L * LN_CH * d.m, *v2 = v1 + d.m;
+        for (int a = lane; a < d.m; a += 64) {
+            const double sv = Sm[(long)o * d.m + a];
+            s1 += sv * v1[a];
+            s2 += sv * v2[a];
+        }
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            ps[o + (long)d.m * (d.ny / 4 - 1)] = s1;
+            ps[o + (long)d.m * (3 * d.ny / 4 - 1)] = s2;
+        }
+    } else {        // ψ(I, c) = Σ_b w_I[b] S_p[b, c]
+        const int c = o - d.m;
+        const double *w1 = Wb + (2L * LN_CH + 2) * d.m, *w2 = w1 + d.p;
+        for (int b = lane; b < d.p; b += 64) {
+            const double sv = Sp[(long)c * d.p + b];
+            s1 += w1[b] * sv;
+            s2 += w2[b] * sv;
+        }
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        // the four crossings with the columns belong to the column part (one writer each,
+        // so the result does not depend on which wave finishes last)
+        if (lane == 0 && c != d.ny / 4 - 1 && c != 3 * d.ny / 4 - 1) {
+            ps[(d.nx / 4 - 1) + (long)d.m * c] = s1;
+            ps[(3 * d.nx / 4 - 1) + (long)d.m * c] = s2;
+        }
+    }
+}
+
+long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d.p; }
+
+void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
+                  double *psi, long spsi, double *W, int nbatch, const int *skip,
+                  hipStream_t s) {
+    const long sW = lines_work_size(d);
+    hipLaunchKernelGGL(k_lines_colpart, dim3(nblk(d.m, 256), LN_CH, nbatch), dim3(256), 0, s, d, F,
+                       sF, Sp, W, sW, skip);
+    hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 256), nbatch), dim3(256), 0, s, d, W, sW,
+                       skip);
+    hipLaunchKernelGGL(k_lines_row, dim3(nblk(d.p, 4), nbatch), dim3(256), 0, s, d, F, sF, Sm, W,
+                       sW, skip);
+    hipLaunchKernelGGL(k_lines_out, dim3(nblk(d.m + d.p, 4), nbatch), dim3(256), 0, s, d, Sm, Sp,
+                       W, sW, psi, spsi, skip);
+}
+
 void launch_fill(double *p, long n, double v, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);

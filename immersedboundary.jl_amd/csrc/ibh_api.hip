@@ -364,9 +364,10 @@ void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const dou
 // outside the rows/columns [bx_lo, bx_hi) × [by_lo, by_hi) of that level: the two forward
 // contractions sum only those K tiles, and the boundary terms enter in the sine basis as a
 // rank-4 update of the scaled spectrum (bc null: zero boundary values).  Up to roundoff this
-// is the LaplacianInv apply of G + edges (fluid-ops.jl:42-56, 249-275).
+// is the LaplacianInv apply of G + edges (fluid-ops.jl:42-56, 249-275).  With `lines_only`
+// the inverse transform is evaluated only on the rows and columns get_bc! reads.
 void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi, int nc, int lev,
-                   const double *bc, long sbc, const int *skip, int prec) {
+                   const double *bc, long sbc, const int *skip, int prec, bool lines_only) {
     LapInvPlan P;
     lapinv_plan(c, G, sG, Psi, sPsi, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
     P.g[0].klo = c->bx_lo[lev] / 2;
@@ -378,14 +379,19 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
     if (bc)
         launch_edge_spectral(c->d, c->Sm, c->Sp, bc, sbc, c->edgeW, c->sD, c->T2, c->d.ng, nc,
                              skip, c->s);
+    if (lines_only) {
+        launch_lines(c->d, c->Sm, c->Sp, c->T2, c->d.ng, Psi, sPsi, c->lineW, nc, skip, c->s);
+        return;
+    }
     gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
     gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
 }
 
 // `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
-// `sparse` (B-products): levels ≥ 1 hold only the coarsified image of level 1, so their
-// solves go through lapinv_sparse.
+// `sparse` (B-products, curl on level 1 only): levels ≥ 1 hold only the coarsified image of
+// level 1, so their solves go through lapinv_sparse, and their ψ, which only feeds get_bc!
+// for the next finer level, is formed on the lines get_bc! reads.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
                const int *skip, int prec = -1, int refine = 0, int nlev = -1,
                bool sparse = false) {
@@ -394,7 +400,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     const long ng = d.ng, nq = d.nq;
     const long sG = Ls * ng, sQ = curl_all ? Ls * nq : nq, sB = (long)Ls * d.nbc;
     if (prec < 0) prec = c->precision;
-    sparse = sparse && c->use_eo && c->sparse_B && refine == 0;
+    sparse = sparse && c->use_eo && c->sparse_B && refine == 0 && !curl_all;
     for (int lev = 1; lev < L; ++lev)
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
@@ -403,7 +409,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
     if (sparse && L > 1)
         lapinv_sparse(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, nc, L - 1, nullptr, 0,
-                      skip, prec);
+                      skip, prec, c->lines_B);
     else
         lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr,
                skip, prec);
@@ -416,7 +422,8 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         double *bcl = bc + (long)lev * d.nbc;
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
         if (sparse && lev >= 1) {
-            lapinv_sparse(c, G + lev * ng, sG, Psi + lev * ng, sG, nc, lev, bcl, sB, skip, prec);
+            lapinv_sparse(c, G + lev * ng, sG, Psi + lev * ng, sG, nc, lev, bcl, sB, skip, prec,
+                          c->lines_B);
         } else {
             launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
             lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
@@ -463,11 +470,19 @@ int check_flags(ibh_ctx *c) {
 
 void drop_graphs(ibh_ctx *c);
 
+void dfree(ibh_ctx *c, void *p);
+
 int ensure_B_work(ibh_ctx *c, int nc) {
     if (c->ncB >= nc) return IBH_OK;
     drop_graphs(c);  // captured graphs hold the old work pointers
+    HIPCK(hipStreamSynchronize(c->s));
     const Dom &d = c->d;
     const long L = c->L;
+    for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB, &c->edgeW,
+                       &c->lineW}) {
+        dfree(c, *p);
+        *p = nullptr;
+    }
     ALLOC(c->qB, (size_t)nc * d.nq);
     ALLOC(c->g1B, (size_t)nc * d.ng);
     ALLOC(c->gB, (size_t)nc * L * d.ng);
@@ -476,16 +491,23 @@ int ensure_B_work(ibh_ctx *c, int nc) {
     ALLOC(c->bcB, (size_t)nc * L * d.nbc);
     ALLOC(c->zB, (size_t)nc * 2 * c->nb + 1);
     ALLOC(c->edgeW, (size_t)nc * 2 * (d.m + d.p));
+    ALLOC(c->lineW, (size_t)nc * lines_work_size(d));
     if (nc > c->etcap) {
+        dfree(c, c->etscale);
         ALLOC(c->etscale, nc);
         c->etcap = nc;
     }
     if ((long)nc * d.ng > c->Tcap) {
+        dfree(c, c->T1);
+        dfree(c, c->T2);
         ALLOC(c->T1, (size_t)nc * d.ng);
         ALLOC(c->T2, (size_t)nc * d.ng);
         c->Tcap = (long)nc * d.ng;
     }
-    if ((long)nc * d.ng > (long)L * d.ng) ALLOC(c->gtmp, (size_t)nc * d.ng);
+    if ((long)nc * d.ng > (long)L * d.ng) {
+        dfree(c, c->gtmp);
+        ALLOC(c->gtmp, (size_t)nc * d.ng);
+    }
     c->ncB = nc;
     return IBH_OK;
 }
@@ -581,7 +603,8 @@ void dfree(ibh_ctx *c, void *p) {
 int release_B_work(ibh_ctx *c) {
     HIPCK(hipStreamSynchronize(c->s));
     drop_graphs(c);
-    for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB}) {
+    for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB, &c->edgeW,
+                       &c->lineW}) {
         dfree(c, *p);
         *p = nullptr;
     }
@@ -999,6 +1022,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             coarse(D.ny, c->by_lo[l - 1], c->by_hi[l - 1], &c->by_lo[l], &c->by_hi[l]);
         }
         if (const char *sb = std::getenv("IBH_SPARSE_B")) c->sparse_B = std::atoi(sb) != 0;
+        if (const char *sl = std::getenv("IBH_SPARSE_LINES")) c->lines_B = std::atoi(sl) != 0;
     }
     if (const char *rl = std::getenv("IBH_REFINE_LEVELS"))
         c->refine_levels = std::min(std::max(1, std::atoi(rl)), c->L);

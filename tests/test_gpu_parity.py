@@ -124,6 +124,33 @@ def test_regularization_E_ET_B(ibhip):
     assert rel(case.B_times(fb), orc.B_times(fb)) < 1e-11
 
 
+def test_b_product_four_levels_reproducible(ibhip, monkeypatch):
+    # four levels: the B-product's coarse levels take the sparse forward transforms, the
+    # boundary terms in the sine basis and ψ on the lines get_bc! reads; it matches the oracle,
+    # repeats bitwise, and does not depend on the tuner's tile choice
+    prob = cylinder_case(n=128, nlev=4)
+    z = np.random.default_rng(3).standard_normal(2 * prob.bodies.npoints)
+    r = prob.stacked_points.T.reshape(-1).copy()
+    orc = ibref.OracleRun(prob, setup=False)
+    orc.reg_update(r)
+    want = orc.B_times(z)
+
+    def run(v=None):
+        if v is not None:
+            monkeypatch.setenv("IBH_GEMM_VARIANT", str(v))
+        c = ibhip.DeviceCase(prob)
+        c.reg_update(r)
+        a, b = c.B_times(z), c.B_times(z)
+        c.close()
+        assert np.array_equal(a, b)
+        return a
+
+    ref = run()
+    assert rel(ref, want) < 1e-11
+    for v in (0, 9, 20, 30, 39):
+        assert np.array_equal(run(v), ref)
+
+
 def run_pair(ibhip, prob, nsteps, t0=0.0):
     import ibhip as m
 
