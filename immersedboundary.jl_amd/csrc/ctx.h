@@ -79,7 +79,8 @@ struct ibh_ctx {
     // B-product work (batched over nc columns)
     int ncB = 0;
     double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,
-           *bcB = nullptr, *g1B = nullptr, *edgeW = nullptr, *lineW = nullptr;
+           *bcB = nullptr, *g1B = nullptr, *edgeW = nullptr, *lineW = nullptr,
+           *specB = nullptr;  // scaled spectrum of the B-product's Ainv₁ (nc × nΓ)
     // In a B-product, level ℓ ≥ 1 of Γ is zero outside the region that the coarsify! chain
     // writes: 0-based index ranges [lo, hi) per level, in x (i) and y (j)
     std::vector<int> bx_lo, bx_hi, by_lo, by_hi;

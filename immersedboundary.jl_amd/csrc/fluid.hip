@@ -576,6 +576,22 @@ __global__ void k_lines_out(Dom d, const double *__restrict__ Sm, const double *
 
 long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d.p; }
 
+__global__ void k_spectral_scale(Dom d, const double *__restrict__ A, long sA,
+                                 const double *__restrict__ D, double kappa,
+                                 double *__restrict__ F, long sF, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    F[b * sF + t] = kappa * A[b * sA + t] * D[t];
+}
+
+void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
+                           double *F, long sF, int nbatch, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_spectral_scale, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, A, sA,
+                       D, kappa, F, sF, skip);
+}
+
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
                   double *psi, long spsi, double *W, int nbatch, const int *skip,
                   hipStream_t s) {

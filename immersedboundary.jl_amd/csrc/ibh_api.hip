@@ -391,10 +391,13 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
 // `sparse` (B-products, curl on level 1 only): levels ≥ 1 hold only the coarsified image of
 // level 1, so their solves go through lapinv_sparse, and their ψ, which only feeds get_bc!
-// for the next finer level, is formed on the lines get_bc! reads.
+// for the next finer level, is formed on the lines get_bc! reads.  `spec0`, with sparse: the
+// scaled spectrum F of level 1's Γ = S F S (the Ainv₁ that produced it); S·S = (n/2)·I per
+// dimension, so the Poisson solve of level 1 needs only the inverse transform of
+// (nx·ny/4)·F ⊙ sD plus the boundary terms in the sine basis.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
                const int *skip, int prec = -1, int refine = 0, int nlev = -1,
-               bool sparse = false) {
+               bool sparse = false, const double *spec0 = nullptr) {
     const Dom &d = c->d;
     const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
     const long ng = d.ng, nq = d.nq;
@@ -424,6 +427,15 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         if (sparse && lev >= 1) {
             lapinv_sparse(c, G + lev * ng, sG, Psi + lev * ng, sG, nc, lev, bcl, sB, skip, prec,
                           c->lines_B);
+        } else if (sparse && spec0) {
+            LapInvPlan P;
+            lapinv_plan(c, nullptr, 0, Psi, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+            launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
+                                  c->s);
+            launch_edge_spectral(d, c->Sm, c->Sp, bcl, sB, c->edgeW, c->sD, c->T2, ng, nc, skip,
+                                 c->s);
+            gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
+            gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
         } else {
             launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
             lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
@@ -450,8 +462,20 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
     for (int col = 0; col < nc; ++col)
         if (Lv > 1) launch_fill(c->gB + col * L * ng + ng, (Lv - 1) * ng, 0.0, c->s);
-    lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
-    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv, true);
+    // Ainv₁ with its scaled spectrum kept (specB) for the level-1 Poisson solve
+    const bool fuse = c->use_eo && c->sparse_B && Lv > 1;
+    if (fuse) {
+        LapInvPlan P;
+        lapinv_plan(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, &P);
+        P.g[1].C = c->specB;
+        P.g[2].B = c->specB;
+        if (prec < 0) prec = c->precision;
+        for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], prec, P.kind[k] == 1, P.kind[k]);
+    } else {
+        lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
+    }
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv, true,
+              fuse ? c->specB : nullptr);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -479,7 +503,7 @@ int ensure_B_work(ibh_ctx *c, int nc) {
     const Dom &d = c->d;
     const long L = c->L;
     for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB, &c->edgeW,
-                       &c->lineW}) {
+                       &c->lineW, &c->specB}) {
         dfree(c, *p);
         *p = nullptr;
     }
@@ -492,6 +516,7 @@ int ensure_B_work(ibh_ctx *c, int nc) {
     ALLOC(c->zB, (size_t)nc * 2 * c->nb + 1);
     ALLOC(c->edgeW, (size_t)nc * 2 * (d.m + d.p));
     ALLOC(c->lineW, (size_t)nc * lines_work_size(d));
+    ALLOC(c->specB, (size_t)nc * d.ng);
     if (nc > c->etcap) {
         dfree(c, c->etscale);
         ALLOC(c->etscale, nc);
@@ -604,7 +629,7 @@ int release_B_work(ibh_ctx *c) {
     HIPCK(hipStreamSynchronize(c->s));
     drop_graphs(c);
     for (double **p : {&c->qB, &c->g1B, &c->gB, &c->psiB, &c->qLB, &c->bcB, &c->zB, &c->edgeW,
-                       &c->lineW}) {
+                       &c->lineW, &c->specB}) {
         dfree(c, *p);
         *p = nullptr;
     }

@@ -113,6 +113,9 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
 // values get_bc! reads); other entries of ψ[b] are left as they were.  W: lines_work_size(d)
 // doubles per batch entry.
 long lines_work_size(const Dom &d);
+// F[b] = kappa·A[b] ⊙ D (D shared by the batch)
+void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
+                           double *F, long sF, int nbatch, const int *skip, hipStream_t s);
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
                   double *psi, long spsi, double *W, int nbatch, const int *skip, hipStream_t s);
 // trial right-hand side of L levels: out = A_ℓ·g + f1·N1 + f2·N2 + edges(bcacc), A_ℓ as the
