@@ -410,12 +410,19 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     double *bcL = bc + (long)(L - 1) * d.nbc;
     launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
-    if (sparse && L > 1)
+    if (sparse && L > 1) {
         lapinv_sparse(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, nc, L - 1, nullptr, 0,
                       skip, prec, c->lines_B);
-    else
+    } else if (sparse && spec0) {  // one level: Γ = S F S, zero boundary values
+        LapInvPlan P;
+        lapinv_plan(c, nullptr, 0, Psi, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+        launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip, c->s);
+        gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
+        gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
+    } else {
         lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr,
                skip, prec);
+    }
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
     if (curl_all || L == 1)
@@ -463,7 +470,7 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     for (int col = 0; col < nc; ++col)
         if (Lv > 1) launch_fill(c->gB + col * L * ng + ng, (Lv - 1) * ng, 0.0, c->s);
     // Ainv₁ with its scaled spectrum kept (specB) for the level-1 Poisson solve
-    const bool fuse = c->use_eo && c->sparse_B && Lv > 1;
+    const bool fuse = c->use_eo && c->sparse_B;
     if (fuse) {
         LapInvPlan P;
         lapinv_plan(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, &P);
