@@ -76,6 +76,9 @@ struct ibh_ctx {
     int refine = 0;          // FP64-residual refinement sweeps of the low-precision solves
     int refine_levels = 1;   // ... on the finest refine_levels levels
     double *rres = nullptr;  // refinement residuals, L levels
+    // scaled spectra kept for the level-1 Poisson solves (trial Ainv over L levels, projection)
+    // and the step's work for ψ on get_bc!'s lines and for boundary terms in the sine basis
+    double *specT = nullptr, *specP = nullptr, *lineS = nullptr, *edgeS = nullptr;
     // B-product work (batched over nc columns)
     int ncB = 0;
     double *zB = nullptr, *qB = nullptr, *gB = nullptr, *psiB = nullptr, *qLB = nullptr,

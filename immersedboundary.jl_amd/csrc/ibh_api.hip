@@ -389,40 +389,71 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
 
 // `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
-// `sparse` (B-products, curl on level 1 only): levels ≥ 1 hold only the coarsified image of
-// level 1, so their solves go through lapinv_sparse, and their ψ, which only feeds get_bc!
-// for the next finer level, is formed on the lines get_bc! reads.  `spec0`, with sparse: the
-// scaled spectrum F of level 1's Γ = S F S (the Ainv₁ that produced it); S·S = (n/2)·I per
-// dimension, so the Poisson solve of level 1 needs only the inverse transform of
-// (nx·ny/4)·F ⊙ sD plus the boundary terms in the sine basis.
+// Options that change only the order of roundoff:
+// * `sparse` (B-products): levels ≥ 1 hold only the coarsified image of level 1, so their
+//   solves go through lapinv_sparse;
+// * `lines_ok` (callers that curl level 1 only): a coarser level's ψ only feeds get_bc! for the
+//   next finer level, so it is formed on the lines get_bc! reads (levels that are refined
+//   excepted);
+// * `spec0`: the scaled spectrum F of level 1's Γ = S F S (the Ainv that produced it, stride
+//   nΓ per column); S·S = (n/2)·I per dimension, so level 1's Poisson solve needs only the
+//   inverse transform of (nx·ny/4)·F ⊙ sD plus the boundary terms in the sine basis.
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
                const int *skip, int prec = -1, int refine = 0, int nlev = -1,
-               bool sparse = false, const double *spec0 = nullptr) {
+               bool sparse = false, const double *spec0 = nullptr, bool lines_ok = false) {
     const Dom &d = c->d;
     const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
     const long ng = d.ng, nq = d.nq;
     const long sG = Ls * ng, sQ = curl_all ? Ls * nq : nq, sB = (long)Ls * d.nbc;
     if (prec < 0) prec = c->precision;
-    sparse = sparse && c->use_eo && c->sparse_B && refine == 0 && !curl_all;
+    const bool bsp = sparse && c->use_eo && c->sparse_B && refine == 0 && !curl_all;
+    const bool lines = lines_ok && c->use_eo && c->lines_B && !curl_all;
+    const bool fuse0 = spec0 && c->use_eo && !(refine && 0 < c->refine_levels);
+    // B-products work on their column batch's buffers, the step on its own
+    double *lineW = sparse ? c->lineW : c->lineS, *edgeW = sparse ? c->edgeW : c->edgeS;
+    auto solve = [&](int lev, const double *bcl) {
+        double *Gl = G + lev * ng, *Pl = Psi + lev * ng;
+        const bool ln = lines && lev >= 1 && !(refine && lev < c->refine_levels);
+        if (bsp && lev >= 1) {
+            lapinv_sparse(c, Gl, sG, Pl, sG, nc, lev, bcl, sB, skip, prec, ln);
+            return;
+        }
+        if (lev == 0 && fuse0) {
+            LapInvPlan P;
+            lapinv_plan(c, nullptr, 0, Pl, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+            launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
+                                  c->s);
+            if (bcl)
+                launch_edge_spectral(d, c->Sm, c->Sp, bcl, sB, edgeW, c->sD, c->T2, ng, nc, skip,
+                                     c->s);
+            gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
+            gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
+            return;
+        }
+        const double *in = Gl;
+        long sin = sG;
+        if (bcl) {
+            launch_copy_apply_bc(d, Gl, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
+            in = c->gtmp;
+            sin = ng;
+        }
+        if (ln) {
+            LapInvPlan P;
+            lapinv_plan(c, in, sin, Pl, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+            gemm(c, P.g[0], P.epi[0], prec, true, P.kind[0]);
+            gemm(c, P.g[1], P.epi[1], prec, false, P.kind[1]);
+            launch_lines(d, c->Sm, c->Sp, c->T2, ng, Pl, sG, lineW, nc, skip, c->s);
+            return;
+        }
+        lapinv(c, in, sin, Pl, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, prec);
+    };
     for (int lev = 1; lev < L; ++lev)
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
     double *bcL = bc + (long)(L - 1) * d.nbc;
     launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
     for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
-    if (sparse && L > 1) {
-        lapinv_sparse(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, nc, L - 1, nullptr, 0,
-                      skip, prec, c->lines_B);
-    } else if (sparse && spec0) {  // one level: Γ = S F S, zero boundary values
-        LapInvPlan P;
-        lapinv_plan(c, nullptr, 0, Psi, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
-        launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip, c->s);
-        gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
-        gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
-    } else {
-        lapinv(c, G + (L - 1) * ng, sG, Psi + (L - 1) * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr,
-               skip, prec);
-    }
+    solve(L - 1, nullptr);
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
     if (curl_all || L == 1)
@@ -431,23 +462,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     for (int lev = L - 2; lev >= 0; --lev) {
         double *bcl = bc + (long)lev * d.nbc;
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
-        if (sparse && lev >= 1) {
-            lapinv_sparse(c, G + lev * ng, sG, Psi + lev * ng, sG, nc, lev, bcl, sB, skip, prec,
-                          c->lines_B);
-        } else if (sparse && spec0) {
-            LapInvPlan P;
-            lapinv_plan(c, nullptr, 0, Psi, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
-            launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
-                                  c->s);
-            launch_edge_spectral(d, c->Sm, c->Sp, bcl, sB, c->edgeW, c->sD, c->T2, ng, nc, skip,
-                                 c->s);
-            gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
-            gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
-        } else {
-            launch_copy_apply_bc(d, G + lev * ng, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
-            lapinv(c, c->gtmp, ng, Psi + lev * ng, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip,
-                   prec);
-        }
+        solve(lev, bcl);
         if (refine && lev < c->refine_levels)
             refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine, prec);
         if (curl_all || lev == 0)
@@ -482,7 +497,7 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
         lapinv(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, prec);
     }
     vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, skip, prec, 0, Lv, true,
-              fuse ? c->specB : nullptr);
+              fuse ? c->specB : nullptr, true);
     launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
 }
 
@@ -753,7 +768,19 @@ void seg_trial(ibh_ctx *c) {
     // A_ℓΓ by its exact stencil, fused with the right-hand side terms; then Ainv_ℓ
     launch_rhs(d, c->gam, rx.N1, rx.N2, rx.bcacc, c->rhsb, L, c->h, c->dt, c->Re, rx.f1, rx.f2,
                c->s);
-    lapinv(c, c->rhsb, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
+    // without refinement Γs₁ = S F S exactly for the trial spectrum F, which is kept (specT)
+    // for the Poisson solves of level 1 in this step's two Vort2Flux passes
+    const bool fuse = c->use_eo && c->refine == 0;
+    if (fuse) {
+        LapInvPlan P;
+        lapinv_plan(c, c->rhsb, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr, &P);
+        P.g[1].C = c->specT;
+        P.g[2].B = c->specT;
+        for (int k = 0; k < 4; ++k)
+            gemm(c, P.g[k], P.epi[k], c->precision, P.kind[k] == 1, P.kind[k]);
+    } else {
+        lapinv(c, c->rhsb, ng, c->gs, ng, c->sAinv, ng, L, EPI_STORE, nullptr, nullptr);
+    }
     // low-precision transforms: the finest level's two solves are refined against their FP64
     // residuals (the forces see the trial flux at the body through Binv, which amplifies
     // high-frequency noise there; coarser levels reach it only through smooth boundary values)
@@ -761,7 +788,8 @@ void seg_trial(ibh_ctx *c) {
         refine_sweeps(c, c->gs, c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, ng,
                       std::min(c->refine_levels, L), c->refine, c->precision);
     trial_swap(c);
-    vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr, -1, c->refine);
+    vort2flux(c, c->gs, c->psi, c->qs, c->psibc, 1, false, nullptr, -1, c->refine, -1, false,
+              fuse ? c->specT : nullptr, true);
     // ---- BoundaryForcer right-hand side (timestepping.jl:106-139): E(qs + q0) [- v·h]
     const int nf = 2 * c->nb;
     if (nf) {
@@ -804,15 +832,29 @@ void seg_post(ibh_ctx *c) {
                       c->CL, c->fbs, c->s);
     // ---- CircProjecter (timestepping.jl:156-171): Γ = Γs; Γ₁ -= Ainv₁ C' Eᵀ Fb
     post_swap(c);
+    const bool fuse = c->use_eo && c->refine == 0;  // Γ₁'s spectrum known (seg_trial)
     if (nf) {
         launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s);
         launch_rot(d, c->qtmp, d.nq, c->g1, ng, 1, nullptr, c->s);
-        lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
-               IBH_FP64);
+        if (fuse) {
+            // Γ₁ = S(F_trial − F_proj)S: keep the projection's spectrum and subtract it
+            LapInvPlan P;
+            lapinv_plan(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
+                        &P);
+            P.g[1].C = c->specP;
+            P.g[2].B = c->specP;
+            for (int k = 0; k < 4; ++k)
+                gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
+            launch_axpy_sub((int)ng, c->specT, c->specP, 1.0, c->s);
+        } else {
+            lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
+                   IBH_FP64);
+        }
     }
     // ---- vort2flux!(qty) (fluid-ops.jl:438); the finest level is refined like the trial's (its
     // flux feeds the next step's nonlinear term next to the bodies)
-    vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr, -1, c->refine);
+    vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr, -1, c->refine, -1, false,
+              fuse ? c->specT : nullptr);
 }
 
 // status word: [CG done, CG iterations, error flags]
@@ -1113,6 +1155,10 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->etscale, 1));
     c->etcap = 1;
     CK(dalloc(c, &c->rhsb, L * ng));
+    CK(dalloc(c, &c->specT, L * ng));
+    CK(dalloc(c, &c->specP, ng));
+    CK(dalloc(c, &c->lineS, lines_work_size(d)));
+    CK(dalloc(c, &c->edgeS, 2 * (d.m + d.p)));
     if (c->refine) CK(dalloc(c, &c->rres, L * ng));
     CK(dalloc(c, &c->T1, L * ng));
     CK(dalloc(c, &c->T2, L * ng));
