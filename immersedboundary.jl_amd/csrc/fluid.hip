@@ -436,6 +436,14 @@ __global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restri
     }
 }
 
+// the rank-4 boundary term at spectral entry (a, c), from one batch entry's W
+__device__ inline double edge_term(const Dom &d, const double *__restrict__ Sm,
+                                   const double *__restrict__ Sp, const double *__restrict__ Wb,
+                                   int a, int c) {
+    return Wb[a] * Sp[c] + Wb[d.m + a] * Sp[(long)(d.p - 1) * d.p + c] + Sm[a] * Wb[2 * d.m + c] +
+           Sm[(long)(d.m - 1) * d.m + a] * Wb[2 * d.m + d.p + c];
+}
+
 __global__ void k_edge_add(Dom d, const double *__restrict__ Sm, const double *__restrict__ Sp,
                            const double *__restrict__ W, long sW, const double *__restrict__ D,
                            double *__restrict__ F, long sF, const int *skip) {
@@ -444,18 +452,20 @@ __global__ void k_edge_add(Dom d, const double *__restrict__ Sm, const double *_
     if (t >= d.ng) return;
     const int b = blockIdx.y;
     const int a = (int)(t % d.m), c = (int)(t / d.m);
-    const double *Wb = W + b * sW;
-    const double e = Wb[a] * Sp[c] + Wb[d.m + a] * Sp[(long)(d.p - 1) * d.p + c] +
-                     Sm[a] * Wb[2 * d.m + c] + Sm[(long)(d.m - 1) * d.m + a] * Wb[2 * d.m + d.p + c];
-    F[b * sF + t] += D[t] * e;
+    F[b * sF + t] += D[t] * edge_term(d, Sm, Sp, W + b * sW, a, c);
+}
+
+void launch_edge_gemv(const Dom &d, const double *Sm, const double *Sp, const double *bc,
+                      long sbc, double *W, int nbatch, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_gemv, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp, bc, sbc,
+                       W, 2L * (d.m + d.p), skip);
 }
 
 void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, const double *bc,
                           long sbc, double *W, const double *D, double *F, long sF, int nbatch,
                           const int *skip, hipStream_t s) {
     const long sW = 2L * (d.m + d.p);
-    hipLaunchKernelGGL(k_edge_gemv, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp, bc, sbc,
-                       W, sW, skip);
+    launch_edge_gemv(d, Sm, Sp, bc, sbc, W, nbatch, skip, s);
     hipLaunchKernelGGL(k_edge_add, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, Sm, Sp, W,
                        sW, D, F, sF, skip);
 }
@@ -467,59 +477,65 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
 // the column index, their sum, the row sums w_I = S_m[I, :]·F (one wave per column), and the
 // line values S_m·v_J and w_I·S_p (one wave per output).  Work per batch entry:
 // 2·LN_CH·m + 2m + 2p doubles.
-constexpr int LN_CH = 16;
+constexpr int LN_CH = 64;  // = the wave width (k_lines_colsum)
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
 
-__global__ void k_lines_colpart(Dom d, const double *__restrict__ F, long sF,
-                                const double *__restrict__ Sp, double *__restrict__ W, long sW,
-                                const int *skip) {
-    SKIP_GUARD
-    const int a = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y, bt = blockIdx.z;
+// With EW (boundary-term GEMVs, 2(m+p) per batch entry) the spectrum read is F + D ⊙ edges.
+__device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F, long sF,
+                                     const double *__restrict__ Sm, const double *__restrict__ Sp,
+                                     const double *__restrict__ EW, const double *__restrict__ D,
+                                     double *__restrict__ W, long sW, int a, int ch, int bt) {
     if (a >= d.m) return;
     const double *Fb = F + bt * sF;
+    const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
     const int J1 = d.ny / 4 - 1, J2 = 3 * d.ny / 4 - 1;
     const int b0 = (int)((long)ch * d.p / LN_CH), b1 = (int)((long)(ch + 1) * d.p / LN_CH);
     double s1 = 0.0, s2 = 0.0;
     for (int b = b0; b < b1; ++b) {
-        const double f = Fb[a + (long)d.m * b];
+        const long t = a + (long)d.m * b;
+        double f = Fb[t];
+        if (Eb) f += D[t] * edge_term(d, Sm, Sp, Eb, a, b);
         s1 += f * Sp[(long)J1 * d.p + b];
         s2 += f * Sp[(long)J2 * d.p + b];
     }
     double *Wb = W + bt * sW;
-    Wb[(2 * ch) * (long)d.m + a] = s1;
-    Wb[(2 * ch + 1) * (long)d.m + a] = s2;
+    Wb[((long)a * LN_CH + ch) * 2] = s1;  // chunks of one entry contiguous: [a][ch][J]
+    Wb[((long)a * LN_CH + ch) * 2 + 1] = s2;
 }
 
+// one wave per entry a: lane = chunk (LN_CH = 64), a coalesced 1 KB read and a wave sum
 __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int *skip) {
     SKIP_GUARD
-    const int a = blockIdx.x * blockDim.x + threadIdx.x, bt = blockIdx.y;
+    const int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
+    const int lane = threadIdx.x & 63;
     if (a >= d.m) return;
     double *Wb = W + bt * sW;
-    double s1 = 0.0, s2 = 0.0;
-    for (int ch = 0; ch < LN_CH; ++ch) {
-        s1 += Wb[(2 * ch) * (long)d.m + a];
-        s2 += Wb[(2 * ch + 1) * (long)d.m + a];
+    double s1 = Wb[((long)a * LN_CH + lane) * 2], s2 = Wb[((long)a * LN_CH + lane) * 2 + 1];
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        Wb[2L * LN_CH * d.m + a] = s1;          // v_J1
+        Wb[(2L * LN_CH + 1) * d.m + a] = s2;    // v_J2
     }
-    Wb[2L * LN_CH * d.m + a] = s1;          // v_J1
-    Wb[(2L * LN_CH + 1) * d.m + a] = s2;    // v_J2
 }
 
-__global__ void k_lines_row(Dom d, const double *__restrict__ F, long sF,
-                            const double *__restrict__ Sm, double *__restrict__ W, long sW,
-                            const int *skip) {
-    SKIP_GUARD
-    const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
-    const int lane = threadIdx.x & 63;
+__device__ inline void lines_row(const Dom &d, const double *__restrict__ F, long sF,
+                                 const double *__restrict__ Sm, const double *__restrict__ Sp,
+                                 const double *__restrict__ EW, const double *__restrict__ D,
+                                 double *__restrict__ W, long sW, int b, int lane, int bt) {
     if (b >= d.p) return;
     const double *Fc = F + bt * sF + (long)d.m * b;
+    const double *Dc = D + (long)d.m * b;
+    const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
     const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
     double s1 = 0.0, s2 = 0.0;
     for (int a = lane; a < d.m; a += 64) {
-        const double f = Fc[a];
+        double f = Fc[a];
+        if (Eb) f += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
         s1 += Sm[(long)I1 * d.m + a] * f;
         s2 += Sm[(long)I2 * d.m + a] * f;
     }
@@ -529,6 +545,26 @@ __global__ void k_lines_row(Dom d, const double *__restrict__ F, long sF,
         double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
         Wr[b] = s1;              // w_I1
         Wr[d.p + b] = s2;        // w_I2
+    }
+}
+
+// the column chunks (blocks [0, nca·LN_CH)) and the row sums (the blocks after) in one launch
+__global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restrict__ F, long sF,
+                                                    const double *__restrict__ Sm,
+                                                    const double *__restrict__ Sp,
+                                                    const double *__restrict__ EW,
+                                                    const double *__restrict__ D,
+                                                    double *__restrict__ W, long sW, int nca,
+                                                    const int *skip) {
+    SKIP_GUARD
+    const int bt = blockIdx.y;
+    const int blk = blockIdx.x;
+    if (blk < nca * LN_CH) {
+        lines_colpart(d, F, sF, Sm, Sp, EW, D, W, sW, (blk % nca) * 256 + threadIdx.x, blk / nca,
+                      bt);
+    } else {
+        const int b = (blk - nca * LN_CH) * 4 + (threadIdx.x >> 6);
+        lines_row(d, F, sF, Sm, Sp, EW, D, W, sW, b, threadIdx.x & 63, bt);
     }
 }
 
@@ -578,30 +614,37 @@ long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d
 
 __global__ void k_spectral_scale(Dom d, const double *__restrict__ A, long sA,
                                  const double *__restrict__ D, double kappa,
-                                 double *__restrict__ F, long sF, const int *skip) {
+                                 const double *__restrict__ Sm, const double *__restrict__ Sp,
+                                 const double *__restrict__ EW, double *__restrict__ F, long sF,
+                                 const int *skip) {
     SKIP_GUARD
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    F[b * sF + t] = kappa * A[b * sA + t] * D[t];
+    double v = kappa * A[b * sA + t] * D[t];
+    if (EW) {
+        const int a = (int)(t % d.m), c = (int)(t / d.m);
+        v += D[t] * edge_term(d, Sm, Sp, EW + b * 2L * (d.m + d.p), a, c);
+    }
+    F[b * sF + t] = v;
 }
 
 void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
-                           double *F, long sF, int nbatch, const int *skip, hipStream_t s) {
+                           double *F, long sF, int nbatch, const int *skip, hipStream_t s,
+                           const double *Sm, const double *Sp, const double *EW) {
     hipLaunchKernelGGL(k_spectral_scale, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, A, sA,
-                       D, kappa, F, sF, skip);
+                       D, kappa, Sm, Sp, EW, F, sF, skip);
 }
 
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
-                  double *psi, long spsi, double *W, int nbatch, const int *skip,
-                  hipStream_t s) {
+                  double *psi, long spsi, double *W, int nbatch, const int *skip, hipStream_t s,
+                  const double *EW, const double *D) {
     const long sW = lines_work_size(d);
-    hipLaunchKernelGGL(k_lines_colpart, dim3(nblk(d.m, 256), LN_CH, nbatch), dim3(256), 0, s, d, F,
-                       sF, Sp, W, sW, skip);
-    hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 256), nbatch), dim3(256), 0, s, d, W, sW,
+    const int nca = (int)nblk(d.m, 256);
+    hipLaunchKernelGGL(k_lines_part, dim3(nca * LN_CH + nblk(d.p, 4), nbatch), dim3(256), 0, s, d,
+                       F, sF, Sm, Sp, EW, D, W, sW, nca, skip);
+    hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 4), nbatch), dim3(256), 0, s, d, W, sW,
                        skip);
-    hipLaunchKernelGGL(k_lines_row, dim3(nblk(d.p, 4), nbatch), dim3(256), 0, s, d, F, sF, Sm, W,
-                       sW, skip);
     hipLaunchKernelGGL(k_lines_out, dim3(nblk(d.m + d.p, 4), nbatch), dim3(256), 0, s, d, Sm, Sp,
                        W, sW, psi, spsi, skip);
 }

@@ -376,13 +376,15 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
     P.g[1].khi = (c->by_hi[lev] + 1) / 2;
     gemm(c, P.g[0], P.epi[0], prec, true, P.kind[0]);
     gemm(c, P.g[1], P.epi[1], prec, false, P.kind[1]);
+    if (lines_only) {  // the boundary term is added where the line passes read the spectrum
+        if (bc) launch_edge_gemv(c->d, c->Sm, c->Sp, bc, sbc, c->edgeW, nc, skip, c->s);
+        launch_lines(c->d, c->Sm, c->Sp, c->T2, c->d.ng, Psi, sPsi, c->lineW, nc, skip, c->s,
+                     bc ? c->edgeW : nullptr, c->sD);
+        return;
+    }
     if (bc)
         launch_edge_spectral(c->d, c->Sm, c->Sp, bc, sbc, c->edgeW, c->sD, c->T2, c->d.ng, nc,
                              skip, c->s);
-    if (lines_only) {
-        launch_lines(c->d, c->Sm, c->Sp, c->T2, c->d.ng, Psi, sPsi, c->lineW, nc, skip, c->s);
-        return;
-    }
     gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
     gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
 }
@@ -421,11 +423,9 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         if (lev == 0 && fuse0) {
             LapInvPlan P;
             lapinv_plan(c, nullptr, 0, Pl, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
+            if (bcl) launch_edge_gemv(d, c->Sm, c->Sp, bcl, sB, edgeW, nc, skip, c->s);
             launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
-                                  c->s);
-            if (bcl)
-                launch_edge_spectral(d, c->Sm, c->Sp, bcl, sB, edgeW, c->sD, c->T2, ng, nc, skip,
-                                     c->s);
+                                  c->s, c->Sm, c->Sp, bcl ? edgeW : nullptr);
             gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
             gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
             return;

@@ -109,15 +109,23 @@ void launch_fill(double *p, long n, double v, hipStream_t s);
 void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, const double *bc,
                           long sbc, double *W, const double *D, double *F, long sF, int nbatch,
                           const int *skip, hipStream_t s);
+// the GEMV half of it only: W = [S_m cB | S_m cT | S_p cL | S_p cR], for kernels that add the
+// rank-4 term where they read the spectrum
+void launch_edge_gemv(const Dom &d, const double *Sm, const double *Sp, const double *bc,
+                      long sbc, double *W, int nbatch, const int *skip, hipStream_t s);
 // ψ[b] = S_m F[b] S_p on the rows i = nx/4, 3nx/4 and the columns j = ny/4, 3ny/4 only (the
 // values get_bc! reads); other entries of ψ[b] are left as they were.  W: lines_work_size(d)
 // doubles per batch entry.
 long lines_work_size(const Dom &d);
-// F[b] = kappa·A[b] ⊙ D (D shared by the batch)
+// F[b] = kappa·A[b] ⊙ D (D shared by the batch), plus D ⊙ (rank-4 boundary term) with EW
 void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
-                           double *F, long sF, int nbatch, const int *skip, hipStream_t s);
+                           double *F, long sF, int nbatch, const int *skip, hipStream_t s,
+                           const double *Sm = nullptr, const double *Sp = nullptr,
+                           const double *EW = nullptr);
+// EW (optional, from launch_edge_gemv): read F + D ⊙ (rank-4 boundary term) instead of F.
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
-                  double *psi, long spsi, double *W, int nbatch, const int *skip, hipStream_t s);
+                  double *psi, long spsi, double *W, int nbatch, const int *skip, hipStream_t s,
+                  const double *EW = nullptr, const double *D = nullptr);
 // trial right-hand side of L levels: out = A_ℓ·g + f1·N1 + f2·N2 + edges(bcacc), A_ℓ as the
 // exact 5-point stencil I − a_ℓ(−L), a_ℓ = dt/(2 Re (h·2^ℓ)²)
 void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N2,
