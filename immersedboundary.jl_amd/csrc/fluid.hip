@@ -649,6 +649,16 @@ void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double
                        W, sW, psi, spsi, skip);
 }
 
+__global__ void k_fill_strided(double *p, long n, long stride, double v) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) p[blockIdx.y * stride + t] = v;
+}
+
+void launch_fill_strided(double *p, long n, long stride, int count, double v, hipStream_t s) {
+    if (n <= 0 || count <= 0) return;
+    hipLaunchKernelGGL(k_fill_strided, dim3(nblk(n, 256), count), dim3(256), 0, s, p, n, stride, v);
+}
+
 void launch_fill(double *p, long n, double v, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);

@@ -160,7 +160,12 @@ void launch_kind(const GemmArgs &g, int prec, int kind, int v, int epi, bool tou
 }
 
 int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
-    const TuneKey key{prec, kind, g0.M, g0.N, g0.K, g0.batch, tout};
+    // batch counts share a choice within power-of-two buckets (3-4, 5-8, ..., 33+): the column
+    // batches of the preconditioner's rebuilds (e.g. 79, 79, 42) then reuse what the first
+    // build tuned instead of timing every variant inside a step; all variants give the same bits
+    const int bb = g0.batch <= 2 ? g0.batch : g0.batch <= 4 ? 4 : g0.batch <= 8 ? 8
+                 : g0.batch <= 16 ? 16 : g0.batch <= 32 ? 32 : 64;
+    const TuneKey key{prec, kind, g0.M, g0.N, g0.K, bb, tout};
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) return it->second;
     if (c->tune_fixed >= 0) return c->tuned[key] = c->tune_fixed;
@@ -451,8 +456,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
     double *bcL = bc + (long)(L - 1) * d.nbc;
-    launch_fill(bcL, d.nbc, 0.0, c->s);  // single column needs zero; batched handled below
-    for (int col = 1; col < nc; ++col) launch_fill(bcL + col * sB, d.nbc, 0.0, c->s);
+    launch_fill_strided(bcL, d.nbc, sB, nc, 0.0, c->s);
     solve(L - 1, nullptr);
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);

@@ -103,6 +103,8 @@ void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, l
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
                       hipStream_t s);
 void launch_fill(double *p, long n, double v, hipStream_t s);
+// p[b·stride + i] = v for i < n, b < count
+void launch_fill_strided(double *p, long n, long stride, int count, double v, hipStream_t s);
 // F[b] += D ⊙ (S_m E_b S_p) for the edge field E_b of the bc array bc[b] (apply_bc! with fac 1):
 // the rank-4 sine transform of a Poisson right-hand side's boundary terms.  W: work of
 // 2(m + p) doubles per batch entry; D: the m×p scale table (shared by the batch).
