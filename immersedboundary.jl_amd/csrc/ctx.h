@@ -111,6 +111,7 @@ struct ibh_ctx {
     int pc_rebuilds = 0;
     double pc_time_total = 0, pc_time_cols = 0, pc_time_inv = 0;  // seconds, all rebuilds
     double *Bfull = nullptr, *Bwork = nullptr, *Pc = nullptr;
+    void *rb = nullptr;  // rocBLAS handle of the dense inverses (created on first use)
     int *dJ = nullptr, *dinJ = nullptr, *gjws = nullptr;
     std::vector<int> J;                      // unknowns of moving bodies
     std::vector<std::pair<int, int>> Jrange; // contiguous [start, count) runs of J

@@ -595,12 +595,21 @@ void dfree(ibh_ctx *c, void *p);
 // Inv = inverse(A) through LAPACK's getrf + getri as rocSOLVER implements them (the routines
 // behind inv(Bmat), cnab/ops.jl:96): blocked, MFMA-backed, for systems where Gauss–Jordan's
 // one HBM pass per pivot is too slow.  A is destroyed.
+rocblas_handle rb_handle(ibh_ctx *c) {
+    if (!c->rb) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+        c->rb = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(c->rb);
+    rocblas_set_stream(h, c->s);
+    return h;
+}
+
 int invert_lapack(ibh_ctx *c, double *A, double *Inv) {
     const int nf = 2 * c->nb;
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success)
-        return fail(c, IBH_EDEVICE, "rocblas_create_handle failed");
-    rocblas_set_stream(h, c->s);
+    rocblas_handle h = rb_handle(c);
+    if (!h) return fail(c, IBH_EDEVICE, "rocblas_create_handle failed");
     rocblas_int *piv = nullptr, *info = nullptr;
     int rc = dalloc(c, &piv, (size_t)nf);
     if (!rc) rc = dalloc(c, &info, 2);
@@ -614,12 +623,46 @@ int invert_lapack(ibh_ctx *c, double *A, double *Inv) {
             hipStreamSynchronize(c->s) != hipSuccess)
             rc = fail(c, IBH_EDEVICE, "rocSOLVER inverse: device error");
     }
-    rocblas_destroy_handle(h);
     dfree(c, piv);
     dfree(c, info);
     if (rc) return rc;
     if (st != rocblas_status_success) return fail(c, IBH_EDEVICE, "rocSOLVER getrf/getri failed");
     if (hinfo[0] || hinfo[1]) return fail(c, IBH_ESINGULAR, "singular B matrix");
+    return IBH_OK;
+}
+
+// Inv = lower(A)⁻¹ of a symmetric positive definite A by Cholesky (rocSOLVER potrf + potri on
+// the lower triangle, then mirrored), for the PCG preconditioner; falls back to LU when the
+// factorization finds A not positive definite.  A is destroyed.
+__global__ void k_mirror_lower(int n, const double *__restrict__ A, double *__restrict__ Inv) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)n * n) return;
+    const int i = (int)(t % n), j = (int)(t / n);
+    Inv[t] = i >= j ? A[t] : A[j + (long)n * i];
+}
+
+int invert_spd(ibh_ctx *c, double *A, double *Inv) {
+    const int nf = 2 * c->nb;
+    rocblas_handle h = rb_handle(c);
+    if (!h) return fail(c, IBH_EDEVICE, "rocblas_create_handle failed");
+    // keep A for the LU fallback
+    launch_copy(Inv, A, (long)nf * nf, c->s);
+    rocblas_int *info = nullptr;
+    int rc = dalloc(c, &info, 2);
+    if (rc) return rc;
+    rocblas_status st = rocsolver_dpotrf(h, rocblas_fill_lower, nf, Inv, nf, info);
+    if (st == rocblas_status_success) st = rocsolver_dpotri(h, rocblas_fill_lower, nf, Inv, nf, info + 1);
+    int hinfo[2] = {0, 0};
+    if (hipMemcpyAsync(hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+        hipStreamSynchronize(c->s) != hipSuccess)
+        rc = fail(c, IBH_EDEVICE, "rocSOLVER Cholesky inverse: device error");
+    dfree(c, info);
+    if (rc) return rc;
+    if (st != rocblas_status_success || hinfo[0] || hinfo[1]) return invert_lapack(c, A, Inv);
+    // Inv holds the inverse in its lower triangle: mirror it into A, then back
+    hipLaunchKernelGGL(k_mirror_lower, dim3((unsigned)(((long)nf * nf + 255) / 256)), dim3(256), 0,
+                       c->s, nf, Inv, A);
+    launch_copy(Inv, A, (long)nf * nf, c->s);
     return IBH_OK;
 }
 
@@ -724,7 +767,11 @@ int refresh_pc(ibh_ctx *c) {
     launch_copy(c->Bwork, c->Bfull, (long)nf * nf, c->s);
     HIPCK(hipStreamSynchronize(c->s));
     const double t1 = now_s();
-    rc = invert(c, c->Bwork, c->Pc);
+    // P only steers the iteration: the Cholesky inverse of the (symmetric) assembled B
+    // (IBH_PC_INV=spd|lu forces a route)
+    bool spd = nf > 768;
+    if (const char *pi = std::getenv("IBH_PC_INV")) spd = std::strcmp(pi, "spd") == 0;
+    rc = spd ? invert_spd(c, c->Bwork, c->Pc) : invert(c, c->Bwork, c->Pc);
     if (rc) return rc;
     HIPCK(hipStreamSynchronize(c->s));
     const double t2 = now_s();
@@ -1237,6 +1284,7 @@ void ibh_destroy(ibh_ctx *c) {
     if (c->tune_buf) hipFree(c->tune_buf);
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
     if (c->prof.ts) hipFree(c->prof.ts);
+    if (c->rb) rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb));
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }

@@ -218,6 +218,16 @@ def test_step_parity_moving_plate_pcg(ibhip):
     assert_state_close(state.case, orc, 1e-8)
 
 
+def test_step_parity_moving_plate_pcg_cholesky(ibhip, monkeypatch):
+    # the preconditioner inverted by Cholesky (the route for more than 768 unknowns)
+    monkeypatch.setenv("IBH_PC_INV", "spd")
+    prob = cylinder_plate_case(n=64, nlev=3, cg_reltol=1e-15, cyl_spacing=1.0,
+                               force_solver="pcg", pc_refresh=2)
+    state, orc, hist = run_pair(ibhip, prob, 4)
+    assert state.case.cg_iters() < orc.cg_iters()
+    assert_state_close(state.case, orc, 1e-8)
+
+
 def test_step_parity_moving_plate_pcg_truncated_columns(ibhip):
     # three levels: the preconditioner's columns come from a two-level Vort2Flux (and FP32
     # transforms), the iteration itself applies the full FP64 B
