@@ -111,6 +111,10 @@ void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int
         S.n = (int)plans.size();
         for (const LapInvPlan &P : plans) {
             const GemmArgs &g = P.g[k];
+            if (!sine_ready(g, prec)) {
+                std::fprintf(stderr, "libibhip: grouped contraction without its sine parts\n");
+                std::abort();
+            }
             S.gargs.push_back(g);
             const double rows = S.kind == 1 ? g.N : g.M;
             S.flops += 4.0 * g.h * (double)g.h * rows * g.batch;

@@ -20,6 +20,9 @@
 // (the D layouts were measured on gfx950 with an in-bounds probe).
 #include <hip/hip_bf16.h>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "ibh_internal.h"
 
 namespace ibh {
@@ -31,6 +34,9 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 // ------------------------------------------------------------------------ precision policies
 struct PF64 {
     typedef double E;    // LDS element
+    typedef double SL;   // sine part as stored in global memory
+    __device__ static const SL *so(const GemmArgs &g) { return g.So; }
+    __device__ static const SL *se(const GemmArgs &g) { return g.Se; }
     typedef d4 Acc;
     static constexpr int KS = 4;      // K per MFMA
     // LDS row padding (elements): an odd row stride in doubles; measured with rocprofv3 on the
@@ -41,6 +47,9 @@ struct PF64 {
 };
 struct PF32 {
     typedef float E;
+    typedef float SL;
+    __device__ static const SL *so(const GemmArgs &g) { return g.Sof; }
+    __device__ static const SL *se(const GemmArgs &g) { return g.Sef; }
     typedef f4 Acc;
     static constexpr int KS = 4;
     static constexpr int PAD = 4;
@@ -49,6 +58,9 @@ struct PF32 {
 };
 struct PBF3 {
     typedef __bf16 E;    // hi and lo planes
+    typedef unsigned SL; // hi | lo << 16
+    __device__ static const SL *so(const GemmArgs &g) { return g.Sob; }
+    __device__ static const SL *se(const GemmArgs &g) { return g.Seb; }
     typedef f4 Acc;
     static constexpr int KS = 32;
     static constexpr int PAD = 8;
@@ -94,12 +106,35 @@ template <class P, int ROWS, int BK>
 __device__ inline void tput(Tile<P, ROWS, BK> &t, int r, int k, double v) {
     t.s[r][k] = (typename P::E)v;
 }
+// bf16x3 operand split: f = fl32(v), hi = bf16(f), lo = bf16(f − hi), both round-to-nearest-even
+__device__ inline void bf3_split(double v, __bf16 &hi, __bf16 &lo) {
+    const float f = (float)v;
+    hi = __float2bfloat16(f);
+    lo = __float2bfloat16(f - __bfloat162float(hi));
+}
 template <int ROWS, int BK>
 __device__ inline void tput(Tile<PBF3, ROWS, BK> &t, int r, int k, double v) {
-    const float f = (float)v;
-    const __bf16 h = __float2bfloat16(f);
-    t.hi[r][k] = h;
-    t.lo[r][k] = __float2bfloat16(f - __bfloat162float(h));
+    bf3_split(v, t.hi[r][k], t.lo[r][k]);
+}
+// sine elements arrive already rounded (launch_sine_low)
+template <class P, int ROWS, int BK>
+__device__ inline void tputs(Tile<P, ROWS, BK> &t, int r, int k, typename P::SL v) {
+    t.s[r][k] = v;
+}
+template <int ROWS, int BK>
+__device__ inline void tputs(Tile<PBF3, ROWS, BK> &t, int r, int k, unsigned v) {
+    t.hi[r][k] = __builtin_bit_cast(__bf16, (unsigned short)(v & 0xffffu));
+    t.lo[r][k] = __builtin_bit_cast(__bf16, (unsigned short)(v >> 16));
+}
+
+__global__ void k_sine_low(const double *S, float *f, unsigned *b, long n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    __bf16 hi, lo;
+    bf3_split(S[t], hi, lo);
+    f[t] = (float)S[t];
+    b[t] = (unsigned)__builtin_bit_cast(unsigned short, hi) |
+           ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
 template <class P, int ROWS, int BK>
@@ -334,7 +369,8 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
 #pragma unroll
         for (int b = 0; b < FM; ++b) accO[a][b] = accE[a][b] = P::zero();
 
-    double rso[NREG][LS], rse[NREG][LS], rd[NREG][LD];
+    typename P::SL rso[NREG][LS], rse[NREG][LS];
+    double rd[NREG][LD];
     // predicated loads: out-of-range elements are zero.  (Loading them unconditionally from a
     // clamped index was measured slower on every variant, 4-39%.)
     auto gload = [&](int k0, int st) {
@@ -342,8 +378,8 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
         for (int u = 0; u < LS; ++u) {
             const int e = tid + NT * u, r = r0s + e / BK, k = k0 + e % BK;
             const bool in = r < h && k < h;
-            rso[st][u] = in ? g.So[k + (long)h * r] : 0.0;
-            rse[st][u] = in ? g.Se[k + (long)h * r] : 0.0;
+            rso[st][u] = in ? P::so(g)[k + (long)h * r] : typename P::SL(0);
+            rse[st][u] = in ? P::se(g)[k + (long)h * r] : typename P::SL(0);
         }
 #pragma unroll
         for (int u = 0; u < LD; ++u) {
@@ -356,11 +392,11 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
         for (int u = 0; u < LS; ++u) {
             const int e = tid + NT * u;
             if (SIDE == 0) {
-                tput(AsO[buf], e / BK, e % BK, rso[st][u]);
-                tput(AsE[buf], e / BK, e % BK, rse[st][u]);
+                tputs(AsO[buf], e / BK, e % BK, rso[st][u]);
+                tputs(AsE[buf], e / BK, e % BK, rse[st][u]);
             } else {
-                tput(BsO[buf], e / BK, e % BK, rso[st][u]);
-                tput(BsE[buf], e / BK, e % BK, rse[st][u]);
+                tputs(BsO[buf], e / BK, e % BK, rso[st][u]);
+                tputs(BsE[buf], e / BK, e % BK, rse[st][u]);
             }
         }
 #pragma unroll
@@ -707,7 +743,15 @@ const char *gemm_eo_variant_name(int prec, int v) {
     return (v >= 0 && v < n) ? t[v].name : "?";
 }
 
+void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_sine_low, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, f, b, n);
+}
+
 void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi, hipStream_t s) {
+    if (!sine_ready(g, prec)) {
+        std::fprintf(stderr, "libibhip: split contraction without its precision's sine parts\n");
+        std::abort();
+    }
     int n;
     const VariantEO *t = table_eo(prec, &n);
     const VariantEO &V = t[(variant >= 0 && variant < n) ? variant : 0];

@@ -252,6 +252,16 @@ GemmArgs gargs() {
 // LaplacianInv apply (fluid-ops.jl:42-56) on `nbatch` fields: out = S((S·in·S)⊙scale)S where
 // the 16·s factor of two RODFT00 passes lives in the scale table.  The last contraction's
 // epilogue is `epi` (EPI_STORE, EPI_SUBFROM with D = out, or EPI_RHS with the extras in rhs).
+// sine parts of the x (m) or y (p) dimension in every operand precision
+static void set_sine(GemmArgs &g, const ibh_ctx *c, bool y) {
+    g.So = y ? c->Sop : c->Som;
+    g.Se = y ? c->Sep : c->Sem;
+    g.Sof = y ? c->Sopf : c->Somf;
+    g.Sef = y ? c->Sepf : c->Semf;
+    g.Sob = y ? c->Sopb : c->Somb;
+    g.Seb = y ? c->Sepb : c->Semb;
+}
+
 // The four contractions are TN GEMMs (both operands k-fastest): the left ones read S_m
 // through its symmetry and write their result transposed, so the right ones read it
 // k-fastest too.  Through the odd/even split each costs half the MFMA work of the full one.
@@ -265,12 +275,12 @@ void ibh::lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long 
     g.batch = nbatch;
     g.skip = skip;
     g.M = m; g.N = p; g.K = m; g.h = d.nx / 2;
-    g.So = c->Som; g.Se = c->Sem;
+    set_sine(g, c, false);
     g.B = in; g.ldb = m; g.sB = sin;
     g.C = c->T1; g.ldc = p; g.sC = ng;
     P->g[0] = g; P->epi[0] = EPI_STORE; P->kind[0] = 1;
     g.K = p; g.h = d.ny / 2;
-    g.So = c->Sop; g.Se = c->Sep;
+    set_sine(g, c, true);
     g.B = nullptr;
     g.A = c->T1; g.lda = p; g.sA = ng;
     g.C = c->T2; g.ldc = m; g.sC = ng;
@@ -278,12 +288,12 @@ void ibh::lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long 
     P->g[1] = g; P->epi[1] = EPI_SCALE; P->kind[1] = 2;
     g.D = nullptr; g.A = nullptr;
     g.K = m; g.h = d.nx / 2;
-    g.So = c->Som; g.Se = c->Sem;
+    set_sine(g, c, false);
     g.B = c->T2; g.ldb = m; g.sB = ng;
     g.C = c->T1; g.ldc = p; g.sC = ng;
     P->g[2] = g; P->epi[2] = EPI_STORE; P->kind[2] = 1;
     g.K = p; g.h = d.ny / 2;
-    g.So = c->Sop; g.Se = c->Sep;
+    set_sine(g, c, true);
     g.B = nullptr;
     g.A = c->T1; g.lda = p; g.sA = ng;
     g.C = out; g.ldc = m; g.sC = sout;
@@ -1178,11 +1188,23 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         const size_t hx = D.nx / 2, hy = D.ny / 2;
         CK(dalloc(c, &c->Som, hx * hx));
         CK(dalloc(c, &c->Sem, hx * hx));
+        CK(dalloc(c, &c->Somf, hx * hx));
+        CK(dalloc(c, &c->Semf, hx * hx));
+        CK(dalloc(c, &c->Somb, hx * hx));
+        CK(dalloc(c, &c->Semb, hx * hx));
         c->Sop = c->Som;
         c->Sep = c->Sem;
+        c->Sopf = c->Somf;
+        c->Sepf = c->Semf;
+        c->Sopb = c->Somb;
+        c->Sepb = c->Semb;
         if (D.ny != D.nx) {
             CK(dalloc(c, &c->Sop, hy * hy));
             CK(dalloc(c, &c->Sep, hy * hy));
+            CK(dalloc(c, &c->Sopf, hy * hy));
+            CK(dalloc(c, &c->Sepf, hy * hy));
+            CK(dalloc(c, &c->Sopb, hy * hy));
+            CK(dalloc(c, &c->Sepb, hy * hy));
         }
         if (const char *e = std::getenv("IBH_DST_SPLIT")) c->use_eo = std::atoi(e) != 0;
     }
@@ -1252,6 +1274,15 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     if (c->Sp != c->Sm) launch_sine_matrix(c->Sp, D.ny, c->s);
     launch_sine_split(c->Som, c->Sem, D.nx, c->s);
     if (c->Sop != c->Som) launch_sine_split(c->Sop, c->Sep, D.ny, c->s);
+    {
+        const long hx = D.nx / 2, hy = D.ny / 2;
+        launch_sine_low(c->Som, c->Somf, c->Somb, hx * hx, c->s);
+        launch_sine_low(c->Sem, c->Semf, c->Semb, hx * hx, c->s);
+        if (c->Sop != c->Som) {
+            launch_sine_low(c->Sop, c->Sopf, c->Sopb, hy * hy, c->s);
+            launch_sine_low(c->Sep, c->Sepf, c->Sepb, hy * hy, c->s);
+        }
+    }
     launch_scale_tables(d, c->sD, c->sA, c->sAinv, c->L, c->h, c->dt, c->Re, c->s);
     std::vector<double> fac(L + 1, 0.0);
     for (int l = 0; l < L; ++l) {

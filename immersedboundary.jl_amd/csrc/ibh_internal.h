@@ -50,6 +50,10 @@ struct GemmArgs {
     const int *skip;  // if non-null and *skip != 0 the launch does nothing
     // even/odd split: h = n/2 pairs, So/Se h×h sine parts stored k-fastest ([k + h·a])
     const double *So, *Se;
+    // the same parts rounded for the low-precision contractions: FP32 (Sof, Sef) and bf16
+    // pairs (Sob, Seb: the rounded value in the low 16 bits, the rounded remainder in the high)
+    const float *Sof, *Sef;
+    const unsigned *Sob, *Seb;
     int h;
     int batch;
     // even/odd split: only pair indices [klo, khi) of the data's contiguous dimension are
@@ -141,6 +145,12 @@ void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+
 // So[a][k] = sin(π(a+1)(2k+1)/n), Se[a][k] = sin(2π(a+1)(k+1)/n) (0 on the last row/column),
 // both h×h (h = n/2) stored [k + h·a]
 void launch_sine_split(double *So, double *Se, int n, hipStream_t s);
+// FP32 and bf16-pair copies of n sine elements, rounded as the contraction rounds FP64 operands
+void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t s);
+// a split contraction in precision prec (0 FP64, 1 FP32, 2 bf16x3) has its sine parts
+inline bool sine_ready(const GemmArgs &g, int prec) {
+    return prec == 0 ? g.So && g.Se : prec == 1 ? g.Sof && g.Sef : g.Sob && g.Seb;
+}
 void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
                          double dt, double Re, hipStream_t s);
 
