@@ -789,10 +789,14 @@ static const VariantGEO GF64[] = {
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),
     GEOV(PF32, 64, 32, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 3),
+    GEOVS(PF32, 32, 32, 16, 2, 2, 1), GEOVS(PF32, 32, 64, 16, 2, 2, 1), GEOVS(PF32, 64, 64, 16, 2, 2, 1),
+    GEOVS(PF32, 32, 32, 32, 2, 2, 4), GEOVS(PF32, 32, 64, 32, 2, 2, 3),
 };
 static const VariantGEO GBF3[] = {
     GEOV(PBF3, 64, 64, 32, 2, 2, 1), GEOV(PBF3, 128, 64, 32, 2, 2, 1), GEOV(PBF3, 64, 128, 32, 2, 2, 1),
     GEOV(PBF3, 64, 64, 32, 2, 2, 2), GEOV(PBF3, 64, 64, 32, 2, 2, 3),
+    GEOVS(PBF3, 32, 32, 32, 2, 2, 1), GEOVS(PBF3, 32, 64, 32, 2, 2, 1), GEOVS(PBF3, 64, 64, 32, 2, 2, 1),
+    GEOVS(PBF3, 32, 32, 32, 2, 2, 3), GEOVS(PBF3, 32, 64, 64, 2, 2, 1),
 };
 
 static const VariantGEO *table_geo(int prec, int *n) {

@@ -139,15 +139,16 @@ def test_batch_graph_replay_matches_eager(monkeypatch):
             assert rel(a, b) < 1e-12
 
 
-def test_batch_grouped_variants_identical(monkeypatch):
-    # every grouped tile variant (forced for all stages) sums k in the same order, so the
-    # tuner's choice cannot change a batch's numbers
+@pytest.mark.parametrize("precision,nvar", [("fp64", 15), ("fp32", 10), ("bf16x3", 10)])
+def test_batch_grouped_variants_identical(monkeypatch, precision, nvar):
+    # every grouped tile variant of a precision (forced for all stages) sums k in the same
+    # order, so the tuner's choice cannot change a batch's numbers
     import ibhip as m
 
     runs = []
-    for v in range(15):  # every FP64 grouped variant
+    for v in range(nvar):
         monkeypatch.setenv("IBH_GRP_VARIANT", str(v))
-        probs = mixed_problems("fp64")[:4]
+        probs = mixed_problems(precision)[:4]
         states = [m.initstate(p) for p in probs]
         for p, st in zip(probs, states):
             m.initsolver(p, st)
