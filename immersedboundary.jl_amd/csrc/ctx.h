@@ -92,6 +92,11 @@ struct ibh_ctx {
     std::vector<int> bx_lo, bx_hi, by_lo, by_hi;
     bool sparse_B = true;   // skip those zero tiles in the B-products' forward transforms
     bool lines_B = true;    // ... and form coarse-level ψ only on the lines get_bc! reads
+    // Level 1 of a B-product needs Γ and ψ only within reach of the body points' deltas: an
+    // inclusive box of interior indices [bxa, bxb] × [bya, byb] with slack for the motion.
+    // Its forward transforms sum only the box's pairs and its Poisson solve forms ψ only on it.
+    bool box_B = true, box_on = false;
+    int bxa = 0, bxb = 0, bya = 0, byb = 0;
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
     int *d_boff = nullptr, *d_npts = nullptr;

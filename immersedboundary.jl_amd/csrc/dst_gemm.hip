@@ -20,6 +20,7 @@
 // (the D layouts were measured on gfx950 with an in-bounds probe).
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -587,6 +588,8 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo(GemmArgs g) {
     ts_mark(g.ts, false, lin_block());
     int bxt, byt, bz;
     xcd_tile(bxt, byt, bz);
+    bxt += g.olo[0] / BM;
+    byt += g.olo[1] / BN;
     eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, bxt, byt, bz);
     ts_mark(g.ts, true, lin_block());
 }
@@ -756,8 +759,16 @@ void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi,
     const VariantEO *t = table_eo(prec, &n);
     const VariantEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
     const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
-    dim3 grid((gm + V.bm - 1) / V.bm, (gn + V.bn - 1) / V.bn, g.batch);
-    hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], grid, dim3(V.nthreads), 0, s, g);
+    const int ex = g.ohi[0] > 0 ? std::min(g.ohi[0], gm) : gm;
+    const int ey = g.ohi[1] > 0 ? std::min(g.ohi[1], gn) : gn;
+    const int tx = (ex + V.bm - 1) / V.bm - (g.ohi[0] > 0 ? g.olo[0] / V.bm : 0);
+    const int ty = (ey + V.bn - 1) / V.bn - (g.ohi[1] > 0 ? g.olo[1] / V.bn : 0);
+    if (tx <= 0 || ty <= 0) return;
+    GemmArgs ga = g;
+    if (g.ohi[0] <= 0) ga.olo[0] = 0;
+    if (g.ohi[1] <= 0) ga.olo[1] = 0;
+    dim3 grid(tx, ty, g.batch);
+    hipLaunchKernelGGL(V.fn[side][side == 0 ? 0 : epi], grid, dim3(V.nthreads), 0, s, ga);
 }
 
 // grouped split variants: [side][epi]

@@ -218,9 +218,13 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
 // split one (two h×k parts; k = h pairs, or the khi - klo pairs a narrowed launch sums)
 double gemm_flops(const GemmArgs &g, int kind) {
     if (kind == 0) return 2.0 * g.M * (double)g.N * g.K * g.batch;
-    const double rows = kind == 1 ? g.N : g.M;
+    const int ext[2] = {kind == 1 ? g.h : g.M, kind == 1 ? g.N : g.h};
+    double e[2];
+    for (int a = 0; a < 2; ++a)
+        e[a] = g.ohi[a] > 0 ? (double)(std::min(g.ohi[a], ext[a]) - g.olo[a]) : (double)ext[a];
+    const double pairs = kind == 1 ? e[0] : e[1], rows = kind == 1 ? e[1] : e[0];
     const double kp = g.khi > 0 ? (double)(g.khi - g.klo) : (double)g.h;
-    return 4.0 * g.h * kp * rows * g.batch;
+    return 4.0 * pairs * kp * rows * g.batch;
 }
 
 void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
@@ -404,6 +408,15 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
     gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
 }
 
+// Pairs a'' of a split transform of size n whose output rows (a'' and n − 2 − a'') meet the
+// rows [lo, hi]: [*plo, *phi).  min(r, n − 2 − r) rises to the middle row h − 1 and falls after.
+void pair_range(int lo, int hi, int n, int *plo, int *phi) {
+    const int h = n / 2;
+    auto f = [n](int r) { return std::min(r, n - 2 - r); };
+    *plo = std::min(f(lo), f(hi));
+    *phi = (lo <= h - 1 && hi >= h - 1) ? h : std::max(f(lo), f(hi)) + 1;
+}
+
 // `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
 // Options that change only the order of roundoff:
@@ -441,6 +454,14 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
             if (bcl) launch_edge_gemv(d, c->Sm, c->Sp, bcl, sB, edgeW, nc, skip, c->s);
             launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
                                   c->s, c->Sm, c->Sp, bcl ? edgeW : nullptr);
+            if (sparse && c->box_on) {
+                // E reads level 1 only inside the body box: the x transform forms the pairs
+                // whose rows meet it, the y transform its rows and the pairs of its columns
+                pair_range(c->bxa, c->bxb, d.nx, &P.g[2].olo[0], &P.g[2].ohi[0]);
+                P.g[3].olo[0] = c->bxa;
+                P.g[3].ohi[0] = c->bxb + 1;
+                pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
+            }
             gemm(c, P.g[2], P.epi[2], prec, true, P.kind[2]);
             gemm(c, P.g[3], P.epi[3], prec, false, P.kind[3]);
             return;
@@ -505,6 +526,12 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
         lapinv_plan(c, c->g1B, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, skip, &P);
         P.g[1].C = c->specB;
         P.g[2].B = c->specB;
+        if (c->box_on) {  // rot Eᵀz vanishes outside the body box
+            P.g[0].klo = c->bxa / 2;
+            P.g[0].khi = c->bxb / 2 + 1;
+            P.g[1].klo = c->bya / 2;
+            P.g[1].khi = c->byb / 2 + 1;
+        }
         if (prec < 0) prec = c->precision;
         for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], prec, P.kind[k] == 1, P.kind[k]);
     } else {
@@ -794,8 +821,53 @@ int refresh_pc(ibh_ctx *c) {
     return IBH_OK;
 }
 
+void drop_graphs(ibh_ctx *c);
+
+// Body box of the B-products (host positions r, x then y): the interior indices of the points'
+// cells ± BOX_REACH (the 13-point delta stencil of E and Eᵀ, 6 cells, plus the neighbour of
+// curl and rot and the node offset) and BOX_SLACK cells more for the motion.  It is recomputed
+// when a point comes within BOX_REACH of its edge; the captured graphs carry it in their
+// launches, so they are dropped then (after the stream drains).
+constexpr int BOX_REACH = 10, BOX_SLACK = 16;
+int update_box(ibh_ctx *c, const double *r) {
+    const int nb = c->nb;
+    bool ok = c->box_B && c->use_eo && c->sparse_B && nb > 0;
+    int xa = 0, xb = 0, ya = 0, yb = 0;
+    for (int k = 0; ok && k < nb; ++k) {
+        const double fx = std::floor((r[k] - c->x0) / c->h), fy = std::floor((r[nb + k] - c->y0) / c->h);
+        if (!(std::fabs(fx) < 1e9 && std::fabs(fy) < 1e9)) {
+            ok = false;  // non-finite or far outside: the regularization reports it
+            break;
+        }
+        const int ix = (int)fx - 1, iy = (int)fy - 1;
+        if (k == 0) { xa = xb = ix; ya = yb = iy; }
+        xa = std::min(xa, ix); xb = std::max(xb, ix);
+        ya = std::min(ya, iy); yb = std::max(yb, iy);
+    }
+    const int m = c->d.m, p = c->d.p;
+    if (ok && c->box_on && xa - BOX_REACH >= c->bxa && xb + BOX_REACH <= c->bxb &&
+        ya - BOX_REACH >= c->bya && yb + BOX_REACH <= c->byb)
+        return IBH_OK;
+    if (!ok && !c->box_on) return IBH_OK;
+    HIPCK(hipStreamSynchronize(c->s));
+    drop_graphs(c);
+    c->box_on = ok;
+    if (ok) {
+        c->bxa = std::max(0, xa - BOX_REACH - BOX_SLACK);
+        c->bxb = std::min(m - 1, xb + BOX_REACH + BOX_SLACK);
+        c->bya = std::max(0, ya - BOX_REACH - BOX_SLACK);
+        c->byb = std::min(p - 1, yb + BOX_REACH + BOX_SLACK);
+        if (c->bxa > c->bxb || c->bya > c->byb) c->box_on = false;
+    }
+    return IBH_OK;
+}
+
 int upload_points(ibh_ctx *c, const double *r, const double *v) {
     if (c->nb == 0) return IBH_OK;
+    if (r) {
+        int rc = update_box(c, r);
+        if (rc) return rc;
+    }
     if (r) HIPCK(hipMemcpyAsync(c->pts, r, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
     if (v) HIPCK(hipMemcpyAsync(c->vel, v, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
     return IBH_OK;
@@ -1157,6 +1229,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             coarse(D.ny, c->by_lo[l - 1], c->by_hi[l - 1], &c->by_lo[l], &c->by_hi[l]);
         }
         if (const char *sb = std::getenv("IBH_SPARSE_B")) c->sparse_B = std::atoi(sb) != 0;
+        if (const char *bb = std::getenv("IBH_BODY_BOX")) c->box_B = std::atoi(bb) != 0;
         if (const char *sl = std::getenv("IBH_SPARSE_LINES")) c->lines_B = std::atoi(sl) != 0;
     }
     if (const char *rl = std::getenv("IBH_REFINE_LEVELS"))
@@ -1291,6 +1364,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     }
     hipMemcpyAsync(c->fac, fac.data(), sizeof(double) * (L + 1), hipMemcpyHostToDevice, c->s);
     if (nb) {
+        CK(update_box(c, D.xb));
         hipMemcpyAsync(c->pts, D.xb, sizeof(double) * nf, hipMemcpyHostToDevice, c->s);
         hipMemcpyAsync(c->ds, D.ds, sizeof(double) * nb, hipMemcpyHostToDevice, c->s);
     }
@@ -1410,6 +1484,10 @@ int ibh_set_field(ibh_ctx *c, int32_t which, const double *in) {
     if (n < 0 || which == IBH_F_Q0 || which == IBH_F_CD || which == IBH_F_CL)
         return fail(c, IBH_EINVAL, "field cannot be set");
     if (n == 0) return IBH_OK;
+    if (which == IBH_F_POINTS_R) {
+        int rc = update_box(c, in);
+        if (rc) return rc;
+    }
     HIPCK(hipMemcpyAsync(field_ptr(c, which), in, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
     HIPCK(hipStreamSynchronize(c->s));
     return IBH_OK;

@@ -59,6 +59,11 @@ struct GemmArgs {
     // even/odd split: only pair indices [klo, khi) of the data's contiguous dimension are
     // summed (the data is zero elsewhere); khi = 0 sums all of them
     int klo, khi;
+    // split launches of one case: only the tiles that cover elements [olo[0], ohi[0]) of the
+    // grid's x extent (pairs for SIDE 0, data rows for SIDE 1) and [olo[1], ohi[1]) of its y
+    // extent (data rows, pairs) run; ohi = 0 covers the whole extent.  Outputs outside the
+    // launched tiles keep their previous contents.
+    int olo[2], ohi[2];
     // profiling only: a PROF_REC record of device wall-clock ticks, zeroed beforehand: the
     // start of workgroup 0 (the first dispatched) at ts[0], each workgroup's end folded by
     // atomicMax into one of 16 sub-slots (ts[16 + 16·j], 128 B apart, so that the end

@@ -151,6 +151,36 @@ def test_b_product_four_levels_reproducible(ibhip, monkeypatch):
         assert np.array_equal(run(v), ref)
 
 
+def test_b_product_body_box(ibhip, monkeypatch):
+    # a B-product transforms level 1 only within reach of the body points: the tiles it skips
+    # hold exact zeros or outputs that E does not read, so the product is bitwise the one
+    # without the box, also after the points move far enough to move the box
+    prob = cylinder_case(n=128, nlev=3)
+    nb = prob.bodies.npoints
+    r = prob.stacked_points.T.reshape(-1).copy()
+    moved = r.copy()
+    moved[:nb] += 0.6  # 19 cells in x
+    moved[nb:] -= 0.25
+    z = np.random.default_rng(4).standard_normal(2 * nb)
+
+    def run(box):
+        monkeypatch.setenv("IBH_BODY_BOX", "1" if box else "0")
+        c = ibhip.DeviceCase(prob)
+        out = []
+        for pts in (r, moved):
+            c.reg_update(pts)
+            out.append(c.B_times(z))
+        c.close()
+        return out
+
+    with_box, without = run(True), run(False)
+    for a, b in zip(with_box, without):
+        assert np.array_equal(a, b)
+    orc = ibref.OracleRun(prob, setup=False)
+    orc.reg_update(moved)
+    assert rel(with_box[1], orc.B_times(z)) < 1e-11
+
+
 def run_pair(ibhip, prob, nsteps, t0=0.0):
     import ibhip as m
 
