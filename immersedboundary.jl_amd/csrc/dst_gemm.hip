@@ -341,6 +341,8 @@ __device__ __forceinline__ void eo_tile(const GemmArgs &g, int bxt, int byt, int
     constexpr int RD = SIDE == 0 ? BN : BM;  // rows of the data tile
     constexpr int LS = RS * BK / NT, LD = RD * 2 * BK / NT;
     static_assert(LS * NT == RS * BK && LD * NT == RD * 2 * BK, "load split");
+    // callers that form only the rows a narrowed second contraction reads align them to 128
+    static_assert(BK <= 64 && (BK & (BK - 1)) == 0, "K tile of at most 64 pairs, a power of 2");
 
     const int h = g.h;                       // pairs
     const int Kd = g.K;                      // data length along k (= n - 1)

@@ -379,6 +379,16 @@ void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const dou
     }
 }
 
+// A forward pair whose second contraction sums only the pairs [klo, khi) of the first one's
+// output rows: the first contraction forms only the rows those K tiles read (128-aligned, so
+// every tile variant, BK ≤ 64 pairs, reads rows inside the range); other rows of its output keep
+// stale values that nothing reads.
+void rows_for_pairs(GemmArgs &g0, const GemmArgs &g1) {
+    if (g1.khi <= 0) return;
+    g0.olo[1] = std::max(0, (2 * g1.klo / 128) * 128);
+    g0.ohi[1] = std::min(g0.N, ((2 * g1.khi + 127) / 128) * 128);
+}
+
 // Poisson solve ψ = Δinv(G + edges(bc)) on level `lev` of a B-product, where G is zero
 // outside the rows/columns [bx_lo, bx_hi) × [by_lo, by_hi) of that level: the two forward
 // contractions sum only those K tiles, and the boundary terms enter in the sine basis as a
@@ -393,6 +403,7 @@ void lapinv_sparse(ibh_ctx *c, const double *G, long sG, double *Psi, long sPsi,
     P.g[0].khi = (c->bx_hi[lev] + 1) / 2;
     P.g[1].klo = c->by_lo[lev] / 2;
     P.g[1].khi = (c->by_hi[lev] + 1) / 2;
+    rows_for_pairs(P.g[0], P.g[1]);
     gemm(c, P.g[0], P.epi[0], prec, true, P.kind[0]);
     gemm(c, P.g[1], P.epi[1], prec, false, P.kind[1]);
     if (lines_only) {  // the boundary term is added where the line passes read the spectrum
@@ -531,6 +542,7 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
             P.g[0].khi = c->bxb / 2 + 1;
             P.g[1].klo = c->bya / 2;
             P.g[1].khi = c->byb / 2 + 1;
+            rows_for_pairs(P.g[0], P.g[1]);
         }
         if (prec < 0) prec = c->precision;
         for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], prec, P.kind[k] == 1, P.kind[k]);
@@ -976,6 +988,13 @@ void seg_post(ibh_ctx *c) {
                         &P);
             P.g[1].C = c->specP;
             P.g[2].B = c->specP;
+            if (c->box_on) {  // rot Eᵀ Fb vanishes outside the body box
+                P.g[0].klo = c->bxa / 2;
+                P.g[0].khi = c->bxb / 2 + 1;
+                P.g[1].klo = c->bya / 2;
+                P.g[1].khi = c->byb / 2 + 1;
+                rows_for_pairs(P.g[0], P.g[1]);
+            }
             for (int k = 0; k < 4; ++k)
                 gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
             launch_axpy_sub((int)ng, c->specT, c->specP, 1.0, c->s);
