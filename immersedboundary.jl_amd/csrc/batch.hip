@@ -317,7 +317,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             if (c->nb == 0) continue;
             const int nf = 2 * c->nb;
             e.push_back({c->d, c->reg, c->qs, c->Ftmp, b->uv, c->h});
-            be.push_back(nblk(nf, GRP_BS));
+            be.push_back(nblk(nf, GRP_BS / 64));  // one wave per output
             g.push_back({nf, c->Binv, c->Ftmp, c->cg.ws, c->Fb});
             bp.push_back(nblk(nf, GRP_BS) * ((nf + GV_CHUNK - 1) / GV_CHUNK));
             bs.push_back(nblk(nf, GRP_BS));

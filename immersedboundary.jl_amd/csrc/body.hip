@@ -75,29 +75,27 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
                        pts, h, x0, y0);
 }
 
-// E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w, summed column-major like the
-// reference's sumprod; q0 (constant per component) is added to q first, as in Q = qs + q0.
-__device__ inline double E_at(const Dom &d, const RegDev &r, const double *__restrict__ q,
-                              double u0, double v0, int t) {
+// E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w over the 13×13 stencil of point
+// k, with q0 (constant per component) added to q first, as in Q = qs + q0.  One wave per output:
+// lane l sums the entries l, l + 64, l + 128 (column-major, s fastest), then a fixed-order wave
+// sum; the total is returned in lane 0.  (The reference's sumprod adds the 169 terms in one
+// column-major sweep; the two orders differ by roundoff.)
+__device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__restrict__ q,
+                                double u0, double v0, int t, int lane) {
     const int comp = t / r.nb, k = t % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
     const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
     const double *wy = wx + NS;
+    const double *base = comp == 0 ? q - 1 : q + d.nu - 1;
+    const long ld = comp == 0 ? d.nx + 1 : d.nx;
+    const double q0 = comp == 0 ? u0 : v0;
     double acc = 0.0;
-    if (comp == 0) {
-        for (int tt = 0; tt < NS; ++tt) {
-            const int j = by + tt - SUPP;
-            const double *col = q + (long)(d.nx + 1) * (j - 1) - 1;
-            for (int s = 0; s < NS; ++s) acc += (col[bx + s - SUPP] + u0) * (wx[s] * wy[tt]);
-        }
-    } else {
-        for (int tt = 0; tt < NS; ++tt) {
-            const int j = by + tt - SUPP;
-            const double *col = q + d.nu + (long)d.nx * (j - 1) - 1;
-            for (int s = 0; s < NS; ++s) acc += (col[bx + s - SUPP] + v0) * (wx[s] * wy[tt]);
-        }
+    for (int e = lane; e < NS * NS; e += 64) {
+        const int tt = e / NS, s = e % NS;
+        const int j = by + tt - SUPP;
+        acc += (base[ld * (j - 1) + bx + s - SUPP] + q0) * (wx[s] * wy[tt]);
     }
-    return acc;
+    return wave_sum(acc);
 }
 
 __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
@@ -108,17 +106,18 @@ __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, doub
         u0 = uv[0] * uvscale;
         v0 = uv[1] * uvscale;
     }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= 2 * r.nb) return;
     const int b = blockIdx.y;
-    fb[b * sfb + t] = E_at(d, r, q + b * sq, u0, v0, t);
+    const double v = E_wave(d, r, q + b * sq, u0, v0, t, lane);
+    if (lane == 0) fb[b * sfb + t] = v;
 }
 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s, const double *uv,
               double uvscale) {
     if (r.nb == 0) return;
-    hipLaunchKernelGGL(k_E, dim3(nblk(2L * r.nb, 128), nbatch), dim3(128), 0, s, d, r, q, sq, u0,
+    hipLaunchKernelGGL(k_E, dim3(nblk(2L * r.nb, 4), nbatch), dim3(256), 0, s, d, r, q, sq, u0,
                        v0, fb, sfb, skip, uv, uvscale);
 }
 
@@ -533,9 +532,11 @@ static_assert(GRP_NS == NS && GV_CHUNK == GV_CH, "grouped constants");
 __global__ __launch_bounds__(GRP_BS) void kg_E(const EA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const EA A = args[p];
-    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const int t = (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (t >= 2 * A.r.nb) return;
-    A.fb[t] = E_at(A.d, A.r, A.q, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t);
+    const double v = E_wave(A.d, A.r, A.q, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t, lane);
+    if (lane == 0) A.fb[t] = v;
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_ET_zero(const ETA *args, const int *off, int n) {

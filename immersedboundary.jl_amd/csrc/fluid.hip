@@ -479,11 +479,6 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
 // 2·LN_CH·m + 2m + 2p doubles.
 constexpr int LN_CH = 64;  // = the wave width (k_lines_colsum)
 
-__device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
 // With EW (boundary-term GEMVs, 2(m+p) per batch entry) the spectrum read is F + D ⊙ edges.
 __device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F, long sF,
                                      const double *__restrict__ Sm, const double *__restrict__ Sp,

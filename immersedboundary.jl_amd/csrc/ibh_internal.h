@@ -258,6 +258,12 @@ __host__ __device__ inline long grp_elems_coarsify(const Dom &d) {
     return (long)(d.nx / 2 - 1) * (d.ny / 2 - 1);
 }
 static constexpr int GRP_BS = 256;
+
+// sum over the 64 lanes of a wave, in a fixed tree order; the total lands in lane 0
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
 static constexpr int GRP_NS = 13;  // delta support per dimension
 static constexpr int GV_CHUNK = 64; // GEMV column chunk
 
