@@ -320,7 +320,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             be.push_back(nblk(nf, GRP_BS / 64));  // one wave per output
             g.push_back({nf, c->Binv, c->Ftmp, c->cg.ws, c->Fb});
             bp.push_back(nblk(nf, GRP_BS) * ((nf + GV_CHUNK - 1) / GV_CHUNK));
-            bs.push_back(nblk(nf, GRP_BS));
+            bs.push_back(nblk(nf, GRP_BS / 64));  // one wave per row
         }
         add(v, OP_E, e, be);
         add(v, OP_GEMVP, g, bp);

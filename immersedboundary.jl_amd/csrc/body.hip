@@ -211,24 +211,31 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
 
 // y = A·x for a dense n×n column-major matrix (the Binv GEMV of cnab/ops.jl:97 and the PCG
 // preconditioner).  Phase 1: block (row block, column chunk of GV_CH) writes partial sums,
-// one thread per row so every column read is coalesced; phase 2 sums the chunks in a fixed
-// order (deterministic).  ws holds ceil(n/GV_CH)·n doubles.
-static constexpr int GV_CH = 64;
+// one thread per row so every column read is coalesced; phase 2: one wave per row sums that
+// row's chunks (stored contiguously, [row][chunk]) lane-strided and then in a fixed tree order
+// (deterministic).  Short chunks give phase 1 enough waves to keep the matrix stream in flight.
+// ws holds ceil(n/GV_CH)·n doubles.
+static constexpr int GV_CH = 16;
 
 __device__ inline void gemv_part_at(int n, const double *__restrict__ A,
                                     const double *__restrict__ x, double *__restrict__ ws, int i,
                                     int chunk) {
     const int j0 = chunk * GV_CH, j1 = min(n, j0 + GV_CH);
+    const int nch = (n + GV_CH - 1) / GV_CH;
     double s = 0.0;
     for (int j = j0; j < j1; ++j) s += A[i + (long)n * j] * x[j];
-    ws[(long)chunk * n + i] = s;
+    ws[(long)i * nch + chunk] = s;
 }
 
-__device__ inline void gemv_sum_at(int n, int nch, const double *__restrict__ ws,
-                                   double *__restrict__ y, int i) {
+// lane `lane` of the wave that owns row i; lane 0 writes y[i]
+__device__ inline void gemv_sum_at(int n, const double *__restrict__ ws, double *__restrict__ y,
+                                   int i, int lane) {
+    const int nch = (n + GV_CH - 1) / GV_CH;
+    const double *w = ws + (long)i * nch;
     double s = 0.0;
-    for (int c = 0; c < nch; ++c) s += ws[(long)c * n + i];
-    y[i] = s;
+    for (int c = lane; c < nch; c += 64) s += w[c];
+    s = wave_sum(s);
+    if (lane == 0) y[i] = s;
 }
 
 __global__ void k_gemv_part(int n, const double *__restrict__ A, const double *__restrict__ x,
@@ -239,12 +246,12 @@ __global__ void k_gemv_part(int n, const double *__restrict__ A, const double *_
     gemv_part_at(n, A, x, ws, i, blockIdx.y);
 }
 
-__global__ void k_gemv_sum(int n, int nch, const double *__restrict__ ws, double *__restrict__ y,
+__global__ void k_gemv_sum(int n, const double *__restrict__ ws, double *__restrict__ y,
                            const int *skip) {
     if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (i >= n) return;
-    gemv_sum_at(n, nch, ws, y, i);
+    gemv_sum_at(n, ws, y, i, threadIdx.x & 63);
 }
 
 long gemv_ws_size(int n) { return (long)((n + GV_CH - 1) / GV_CH) * n; }
@@ -254,7 +261,7 @@ void launch_gemv(int n, const double *A, const double *x, double *y, double *ws,
     if (n == 0) return;
     const int nch = (n + GV_CH - 1) / GV_CH;
     hipLaunchKernelGGL(k_gemv_part, dim3(nblk(n, 256), nch), dim3(256), 0, s, n, A, x, ws, skip);
-    hipLaunchKernelGGL(k_gemv_sum, dim3(nblk(n, 256)), dim3(256), 0, s, n, nch, ws, y, skip);
+    hipLaunchKernelGGL(k_gemv_sum, dim3(nblk(n, 4)), dim3(256), 0, s, n, ws, y, skip);
 }
 
 __global__ void k_axpy_sub(int n, double *y, const double *v, double a) {
@@ -580,9 +587,9 @@ __global__ __launch_bounds__(GRP_BS) void kg_gemv_part(const GemvA *args, const 
 __global__ __launch_bounds__(GRP_BS) void kg_gemv_sum(const GemvA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const GemvA A = args[p];
-    const int i = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const int i = (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6);
     if (i >= A.n) return;
-    gemv_sum_at(A.n, (A.n + GV_CH - 1) / GV_CH, A.ws, A.y, i);
+    gemv_sum_at(A.n, A.ws, A.y, i, threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const int *off, int n) {

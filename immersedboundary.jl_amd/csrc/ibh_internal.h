@@ -265,7 +265,7 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 static constexpr int GRP_NS = 13;  // delta support per dimension
-static constexpr int GV_CHUNK = 64; // GEMV column chunk
+static constexpr int GV_CHUNK = 16; // GEMV column chunk
 
 void launch_grp_curl(const Grp &g, hipStream_t s);
 void launch_grp_rot(const Grp &g, hipStream_t s);
