@@ -1,6 +1,8 @@
 // Grid stencils of the streamfunction–vorticity discretization (fluid-ops.jl), one thread per
 // output value, x-fastest so consecutive lanes touch consecutive addresses.  Every kernel is
 // batched over levels (or over independent right-hand sides) with element strides.
+#include <algorithm>
+
 #include "ibh_internal.h"
 
 namespace ibh {
@@ -56,6 +58,39 @@ void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pb
                     double *q, long sq, int nbatch, const int *skip, hipStream_t s) {
     hipLaunchKernelGGL(k_curl_bc, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, d, psi, spsi,
                        pbc, sbc, q, sq, skip);
+}
+
+// The same q entries on the faces with 1-based indices i ∈ [i0, i1], j ∈ [j0, j1] only (u and
+// v each clipped to their own extent); the rest of q is left as it was.  For products whose
+// reader (E) stays inside that box.
+__global__ void k_curl_box(Dom d, const double *__restrict__ psi, long spsi,
+                           const double *__restrict__ pbc, long sbc, double *__restrict__ q,
+                           long sq, const int *skip, int i0, int i1, int j0, int j1) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    const int wi = i1 - i0 + 1, wj = j1 - j0 + 1;
+    const long nb = (long)wi * wj;
+    if (t >= 2 * nb) return;
+    const bool uq = t < nb;
+    const long e = uq ? t : t - nb;
+    const int i = i0 + (int)(e % wi), j = j0 + (int)(e / wi);
+    if (uq ? (i > d.nx + 1 || j > d.ny) : (i > d.nx || j > d.ny + 1)) return;
+    const long idx = uq ? (i - 1) + (long)(d.nx + 1) * (j - 1) : d.nu + (i - 1) + (long)d.nx * (j - 1);
+    q[b * sq + idx] = curl_at(d, psi + b * spsi, pbc + b * sbc, idx);
+}
+
+void launch_curl_box(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
+                     double *q, long sq, int nbatch, const int *skip, hipStream_t s, int i0,
+                     int i1, int j0, int j1) {
+    i0 = std::max(i0, 1);
+    j0 = std::max(j0, 1);
+    i1 = std::min(i1, d.nx + 1);
+    j1 = std::min(j1, d.ny + 1);
+    if (i1 < i0 || j1 < j0) return;
+    const long n = 2L * (i1 - i0 + 1) * (j1 - j0 + 1);
+    hipLaunchKernelGGL(k_curl_box, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, psi, spsi, pbc,
+                       sbc, q, sq, skip, i0, i1, j0, j1);
 }
 
 // rot! (C'), fluid-ops.jl:117-129

@@ -502,18 +502,30 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     solve(L - 1, nullptr);
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
-    if (curl_all || L == 1)
-        launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0), sQ,
-                       nc, skip, c->s);
+    // E reads level 1 only inside the body box in a B-product (whose Poisson solve formed ψ there)
+    const bool qbox = !curl_all && sparse && c->box_on && c->use_eo && refine == 0;
+    if (curl_all || L == 1) {
+        if (qbox && L == 1)
+            launch_curl_box(d, Psi, sG, bcL, sB, Q, sQ, nc, skip, c->s, c->bxa + 1, c->bxb + 2,
+                            c->bya + 1, c->byb + 2);
+        else
+            launch_curl_bc(d, Psi + (L - 1) * ng, sG, bcL, sB, Q + (curl_all ? (L - 1) * nq : 0),
+                           sQ, nc, skip, c->s);
+    }
     for (int lev = L - 2; lev >= 0; --lev) {
         double *bcl = bc + (long)lev * d.nbc;
         launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
         solve(lev, bcl);
         if (refine && lev < c->refine_levels)
             refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine, prec);
-        if (curl_all || lev == 0)
-            launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ, nc,
-                           skip, c->s);
+        if (curl_all || lev == 0) {
+            if (qbox && lev == 0)
+                launch_curl_box(d, Psi, sG, bcl, sB, Q, sQ, nc, skip, c->s, c->bxa + 1,
+                                c->bxb + 2, c->bya + 1, c->byb + 2);
+            else
+                launch_curl_bc(d, Psi + lev * ng, sG, bcl, sB, Q + (curl_all ? lev * nq : 0), sQ,
+                               nc, skip, c->s);
+        }
     }
 }
 
@@ -528,8 +540,8 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     const long ng = d.ng;
     launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
     launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
-    for (int col = 0; col < nc; ++col)
-        if (Lv > 1) launch_fill(c->gB + col * L * ng + ng, (Lv - 1) * ng, 0.0, c->s);
+    // Γ's coarse levels (zero here) need no clearing: their buffer is zeroed when allocated and
+    // only ever receives coarsify!'s fixed centre blocks, which every product overwrites
     // Ainv₁ with its scaled spectrum kept (specB) for the level-1 Poisson solve
     const bool fuse = c->use_eo && c->sparse_B;
     if (fuse) {

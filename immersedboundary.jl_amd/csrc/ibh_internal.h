@@ -88,6 +88,10 @@ const char *gemm_eo_variant_name(int prec, int v);
 // All take a batch count `nb` over levels (or columns) with element strides.
 void launch_curl_bc(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
                     double *q, long sq, int nbatch, const int *skip, hipStream_t s);
+// curl on the faces with 1-based indices [i0, i1] × [j0, j1] only (clipped to the grid)
+void launch_curl_box(const Dom &d, const double *psi, long spsi, const double *pbc, long sbc,
+                     double *q, long sq, int nbatch, const int *skip, hipStream_t s, int i0,
+                     int i1, int j0, int j1);
 void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
                 const int *skip, hipStream_t s);
 void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
