@@ -509,10 +509,10 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
 // 3ny/4 (1-based), the values get_bc! reads (fluid-ops.jl:210-239).  Coarse levels of a
 // B-product need nothing else, so this replaces the two inverse contractions there.  Four
 // deterministic passes (no atomics): column partial sums v_J = F·S_p[:, J] over LN_CH chunks of
-// the column index, their sum, the row sums w_I = S_m[I, :]·F (one wave per column), and the
+// the column index, their sum, the row sums w_I = S_m[I, :]·F (one workgroup per column), and the
 // line values S_m·v_J and w_I·S_p (one wave per output).  Work per batch entry:
 // 2·LN_CH·m + 2m + 2p doubles.
-constexpr int LN_CH = 64;  // = the wave width (k_lines_colsum)
+constexpr int LN_CH = 128;  // two per lane in k_lines_colsum
 
 // With EW (boundary-term GEMVs, 2(m+p) per batch entry) the spectrum read is F + D ⊙ edges.
 __device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F, long sF,
@@ -537,14 +537,17 @@ __device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F,
     Wb[((long)a * LN_CH + ch) * 2 + 1] = s2;
 }
 
-// one wave per entry a: lane = chunk (LN_CH = 64), a coalesced 1 KB read and a wave sum
+// one wave per entry a: lane l adds chunks l and l + 64 (coalesced reads), then a wave sum
 __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int *skip) {
     SKIP_GUARD
+    static_assert(LN_CH == 128, "two chunks per lane");
     const int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
     const int lane = threadIdx.x & 63;
     if (a >= d.m) return;
     double *Wb = W + bt * sW;
-    double s1 = Wb[((long)a * LN_CH + lane) * 2], s2 = Wb[((long)a * LN_CH + lane) * 2 + 1];
+    const double *w = Wb + (long)a * LN_CH * 2;
+    double s1 = w[lane * 2] + w[(lane + 64) * 2];
+    double s2 = w[lane * 2 + 1] + w[(lane + 64) * 2 + 1];
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     if (lane == 0) {
@@ -553,17 +556,21 @@ __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int
     }
 }
 
+// one 256-thread workgroup per column b: wave w sums a quarter of the rows lane-strided, the
+// four wave sums are added in order (deterministic)
 __device__ inline void lines_row(const Dom &d, const double *__restrict__ F, long sF,
                                  const double *__restrict__ Sm, const double *__restrict__ Sp,
                                  const double *__restrict__ EW, const double *__restrict__ D,
-                                 double *__restrict__ W, long sW, int b, int lane, int bt) {
-    if (b >= d.p) return;
+                                 double *__restrict__ W, long sW, int b, int bt) {
+    __shared__ double sh[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *Fc = F + bt * sF + (long)d.m * b;
     const double *Dc = D + (long)d.m * b;
     const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
     const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
+    const int a0 = (int)((long)w * d.m / 4), a1 = (int)((long)(w + 1) * d.m / 4);
     double s1 = 0.0, s2 = 0.0;
-    for (int a = lane; a < d.m; a += 64) {
+    for (int a = a0 + lane; a < a1; a += 64) {
         double f = Fc[a];
         if (Eb) f += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
         s1 += Sm[(long)I1 * d.m + a] * f;
@@ -572,9 +579,14 @@ __device__ inline void lines_row(const Dom &d, const double *__restrict__ F, lon
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     if (lane == 0) {
+        sh[0][w] = s1;
+        sh[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
-        Wr[b] = s1;              // w_I1
-        Wr[d.p + b] = s2;        // w_I2
+        Wr[b] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];        // w_I1
+        Wr[d.p + b] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];  // w_I2
     }
 }
 
@@ -593,8 +605,7 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
         lines_colpart(d, F, sF, Sm, Sp, EW, D, W, sW, (blk % nca) * 256 + threadIdx.x, blk / nca,
                       bt);
     } else {
-        const int b = (blk - nca * LN_CH) * 4 + (threadIdx.x >> 6);
-        lines_row(d, F, sF, Sm, Sp, EW, D, W, sW, b, threadIdx.x & 63, bt);
+        lines_row(d, F, sF, Sm, Sp, EW, D, W, sW, blk - nca * LN_CH, bt);
     }
 }
 
@@ -671,7 +682,7 @@ void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double
                   const double *EW, const double *D) {
     const long sW = lines_work_size(d);
     const int nca = (int)nblk(d.m, 256);
-    hipLaunchKernelGGL(k_lines_part, dim3(nca * LN_CH + nblk(d.p, 4), nbatch), dim3(256), 0, s, d,
+    hipLaunchKernelGGL(k_lines_part, dim3(nca * LN_CH + d.p, nbatch), dim3(256), 0, s, d,
                        F, sF, Sm, Sp, EW, D, W, sW, nca, skip);
     hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 4), nbatch), dim3(256), 0, s, d, W, sW,
                        skip);
