@@ -510,7 +510,7 @@ void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, cons
 // B-product need nothing else, so this replaces the two inverse contractions there.  Four
 // deterministic passes (no atomics): column partial sums v_J = F·S_p[:, J] over LN_CH chunks of
 // the column index, their sum, the row sums w_I = S_m[I, :]·F (one workgroup per column), and the
-// line values S_m·v_J and w_I·S_p (one wave per output).  Work per batch entry:
+// line values S_m·v_J and w_I·S_p (one workgroup per output).  Work per batch entry:
 // 2·LN_CH·m + 2m + 2p doubles.
 constexpr int LN_CH = 128;  // two per lane in k_lines_colsum
 
@@ -609,44 +609,59 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
     }
 }
 
-__global__ void k_lines_out(Dom d, const double *__restrict__ Sm, const double *__restrict__ Sp,
-                            const double *__restrict__ W, long sW, double *__restrict__ psi,
-                            long spsi, const int *skip) {
+// one 256-thread workgroup per output entry: wave w sums a quarter of the index range
+// lane-strided, the four wave sums are added in order (deterministic)
+__global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restrict__ Sm,
+                                                   const double *__restrict__ Sp,
+                                                   const double *__restrict__ W, long sW,
+                                                   double *__restrict__ psi, long spsi,
+                                                   const int *skip) {
     SKIP_GUARD
-    const int o = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    if (o >= d.m + d.p) return;
+    __shared__ double sh[2][4];
+    const int o = blockIdx.x, bt = blockIdx.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *Wb = W + bt * sW;
     double *ps = psi + bt * spsi;
+    const bool col = o < d.m;
+    const int n = col ? d.m : d.p;
+    const int k0 = (int)((long)w * n / 4), k1 = (int)((long)(w + 1) * n / 4);
     double s1 = 0.0, s2 = 0.0;
-    if (o < d.m) {  // ψ(o, J) = Σ_a S_m[o, a] v_J[a]
+    if (col) {  // ψ(o, J) = Σ_a S_m[o, a] v_J[a]
         const double *v1 = Wb + 2L * LN_CH * d.m, *v2 = v1 + d.m;
-        for (int a = lane; a < d.m; a += 64) {
+        for (int a = k0 + lane; a < k1; a += 64) {
             const double sv = Sm[(long)o * d.m + a];
             s1 += sv * v1[a];
             s2 += sv * v2[a];
         }
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
-        if (lane == 0) {
-            ps[o + (long)d.m * (d.ny / 4 - 1)] = s1;
-            ps[o + (long)d.m * (3 * d.ny / 4 - 1)] = s2;
-        }
-    } else {        // ψ(I, c) = Σ_b w_I[b] S_p[b, c]
+    } else {    // ψ(I, c) = Σ_b w_I[b] S_p[b, c]
         const int c = o - d.m;
         const double *w1 = Wb + (2L * LN_CH + 2) * d.m, *w2 = w1 + d.p;
-        for (int b = lane; b < d.p; b += 64) {
+        for (int b = k0 + lane; b < k1; b += 64) {
             const double sv = Sp[(long)c * d.p + b];
             s1 += w1[b] * sv;
             s2 += w2[b] * sv;
         }
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        sh[0][w] = s1;
+        sh[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double t1 = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+    const double t2 = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    if (col) {
+        ps[o + (long)d.m * (d.ny / 4 - 1)] = t1;
+        ps[o + (long)d.m * (3 * d.ny / 4 - 1)] = t2;
+    } else {
         // the four crossings with the columns belong to the column part (one writer each,
-        // so the result does not depend on which wave finishes last)
-        if (lane == 0 && c != d.ny / 4 - 1 && c != 3 * d.ny / 4 - 1) {
-            ps[(d.nx / 4 - 1) + (long)d.m * c] = s1;
-            ps[(3 * d.nx / 4 - 1) + (long)d.m * c] = s2;
+        // so the result does not depend on which workgroup finishes last)
+        const int c = o - d.m;
+        if (c != d.ny / 4 - 1 && c != 3 * d.ny / 4 - 1) {
+            ps[(d.nx / 4 - 1) + (long)d.m * c] = t1;
+            ps[(3 * d.nx / 4 - 1) + (long)d.m * c] = t2;
         }
     }
 }
@@ -686,7 +701,7 @@ void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double
                        F, sF, Sm, Sp, EW, D, W, sW, nca, skip);
     hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 4), nbatch), dim3(256), 0, s, d, W, sW,
                        skip);
-    hipLaunchKernelGGL(k_lines_out, dim3(nblk(d.m + d.p, 4), nbatch), dim3(256), 0, s, d, Sm, Sp,
+    hipLaunchKernelGGL(k_lines_out, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp,
                        W, sW, psi, spsi, skip);
 }
 
