@@ -197,6 +197,30 @@ __global__ void k_et_fix(double *__restrict__ q, long n, long sq, const double *
     et_fix_at(q + blockIdx.y * sq, scale[blockIdx.y], t);
 }
 
+// zeroing of the fixed-point image (blocks before the last) and the column's scale (the last)
+__global__ __launch_bounds__(256) void k_et_zero_scale(double *__restrict__ q, long n, long sq,
+                                                       const double *__restrict__ fb, long sfb,
+                                                       int n2, double *scale, const int *skip) {
+    if (skip && *skip) return;
+    const int b = blockIdx.y;
+    if (blockIdx.x == gridDim.x - 1) {
+        et_scale_body(fb + b * sfb, n2, scale + b);
+        return;
+    }
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) q[b * sq + t] = 0.0;
+}
+
+void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
+                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(d.nq, 256) + 1, nbatch), dim3(256), 0, s, img,
+                       d.nq, sq, fb, sfb, 2 * r.nb, scale, skip);
+    if (r.nb == 0) return;
+    long n = 2L * r.nb * NS * NS;
+    hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, img, sq,
+                       (const double *)scale, skip);
+}
+
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, double *scale, hipStream_t s) {
     hipLaunchKernelGGL(k_zero, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq, skip);

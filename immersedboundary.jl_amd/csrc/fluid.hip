@@ -114,6 +114,29 @@ void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int 
                        skip);
 }
 
+// the same from the fixed-point image of q that Eᵀ leaves: each entry converted exactly as the
+// conversion pass of Eᵀ does ((double)v · (1/scale)), then the same differences
+__global__ void k_rot_image(Dom d, const long long *__restrict__ img, long sq,
+                            const double *__restrict__ scale, double *__restrict__ g, long sg,
+                            const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const int b = blockIdx.y;
+    const long long *q = img + b * sq;
+    const double inv = 1.0 / scale[b];
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    const double ux0 = (double)QX(q, i, j - 1) * inv, ux1 = (double)QX(q, i, j) * inv;
+    const double vy1 = (double)QY(q, i, j) * inv, vy0 = (double)QY(q, i - 1, j) * inv;
+    g[b * sg + t] = (ux0 - ux1) + (vy1 - vy0);
+}
+
+void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
+                      long sg, int nbatch, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_rot_image, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d,
+                       reinterpret_cast<const long long *>(img), sq, scale, g, sg, skip);
+}
+
 // coarsify!(Γc, Γ), fluid-ops.jl:184-202: fine (i, j) = (2a + 2, 2b + 2) → coarse
 // (nx/4 + 1 + a, ny/4 + 1 + b), a in 0..nx/2-3, b in 0..ny/2-3.
 __device__ inline void coarsify_at(const Dom &d, double *__restrict__ gc,

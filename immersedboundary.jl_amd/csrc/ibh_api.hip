@@ -538,8 +538,8 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     const Dom &d = c->d;
     const int L = c->L, Lv = nlev > 0 ? std::min(nlev, L) : L;
     const long ng = d.ng;
-    launch_ET(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
-    launch_rot(d, c->qB, d.nq, c->g1B, ng, nc, skip, c->s);
+    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
+    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
     // Γ's coarse levels (zero here) need no clearing: their buffer is zeroed when allocated and
     // only ever receives coarsify!'s fixed centre blocks, which every product overwrites
     // Ainv₁ with its scaled spectrum kept (specB) for the level-1 Poisson solve
@@ -991,8 +991,8 @@ void seg_post(ibh_ctx *c) {
     post_swap(c);
     const bool fuse = c->use_eo && c->refine == 0;  // Γ₁'s spectrum known (seg_trial)
     if (nf) {
-        launch_ET(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s);
-        launch_rot(d, c->qtmp, d.nq, c->g1, ng, 1, nullptr, c->s);
+        launch_ET_image(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s);
+        launch_rot_image(d, c->qtmp, d.nq, c->etscale, c->g1, ng, 1, nullptr, c->s);
         if (fuse) {
             // Γ₁ = S(F_trial − F_proj)S: keep the projection's spectrum and subtract it
             LapInvPlan P;

@@ -94,6 +94,9 @@ void launch_curl_box(const Dom &d, const double *psi, long spsi, const double *p
                      int i1, int j0, int j1);
 void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
                 const int *skip, hipStream_t s);
+// rot of the q whose fixed-point image (scale per batch entry) launch_ET_image left in img
+void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
+                      long sg, int nbatch, const int *skip, hipStream_t s);
 void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
                      const int *skip, hipStream_t s);
 void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
@@ -179,6 +182,10 @@ void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0
 // q = Eᵀ fb (zero, scale, fixed-point scatter, conversion); scale: nbatch doubles of work
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, double *scale, hipStream_t s);
+// Eᵀ up to its fixed-point image (zeroing and the scales in one launch, then the scatter);
+// launch_rot_image reads that image directly, converting as launch_ET's last pass would
+void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
+                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s);
 void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
                  hipStream_t s);
 long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
