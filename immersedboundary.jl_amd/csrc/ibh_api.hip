@@ -498,7 +498,9 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         launch_coarsify(d, G + lev * ng, G + (lev - 1) * ng, sG, sG, nc, skip, c->s);
     // ψ_L = Δinv Γ_L with ψbc = 0
     double *bcL = bc + (long)(L - 1) * d.nbc;
-    launch_fill_strided(bcL, d.nbc, sB, nc, 0.0, c->s);
+    // a B-product over every level finds its coarsest boundary values zero already: their
+    // buffer is zeroed when allocated and get_bc! writes only the finer levels' values
+    if (!(sparse && L == Ls)) launch_fill_strided(bcL, d.nbc, sB, nc, 0.0, c->s);
     solve(L - 1, nullptr);
     if (refine && L - 1 < c->refine_levels)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine, prec);
