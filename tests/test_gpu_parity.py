@@ -239,6 +239,40 @@ def test_step_parity_moving_plate(ibhip):
     assert_state_close(state.case, orc, 1e-8)
 
 
+def test_step_parity_translating_plate_moves_body_box(ibhip, monkeypatch):
+    # a plate translating half a cell per step leaves the B-products' body box (16 cells of
+    # slack for motion) within 40 steps, so the box is recomputed and the captured graphs are
+    # re-captured mid-run; the steps match the oracle and are bitwise the ones without the box
+    # (the preconditioner is rebuilt on a fixed schedule: the adaptive rule times the device,
+    # which would let two runs rebuild at different steps)
+    import ibhip as m
+    from cases import plate_points
+    from ibhip.problem import (BodyGroup, GridFrame, MultiLevel, Problem, RelativeFrame,
+                               RigidBody, SchemeCNAB, StreamFcnFluid, UniformGrid)
+
+    n, U = 64, 2.0
+    grid = UniformGrid(5.12 / n, (-1.28, 3.84), (-2.56, 2.56))
+    frame = RelativeFrame(r=lambda t: (0.3 + U * t, 0.3), v=lambda t: (U, 0.0),
+                          theta=lambda t: -0.3, omega=lambda t: 0.0, base=GridFrame())
+    xb, ds = plate_points(1.0, 13)
+    prob = Problem(StreamFcnFluid(MultiLevel(grid, 2), (1.0, 0.0), 200.0),
+                   BodyGroup(RigidBody(frame, xb, ds)),
+                   SchemeCNAB(0.02, cg_reltol=1e-15, force_solver="pcg", pc_refresh=4))
+    nsteps = 40  # 1.56 units = 19.5 cells of travel
+    state, orc, hist = run_pair(ibhip, prob, nsteps)
+    assert np.all(np.abs(state.case.CD()) < 100.0)  # the flow stays bounded
+    assert_state_close(state.case, orc, 1e-8)
+    monkeypatch.setenv("IBH_BODY_BOX", "0")
+    st2 = m.initstate(prob)
+    sv2 = m.initsolver(prob, st2)
+    for k in range(nsteps):
+        m.advance_(st2, sv2, k * prob.scheme.dt)
+    for a, b in [(state.case.gamma(), st2.case.gamma()), (state.case.psi(), st2.case.psi()),
+                 (state.case.Fb(), st2.case.Fb()), (state.case.CD(), st2.case.CD())]:
+        assert np.array_equal(a, b)
+    st2.case.close()
+
+
 def test_step_parity_moving_plate_pcg(ibhip):
     # the preconditioned solve reaches the same tight tolerance as the oracle's plain cg!
     prob = cylinder_plate_case(n=64, nlev=2, cg_reltol=1e-15, cyl_spacing=1.0,
