@@ -282,7 +282,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
     {
         std::vector<NonlinA> a;
         std::vector<RhsA> ra;
-        std::vector<int> bl;
+        std::vector<int> bl, bln;
         std::vector<LapInvPlan> pI;
         std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
@@ -291,6 +291,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             a.push_back({d, c->q, d.nq, gam[i], d.ng, c->bcraw, d.nbc, 0.25, n0[i], d.ng, c->h, 0,
                          b->uv, c->L});
             bl.push_back(nblk(d.ng, GRP_BS) * c->L);
+            bln.push_back((int)(nl_tiles(d) * c->L));
             // A_ℓΓ as its exact stencil fused with the right-hand side, then Ainv_ℓ
             ra.push_back({d, gam[i], n0[i], n1[i], c->accbc, c->rhsb, c->h, c->dt, c->Re,
                           c->dt * c->beta[0], c->dt * c->beta[1], c->L});
@@ -301,7 +302,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, d.ng,
                           std::min(c->refine_levels, c->L)});
         }
-        add(v, OP_NONLIN, a, bl);
+        add(v, OP_NONLIN, a, bln);
         add(v, OP_RHS, ra, bl);
         add_lapinv(v, pI, b->prec);
         add_refine(b, v, rf);

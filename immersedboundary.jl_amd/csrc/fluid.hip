@@ -310,13 +310,48 @@ __device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q
     return v * (1 / (hc * hc));
 }
 
-__global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
-                            const double *__restrict__ g, long sg, const double *__restrict__ gbc,
-                            long sbc, double gbc_scale, double *__restrict__ out, long sout,
-                            double ux, double uy, double h, int lev0, int q0_given,
-                            const double *__restrict__ uv) {
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= d.ng) return;
+// The same values over an NL_TX×NL_TY tile of outputs per 256-thread workgroup: each thread
+// forms the face products it owns, fqx(i, j) and fqy(i, j), and the tile's first row and column
+// also the ones just below and left of the tile, so every product is formed once instead of
+// once per adjacent output; the sum is nonlinear_at's, term for term.
+__device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q,
+                                      const double *__restrict__ g,
+                                      const double *__restrict__ gbc, double gbc_scale, double U,
+                                      double V, double hc, long tile, double *__restrict__ out) {
+    __shared__ double fx[NL_TY + 1][NL_TX];
+    __shared__ double fy[NL_TY][NL_TX + 1];
+    NLCtx c;
+    c.d = d;
+    c.q = q;
+    c.g = g;
+    c.gbc = gbc;
+    c.U = U;
+    c.V = V;
+    c.gs = gbc_scale;
+    const int tx = threadIdx.x % NL_TX, ty = threadIdx.x / NL_TX;
+    const int ntx = (d.m + NL_TX - 1) / NL_TX;
+    const int i = 2 + (int)(tile % ntx) * NL_TX + tx, j = 2 + (int)(tile / ntx) * NL_TY + ty;
+    const bool in = i <= d.m + 1 && j <= d.p + 1;
+    if (in) {
+        fx[ty + 1][tx] = fqx(c, i, j);
+        fy[ty][tx + 1] = fqy(c, i, j);
+        if (ty == 0) fx[0][tx] = fqx(c, i, j - 1);
+        if (tx == 0) fy[ty][0] = fqy(c, i - 1, j);
+    }
+    __syncthreads();
+    if (in) {
+        const double v = (fx[ty][tx] - fx[ty + 1][tx]) + (fy[ty][tx + 1] - fy[ty][tx]);
+        out[(i - 2) + (long)d.m * (j - 2)] = v * (1 / (hc * hc));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
+                                                   const double *__restrict__ g, long sg,
+                                                   const double *__restrict__ gbc, long sbc,
+                                                   double gbc_scale, double *__restrict__ out,
+                                                   long sout, double ux, double uy, double h,
+                                                   int lev0, int q0_given,
+                                                   const double *__restrict__ uv) {
     if (uv) {  // freestream from the device parameter block (graph-replayable steps)
         ux = uv[0];
         uy = uv[1];
@@ -324,16 +359,18 @@ __global__ void k_nonlinear(Dom d, const double *__restrict__ q, long sq,
     const int b = blockIdx.y;
     const int lev = lev0 + b;
     const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
-    out[b * sout + t] = nonlinear_at(d, q + b * sq, g + b * sg, gbc + b * sbc, gbc_scale,
-                                     q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, t);
+    nonlinear_tile(d, q + b * sq, g + b * sg, gbc + b * sbc, gbc_scale,
+                   q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, blockIdx.x,
+                   out + b * sout);
 }
 
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
                       const double *gbc, long sbc, double gbc_scale, double *out, long sout,
                       double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
                       int q0_given, const double *uv) {
-    hipLaunchKernelGGL(k_nonlinear, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, q, sq, g,
-                       sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given, uv);
+    static_assert(NL_TX * NL_TY == 256, "one thread per tile output");
+    hipLaunchKernelGGL(k_nonlinear, dim3((unsigned)nl_tiles(d), nbatch), dim3(256), 0, s, d, q,
+                       sq, g, sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given, uv);
 }
 
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac[l]·bcraw[l] for ℓ ≤ L-2, accbc[L-1] = 0 (the rhsbc of
@@ -849,13 +886,18 @@ __global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, 
         copy_apply_bc_at(d, A.in ? A.in + b * A.sin : nullptr, A.rbc + b * A.sbc, A.fac, t);
 }
 
+// nl_tiles(d) workgroups per batch entry (level)
 __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, const int *off, int n) {
-    GRP_PROLOGUE(NonlinA, A.d.ng)
-    if (t >= d.ng) return;
+    static_assert(GRP_BS == NL_TX * NL_TY, "one thread per tile output");
+    const int p = grp_find(off, n, blockIdx.x);
+    const NonlinA A = args[p];
+    const long nt = nl_tiles(A.d);
+    const long lb = blockIdx.x - off[p];
+    const int b = (int)(lb / nt);
     const double hc = A.h * (double)(1L << (A.lev0 + b));
     const double ux = A.uv[0], uy = A.uv[1];
-    A.out[b * A.sout + t] = nonlinear_at(d, A.q + b * A.sq, A.g + b * A.sg, A.gbc + b * A.sbc,
-                                         A.gbc_scale, ux * hc, uy * hc, hc, t);
+    nonlinear_tile(A.d, A.q + b * A.sq, A.g + b * A.sg, A.gbc + b * A.sbc, A.gbc_scale, ux * hc,
+                   uy * hc, hc, lb % nt, A.out + b * A.sout);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *off, int n) {

@@ -246,6 +246,11 @@ struct RotA { Dom d; const double *q; long sq; double *g; long sg; int nbatch; }
 struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
 struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
 struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
+// the nonlinear term's output tiles per level (NL_TX × NL_TY outputs each)
+constexpr int NL_TX = 64, NL_TY = 4;
+__host__ __device__ inline long nl_tiles(const Dom &d) {
+    return (long)((d.m + NL_TX - 1) / NL_TX) * ((d.p + NL_TY - 1) / NL_TY);
+}
 struct NonlinA {
     Dom d; const double *q; long sq; const double *g; long sg; const double *gbc; long sbc;
     double gbc_scale; double *out; long sout; double h; int lev0; const double *uv; int nbatch;
