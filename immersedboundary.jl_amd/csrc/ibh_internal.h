@@ -228,14 +228,19 @@ struct Grp {
     int n, blocks;
 };
 
+// The problem p with off[p] <= b < off[p + 1] (off[0] = 0, non-decreasing; a problem without
+// blocks repeats its offset): the number of k in [1, n) with off[k] <= b.  Each wave reads 64
+// offsets per load and counts them with a ballot, one memory round trip per 64 problems
+// instead of a chain of dependent loads.  Called by whole waves (kernel entry).
 __device__ inline int grp_find(const int *off, int n, int b) {
-    int lo = 0, hi = n;  // off[lo] <= b < off[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= b) lo = mid;
-        else hi = mid;
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;
+    for (int k0 = 1; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const bool le = k < n && off[k] <= b;
+        cnt += __popcll(__ballot(le));
     }
-    return lo;
+    return cnt;
 }
 
 // Per-problem arguments; nbatch entries per problem with the given element strides (the same
