@@ -149,7 +149,7 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
             if (c->refine <= k) continue;
             const Dom &d = c->d;
             ra.push_back({d, it.x, it.rhs, it.a, it.a < 0 ? 1.0 : 0.25, c->rres, it.nlev});
-            bl.push_back(nblk(d.ng, GRP_BS) * it.nlev);
+            bl.push_back((int)grp_blocks(d.ng) * it.nlev);
             LapInvPlan P;
             lapinv_plan(c, c->rres, d.ng, it.x, d.ng, it.scale, it.sscale, it.nlev, EPI_SUBFROM,
                         nullptr, nullptr, &P);
@@ -203,7 +203,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             if (curl_all || L == 1) {
                 cu.push_back({d, Psi[i] + (L - 1) * d.ng, 0, bcL, 0,
                               Q[i] + (curl_all ? (L - 1) * d.nq : 0), 0, 1});
-                cb.push_back(nblk(d.nq, GRP_BS));
+                cb.push_back((int)grp_blocks(d.nq));
             }
         }
         add(v, OP_FILL, f, fb);
@@ -227,7 +227,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
             b1.push_back(nblk(d.nbc, GRP_BS));
             cp.push_back({d, G[i] + lev * d.ng, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
-            b2.push_back(nblk(d.ng, GRP_BS));
+            b2.push_back((int)grp_blocks(d.ng));
             LapInvPlan P;
             lapinv_plan(c, c->gtmp, d.ng, Psi[i] + lev * d.ng, d.ng, c->sD, 0, 1, EPI_STORE,
                         nullptr, nullptr, &P);
@@ -237,7 +237,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             if (curl_all || lev == 0) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
-                cb.push_back(nblk(d.nq, GRP_BS));
+                cb.push_back((int)grp_blocks(d.nq));
             }
         }
         add(v, OP_GETBC, gb, b1);
@@ -290,7 +290,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             const Dom &d = c->d;
             a.push_back({d, c->q, d.nq, gam[i], d.ng, c->bcraw, d.nbc, 0.25, n0[i], d.ng, c->h, 0,
                          b->uv, c->L});
-            bl.push_back(nblk(d.ng, GRP_BS) * c->L);
+            bl.push_back((int)grp_blocks(d.ng) * c->L);
             bln.push_back((int)(nl_tiles(d) * c->L));
             // A_ℓΓ as its exact stencil fused with the right-hand side, then Ainv_ℓ
             ra.push_back({d, gam[i], n0[i], n1[i], c->accbc, c->rhsb, c->h, c->dt, c->Re,

@@ -854,10 +854,36 @@ void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, in
     const Dom &d = A.d;                                              \
     (void)d;
 
+// GRP_EPT elements per thread (grp_blocks(ELEMS) workgroups per batch entry): every value is
+// formed before any is stored, so the loads of all of them are in flight together.  Defines A,
+// d, b and the thread's first element t0; workgroups past the problem's batch entries leave.
+#define GRP_PROLOGUE_N(TYPE, ELEMS)                                  \
+    const int p_ = grp_find(off, n, blockIdx.x);                     \
+    const TYPE A = args[p_];                                         \
+    const int lb_ = blockIdx.x - off[p_];                            \
+    const int nbx_ = (int)grp_blocks(ELEMS);                         \
+    const int b = lb_ / nbx_;                                        \
+    const Dom &d = A.d;                                              \
+    if (b >= A.nbatch) return;                                       \
+    const long t0 = (long)(lb_ % nbx_) * GRP_EPT * GRP_BS + threadIdx.x;
+
+#define GRP_ELEMENTS(N, OUT, VALUE)                                  \
+    {                                                                \
+        double v_[GRP_EPT];                                          \
+        _Pragma("unroll") for (int k_ = 0; k_ < GRP_EPT; ++k_) {     \
+            const long t = t0 + (long)k_ * GRP_BS;                   \
+            v_[k_] = t < (N) ? (VALUE) : 0.0;                        \
+        }                                                            \
+        _Pragma("unroll") for (int k_ = 0; k_ < GRP_EPT; ++k_) {     \
+            const long t = t0 + (long)k_ * GRP_BS;                   \
+            if (t < (N)) (OUT)[t] = v_[k_];                          \
+        }                                                            \
+    }
+
 __global__ __launch_bounds__(GRP_BS) void kg_curl(const CurlA *args, const int *off, int n) {
-    GRP_PROLOGUE(CurlA, A.d.nq)
-    if (t >= d.nq) return;
-    A.q[b * A.sq + t] = curl_at(d, A.psi + b * A.spsi, A.pbc + b * A.sbc, t);
+    GRP_PROLOGUE_N(CurlA, A.d.nq)
+    const double *psi = A.psi + b * A.spsi, *pbc = A.pbc + b * A.sbc;
+    GRP_ELEMENTS(d.nq, A.q + b * A.sq, curl_at(d, psi, pbc, t))
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_rot(const RotA *args, const int *off, int n) {
@@ -880,10 +906,9 @@ __global__ __launch_bounds__(GRP_BS) void kg_get_bc(const GetBcA *args, const in
 
 __global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, const int *off,
                                                            int n) {
-    GRP_PROLOGUE(CopyBcA, A.d.ng)
-    if (t >= d.ng) return;
-    A.out[b * A.sout + t] =
-        copy_apply_bc_at(d, A.in ? A.in + b * A.sin : nullptr, A.rbc + b * A.sbc, A.fac, t);
+    GRP_PROLOGUE_N(CopyBcA, A.d.ng)
+    const double *in = A.in ? A.in + b * A.sin : nullptr, *rbc = A.rbc + b * A.sbc;
+    GRP_ELEMENTS(d.ng, A.out + b * A.sout, copy_apply_bc_at(d, in, rbc, A.fac, t))
 }
 
 // nl_tiles(d) workgroups per batch entry (level)
@@ -901,19 +926,19 @@ __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, cons
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *off, int n) {
-    GRP_PROLOGUE(RhsA, A.d.ng)
-    if (t >= d.ng) return;
+    GRP_PROLOGUE_N(RhsA, A.d.ng)
     const double hc = A.h * (double)(1L << b);
     const double a = A.dt / (2 * A.Re * (hc * hc));
     const long o = b * d.ng;
-    A.out[o + t] = rhs_at(d, A.g + o, A.N1 + o, A.N2 + o, A.bc + (long)b * d.nbc, a, A.f1, A.f2, t);
+    const double *bc = A.bc + (long)b * d.nbc;
+    GRP_ELEMENTS(d.ng, A.out + o, rhs_at(d, A.g + o, A.N1 + o, A.N2 + o, bc, a, A.f1, A.f2, t))
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_resid(const ResidA *args, const int *off, int n) {
-    GRP_PROLOGUE(ResidA, A.d.ng)
-    if (t >= d.ng) return;
+    GRP_PROLOGUE_N(ResidA, A.d.ng)
     const long o = (long)b * d.ng;
-    A.r[o + t] = resid_at(d, A.x + o, A.rhs + o, level_coef(A.a, A.ar, b), t);
+    const double c = level_coef(A.a, A.ar, b);
+    GRP_ELEMENTS(d.ng, A.r + o, resid_at(d, A.x + o, A.rhs + o, c, t))
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {

@@ -279,6 +279,12 @@ __host__ __device__ inline long grp_elems_coarsify(const Dom &d) {
     return (long)(d.nx / 2 - 1) * (d.ny / 2 - 1);
 }
 static constexpr int GRP_BS = 256;
+// heavier grouped elementwise kernels take GRP_EPT elements per thread (at stride GRP_BS, so
+// each pass is coalesced): one problem lookup per GRP_EPT·GRP_BS elements
+static constexpr int GRP_EPT = 4;
+__host__ __device__ inline long grp_blocks(long elems) {
+    return (elems + (long)GRP_BS * GRP_EPT - 1) / ((long)GRP_BS * GRP_EPT);
+}
 
 // sum over the 64 lanes of a wave, in a fixed tree order; the total lands in lane 0
 __device__ inline double wave_sum(double v) {
