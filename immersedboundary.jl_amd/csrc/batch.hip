@@ -29,7 +29,8 @@ namespace {
 
 enum Op {
     OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
-    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS, OP_RESID
+    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS, OP_RESID,
+    OP_ETZS, OP_ROTI
 };
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -332,8 +333,8 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
     {
         std::vector<StressA> sa;
         std::vector<ETA> et;
-        std::vector<RotA> ro;
-        std::vector<int> b1, b2, b3, b4, b5;
+        std::vector<RotIA> ro;
+        std::vector<int> b1, b2, b3, b4;
         std::vector<LapInvPlan> plans;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
@@ -342,22 +343,20 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             sa.push_back({c->nbodies, c->d_boff, c->d_npts, c->nb, c->Fb, c->ds, c->h, c->dt, c->CD,
                           c->CL, c->fbs});
             b1.push_back(c->nbodies);
+            // Eᵀ into rot: zeroing and scale in one stage, rot reads the fixed-point image
             et.push_back({d, c->reg, c->Fb, c->qtmp, c->etscale});
-            b2.push_back(nblk(d.nq, GRP_BS));
-            b5.push_back(1);
+            b2.push_back(nblk(d.nq, GRP_BS) + 1);
             b3.push_back(nblk(2L * c->nb * GRP_NS * GRP_NS, GRP_BS));
-            ro.push_back({d, c->qtmp, d.nq, c->g1, d.ng, 1});
+            ro.push_back({d, c->qtmp, c->etscale, c->g1, 1});
             b4.push_back(nblk(d.ng, GRP_BS));
             LapInvPlan P;
             lapinv_plan(c, c->g1, d.ng, gs[i], d.ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr, &P);
             plans.push_back(P);
         }
         add(v, OP_STRESS, sa, b1);
-        add(v, OP_ETZ, et, b2);
-        add(v, OP_ETS, et, b5);
+        add(v, OP_ETZS, et, b2);
         add(v, OP_ET, et, b3);
-        add(v, OP_ETF, et, b2);
-        add(v, OP_ROT, ro, b4);
+        add(v, OP_ROTI, ro, b4);
         add_lapinv(v, plans, IBH_FP64);  // the projection stays FP64
     }
     add_vort2flux(b, v, gs, psi, q, true, true);
@@ -442,6 +441,8 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
     case OP_ETS: launch_grp_ET_scale(g, s); break;
     case OP_ET: launch_grp_ET(g, s); break;
     case OP_ETF: launch_grp_ET_fix(g, s); break;
+    case OP_ETZS: launch_grp_ET_zero_scale(g, s); break;
+    case OP_ROTI: launch_grp_rot_image(g, s); break;
     case OP_GEMVP: launch_grp_gemv_part(g, s); break;
     case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
     case OP_STRESS: launch_grp_stress(g, s); break;

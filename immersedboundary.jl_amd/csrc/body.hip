@@ -577,6 +577,21 @@ __global__ __launch_bounds__(GRP_BS) void kg_ET_zero(const ETA *args, const int 
     if (t < A.d.nq) A.q[t] = 0.0;
 }
 
+// zeroing of the fixed-point image (the first ceil(nq/256) workgroups of a problem) and its
+// scale (the last one)
+__global__ __launch_bounds__(GRP_BS) void kg_ET_zero_scale(const ETA *args, const int *off,
+                                                           int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const ETA A = args[p];
+    const long lb = blockIdx.x - off[p];
+    if (lb == (long)nblk(A.d.nq, GRP_BS)) {
+        et_scale_body(A.fb, 2 * A.r.nb, A.scale);
+        return;
+    }
+    const long t = lb * GRP_BS + threadIdx.x;
+    if (t < A.d.nq) A.q[t] = 0.0;
+}
+
 // one workgroup per problem
 __global__ __launch_bounds__(GRP_BS) void kg_ET_scale(const ETA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
@@ -632,6 +647,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const i
 GRP_LAUNCH(launch_grp_E, kg_E, EA)
 GRP_LAUNCH(launch_grp_ET_zero, kg_ET_zero, ETA)
 GRP_LAUNCH(launch_grp_ET_scale, kg_ET_scale, ETA)
+GRP_LAUNCH(launch_grp_ET_zero_scale, kg_ET_zero_scale, ETA)
 GRP_LAUNCH(launch_grp_ET, kg_ET, ETA)
 GRP_LAUNCH(launch_grp_ET_fix, kg_ET_fix, ETA)
 GRP_LAUNCH(launch_grp_gemv_part, kg_gemv_part, GemvA)

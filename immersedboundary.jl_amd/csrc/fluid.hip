@@ -115,7 +115,15 @@ void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int 
 }
 
 // the same from the fixed-point image of q that Eᵀ leaves: each entry converted exactly as the
-// conversion pass of Eᵀ does ((double)v · (1/scale)), then the same differences
+// conversion pass of Eᵀ does ((double)v · (1/scale), inv = 1/scale), then the same differences
+__device__ inline double rot_image_at(const Dom &d, const long long *__restrict__ q, double inv,
+                                      long t) {
+    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    const double ux0 = (double)QX(q, i, j - 1) * inv, ux1 = (double)QX(q, i, j) * inv;
+    const double vy1 = (double)QY(q, i, j) * inv, vy0 = (double)QY(q, i - 1, j) * inv;
+    return (ux0 - ux1) + (vy1 - vy0);
+}
+
 __global__ void k_rot_image(Dom d, const long long *__restrict__ img, long sq,
                             const double *__restrict__ scale, double *__restrict__ g, long sg,
                             const int *skip) {
@@ -123,12 +131,7 @@ __global__ void k_rot_image(Dom d, const long long *__restrict__ img, long sq,
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    const long long *q = img + b * sq;
-    const double inv = 1.0 / scale[b];
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
-    const double ux0 = (double)QX(q, i, j - 1) * inv, ux1 = (double)QX(q, i, j) * inv;
-    const double vy1 = (double)QY(q, i, j) * inv, vy0 = (double)QY(q, i - 1, j) * inv;
-    g[b * sg + t] = (ux0 - ux1) + (vy1 - vy0);
+    g[b * sg + t] = rot_image_at(d, img + b * sq, 1.0 / scale[b], t);
 }
 
 void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
@@ -892,6 +895,13 @@ __global__ __launch_bounds__(GRP_BS) void kg_rot(const RotA *args, const int *of
     A.g[b * A.sg + t] = rot_at(d, A.q + b * A.sq, t);
 }
 
+__global__ __launch_bounds__(GRP_BS) void kg_rot_image(const RotIA *args, const int *off, int n) {
+    GRP_PROLOGUE(RotIA, A.d.ng)
+    (void)b;  // one batch entry per problem
+    if (t >= d.ng) return;
+    A.g[t] = rot_image_at(d, reinterpret_cast<const long long *>(A.img), 1.0 / *A.scale, t);
+}
+
 __global__ __launch_bounds__(GRP_BS) void kg_coarsify(const CoarsA *args, const int *off, int n) {
     GRP_PROLOGUE(CoarsA, grp_elems_coarsify(A.d))
     if (t >= grp_elems_coarsify(d)) return;
@@ -964,6 +974,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_fill(const FillA *args, const int *
     }
 GRP_LAUNCH(launch_grp_curl, kg_curl, CurlA)
 GRP_LAUNCH(launch_grp_rot, kg_rot, RotA)
+GRP_LAUNCH(launch_grp_rot_image, kg_rot_image, RotIA)
 GRP_LAUNCH(launch_grp_coarsify, kg_coarsify, CoarsA)
 GRP_LAUNCH(launch_grp_get_bc, kg_get_bc, GetBcA)
 GRP_LAUNCH(launch_grp_copy_apply_bc, kg_copy_apply_bc, CopyBcA)

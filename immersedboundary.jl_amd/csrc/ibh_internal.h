@@ -248,6 +248,8 @@ __device__ inline int grp_find(const int *off, int n, int b) {
 // 256-thread workgroups; blocks(problem) = ceil(elements / 256) · nbatch.
 struct CurlA { Dom d; const double *psi; long spsi; const double *pbc; long sbc; double *q; long sq; int nbatch; };
 struct RotA { Dom d; const double *q; long sq; double *g; long sg; int nbatch; };
+// rot of Eᵀ's fixed-point image (img, as written by the ET stages, scale one double)
+struct RotIA { Dom d; const double *img; const double *scale; double *g; int nbatch; };
 struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
 struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
 struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
@@ -296,6 +298,7 @@ static constexpr int GV_CHUNK = 16; // GEMV column chunk
 
 void launch_grp_curl(const Grp &g, hipStream_t s);
 void launch_grp_rot(const Grp &g, hipStream_t s);
+void launch_grp_rot_image(const Grp &g, hipStream_t s);    // RotIA; blocks: ceil(nΓ/256)
 void launch_grp_coarsify(const Grp &g, hipStream_t s);
 void launch_grp_get_bc(const Grp &g, hipStream_t s);
 void launch_grp_copy_apply_bc(const Grp &g, hipStream_t s);
@@ -307,6 +310,7 @@ void launch_grp_fill(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
 void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
 void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)
 void launch_grp_ET_scale(const Grp &g, hipStream_t s);    // ETA; blocks: 1
+void launch_grp_ET_zero_scale(const Grp &g, hipStream_t s); // ETA; blocks: ceil(nq/256) + 1
 void launch_grp_ET(const Grp &g, hipStream_t s);          // ETA; blocks: ceil(2nb·169/256)
 void launch_grp_ET_fix(const Grp &g, hipStream_t s);      // ETA; blocks: ceil(nq/256)
 void launch_grp_gemv_part(const Grp &g, hipStream_t s);   // blocks: ceil(n/256)·ceil(n/64)
