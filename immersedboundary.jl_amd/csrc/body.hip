@@ -53,7 +53,10 @@ __global__ void k_reg_update(Dom d, RegDev r, const double *__restrict__ pts, do
     if (t >= r.nb * 4 * NS) return;
     const int k = t / (4 * NS), c = (t / NS) % 4, s = t % NS;
     const double xb = pts[k], yb = pts[r.nb + k];
-    const int bx = (int)floor((xb - x0) / h), by = (int)floor((yb - y0) / h);
+    // non-finite or far-off points get an index that support_inside rejects
+    const double fx = floor((xb - x0) / h), fy = floor((yb - y0) / h);
+    const int bx = fabs(fx) < 1e9 ? (int)fx : -(1 << 30);
+    const int by = fabs(fy) < 1e9 ? (int)fy : -(1 << 30);
     double v;
     if (c == 0) v = deltah(h * (double)(bx - 1 + (s - SUPP)) + x0, xb, h);
     else if (c == 1) v = deltah(h * (double)(by - 1 + (s - SUPP)) + y0 + h / 2, yb, h);
@@ -63,8 +66,8 @@ __global__ void k_reg_update(Dom d, RegDev r, const double *__restrict__ pts, do
     if (c == 0 && s == 0) {
         r.bidx[k] = bx;
         r.bidx[r.nb + k] = by;
-        if (bx - SUPP < 1 || bx + SUPP > d.nx || by - SUPP < 1 || by + SUPP > d.ny)
-            atomicOr(r.err, 1);
+        // E and Eᵀ skip such a point (see support_inside), so nothing outside q is touched
+        if (!support_inside(d, bx, by)) atomicOr(r.err, 1);
     }
 }
 
@@ -84,6 +87,10 @@ __device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__r
                                 double u0, double v0, int t, int lane) {
     const int comp = t / r.nb, k = t % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
+    if (!support_inside(d, bx, by)) {  // uniform over the wave; the host reports the flag
+        if (lane == 0) atomicOr(r.err, 1);
+        return 0.0;
+    }
     const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
     const double *wy = wx + NS;
     const double *base = comp == 0 ? q - 1 : q + d.nu - 1;
@@ -160,6 +167,10 @@ __device__ inline void ET_at(const Dom &d, const RegDev &r, const double *__rest
     const int s = t % NS, tt = (t / NS) % NS, kc = t / (NS * NS);
     const int comp = kc / r.nb, k = kc % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
+    if (!support_inside(d, bx, by)) {
+        atomicOr(r.err, 1);
+        return;
+    }
     const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
     const double f = fb[comp * r.nb + k];
     const double v = (wx[s] * wx[NS + tt]) * f;
@@ -397,7 +408,10 @@ __global__ void k_cg_init(CGDev cg) {
         cg.sc[2] = cg.reltol * res;
         cg.sc[3] = 1.0;
         *cg.it = 0;
-        *cg.done = (cg.maxiter <= 0 || res <= cg.reltol * res) ? 1 : 0;
+        *cg.done = !isfinite(res)                             ? CG_NONFINITE
+                   : res <= cg.reltol * res                   ? CG_CONVERGED
+                   : cg.maxiter <= 0                          ? CG_MAXITER
+                                                              : 0;
     }
 }
 
@@ -432,7 +446,9 @@ __global__ void k_cg_update(CGDev cg) {
         cg.sc[0] = nres;
         const int it = *cg.it + 1;
         *cg.it = it;
-        if (it >= cg.maxiter || nres <= cg.sc[2]) *cg.done = 1;
+        if (!isfinite(nres)) *cg.done = CG_NONFINITE;
+        else if (nres <= cg.sc[2]) *cg.done = CG_CONVERGED;
+        else if (it >= cg.maxiter) *cg.done = CG_MAXITER;
     }
 }
 

@@ -99,6 +99,10 @@ struct ibh_ctx {
     int bxa = 0, bxb = 0, bya = 0, byb = 0;
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
+    // host copy of the points last given (nb×2, x then y) and whether every point's delta
+    // support lies inside the finest grid (checked on the host before anything reads them)
+    std::vector<double> hpts;
+    bool pts_ok = true;
     int *d_boff = nullptr, *d_npts = nullptr;
     RegDev reg{};
     int *derr = nullptr;       // device error flags (bit 1 bounds, 2 non-finite, 4 singular)

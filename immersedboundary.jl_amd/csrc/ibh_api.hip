@@ -888,10 +888,60 @@ int update_box(ibh_ctx *c, const double *r) {
     return IBH_OK;
 }
 
+// Host check of body points before anything on the device reads them (the reference raises
+// BoundsError at the first E or Eᵀ, structure-ops.jl:70-72, 97-99, before writing anything):
+// every point must be finite with its 13×13 delta support inside the finest grid, computed as
+// k_reg_update computes body_idx.  A failing set of points is not uploaded.
+//
+// It also bounds the fixed-point spreading of Eᵀ: a contribution is at most
+// max δh(x)·δh(y) < 0.274 times 2^51 (the scale puts max|f| below 2^51), so an int64 cell
+// cannot overflow while fewer than 2^63 / (0.274·2^51) ≈ 14900 points reach one cell; the
+// check asks for half of that.  With at most SPREAD_MAX_COVER points in all it holds
+// trivially; beyond, the points in each occupied cell's ±2·6 window are counted.
+constexpr int SPREAD_MAX_COVER = 7000;
+int check_points(ibh_ctx *c, const double *r) {
+    const int nb = c->nb;
+    std::vector<std::pair<int, int>> cells(nb);
+    for (int k = 0; k < nb; ++k) {
+        const double fx = std::floor((r[k] - c->x0) / c->h);
+        const double fy = std::floor((r[nb + k] - c->y0) / c->h);
+        const bool fin = std::fabs(fx) < 1e9 && std::fabs(fy) < 1e9;
+        if (!fin || !support_inside(c->d, (int)fx, (int)fy)) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "body point %d at (%.17g, %.17g): its delta support leaves the finest "
+                          "grid (BoundsError)", k + 1, r[k], r[nb + k]);
+            return fail(c, IBH_EBOUNDS, msg);
+        }
+        cells[k] = {(int)fx, (int)fy};
+    }
+    if (nb > SPREAD_MAX_COVER) {
+        std::map<std::pair<int, int>, int> cnt;
+        for (auto &e : cells) ++cnt[e];
+        const int w = 2 * DELTA_SUPP;
+        for (auto &e : cnt) {
+            long n = 0;
+            for (auto it = cnt.lower_bound({e.first.first - w, e.first.second - w});
+                 it != cnt.end() && it->first.first <= e.first.first + w; ++it)
+                if (std::abs(it->first.second - e.first.second) <= w) n += it->second;
+            if (n > SPREAD_MAX_COVER)
+                return fail(c, IBH_EINVAL,
+                            "more than " + std::to_string(SPREAD_MAX_COVER) +
+                                " body points within one delta support: the fixed-point "
+                                "spreading would overflow");
+        }
+    }
+    return IBH_OK;
+}
+
 int upload_points(ibh_ctx *c, const double *r, const double *v) {
     if (c->nb == 0) return IBH_OK;
     if (r) {
-        int rc = update_box(c, r);
+        int rc = check_points(c, r);
+        if (rc) return rc;
+        c->hpts.assign(r, r + 2 * c->nb);
+        c->pts_ok = true;
+        rc = update_box(c, r);
         if (rc) return rc;
     }
     if (r) HIPCK(hipMemcpyAsync(c->pts, r, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
@@ -1076,6 +1126,9 @@ void drop_graphs(ibh_ctx *c) {
 }
 
 int step(ibh_ctx *c, double t, double ux, double uy) {
+    // moving bodies: points that were never checked (the description's body-frame points,
+    // when no ibh_set_body_points came first) are rejected before anything is enqueued
+    if (c->anymove && c->nb && !c->pts_ok) return check_points(c, c->hpts.data());
     c->ux = ux;
     c->uy = uy;
     launch_fill(c->uv, 1, ux, c->s);
@@ -1147,7 +1200,21 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     c->parity ^= 1;
     ++c->steps_done;
     c->t = t;
-    if (c->use_cg) return flag_error(c, st[2]);
+    if (c->use_cg) {
+        // the step is complete, as after the reference's cg!, which returns its last iterate
+        // without raising; the status says why the iteration stopped
+        int rc2 = flag_error(c, st[2]);
+        if (rc2) return rc2;
+        if (st[0] == CG_NONFINITE)
+            return fail(c, IBH_ENONFINITE, "non-finite residual in the force solve");
+        if (st[0] == CG_MAXITER) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg,
+                          "CG stopped at maxiter = %d without meeting reltol = %g (the step "
+                          "completed with the last iterate)", c->cg_maxiter, c->cg_reltol);
+            return fail(c, IBH_ENOCONV, msg);
+        }
+    }
     return IBH_OK;
 }
 
@@ -1397,6 +1464,11 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     }
     hipMemcpyAsync(c->fac, fac.data(), sizeof(double) * (L + 1), hipMemcpyHostToDevice, c->s);
     if (nb) {
+        // the initial points of moving bodies may be body-frame coordinates that the first
+        // ibh_set_body_points replaces: a failed check here is reported where they are used
+        c->hpts.assign(D.xb, D.xb + nf);
+        c->pts_ok = check_points(c, D.xb) == IBH_OK;
+        c->err.clear();
         CK(update_box(c, D.xb));
         hipMemcpyAsync(c->pts, D.xb, sizeof(double) * nf, hipMemcpyHostToDevice, c->s);
         hipMemcpyAsync(c->ds, D.ds, sizeof(double) * nb, hipMemcpyHostToDevice, c->s);
@@ -1450,6 +1522,7 @@ int ibh_setup(ibh_ctx *c) {
     if (!c) return IBH_EINVAL;
     // moving bodies under CG rebuild the regularization every step
     if (!(c->use_cg && c->anymove)) {
+        if (c->nb && !c->pts_ok) return check_points(c, c->hpts.data());
         launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
         int rc = check_flags(c);
         if (rc) return rc;
@@ -1473,18 +1546,25 @@ int ibh_advance(ibh_ctx *c, double t, double ux, double uy) {
     if (!c->setup_done) return fail(c, IBH_EINVAL, "ibh_setup must run before ibh_advance");
     int rc = step(c, t, ux, uy);
     if (rc) return rc;
-    if (c->anymove && !c->use_cg) return check_flags(c);
+    // the CG path read the step's flags mid-step; the dense path reads them here (one 4-byte
+    // copy after the step), so "infinite flux" and bounds errors surface from this call
+    if (!c->use_cg) return check_flags(c);
     return IBH_OK;
 }
 
 int ibh_advance_n(ibh_ctx *c, double t0, int32_t nsteps, double ux, double uy) {
     if (!c) return IBH_EINVAL;
     if (!c->setup_done) return fail(c, IBH_EINVAL, "ibh_setup must run before ibh_advance_n");
+    if (c->anymove)
+        return fail(c, IBH_EINVAL,
+                    "ibh_advance_n steps fixed bodies only: moving bodies need their points "
+                    "for every t (ibh_set_body_points + ibh_advance)");
     for (int k = 0; k < nsteps; ++k) {
         int rc = step(c, t0 + k * c->dt, ux, uy);
         if (rc) return rc;
     }
-    return IBH_OK;
+    // flags raised in any of the steps (they stay set until read) surface here
+    return check_flags(c);
 }
 
 int ibh_sync(ibh_ctx *c) {
@@ -1498,17 +1578,21 @@ int ibh_get_field(ibh_ctx *c, int32_t which, double *out) {
     const long n = field_len(c, which);
     if (n < 0) return fail(c, IBH_EINVAL, "unknown field");
     HIPCK(hipStreamSynchronize(c->s));
+    // a device error flag still set (e.g. from ibh_advance_n or an operator call) is returned
+    // after the copy, so the caller gets both the values and the error
+    int flags = 0;
+    HIPCK(hipMemcpy(&flags, c->derr, sizeof(int), hipMemcpyDeviceToHost));
     if (which == IBH_F_Q0) {  // base_flux!, fluid-ops.jl:403-417 (never materialized)
         const Dom &d = c->d;
         for (int l = 0; l < c->L; ++l) {
             const double hc = c->h * std::ldexp(1.0, l);
             for (long k = 0; k < d.nq; ++k) out[l * d.nq + k] = k < d.nu ? c->ux * hc : c->uy * hc;
         }
-        return IBH_OK;
+        return flags ? check_flags(c) : IBH_OK;
     }
-    if (n == 0) return IBH_OK;
-    HIPCK(hipMemcpy(out, field_ptr(c, which), sizeof(double) * n, hipMemcpyDeviceToHost));
-    return IBH_OK;
+    if (n > 0)
+        HIPCK(hipMemcpy(out, field_ptr(c, which), sizeof(double) * n, hipMemcpyDeviceToHost));
+    return flags ? check_flags(c) : IBH_OK;
 }
 
 int ibh_set_field(ibh_ctx *c, int32_t which, const double *in) {
@@ -1518,7 +1602,11 @@ int ibh_set_field(ibh_ctx *c, int32_t which, const double *in) {
         return fail(c, IBH_EINVAL, "field cannot be set");
     if (n == 0) return IBH_OK;
     if (which == IBH_F_POINTS_R) {
-        int rc = update_box(c, in);
+        int rc = check_points(c, in);
+        if (rc) return rc;
+        c->hpts.assign(in, in + n);
+        c->pts_ok = true;
+        rc = update_box(c, in);
         if (rc) return rc;
     }
     HIPCK(hipMemcpyAsync(field_ptr(c, which), in, sizeof(double) * n, hipMemcpyHostToDevice, c->s));

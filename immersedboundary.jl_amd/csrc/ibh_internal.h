@@ -167,6 +167,16 @@ void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, in
                          double dt, double Re, hipStream_t s);
 
 // ---------------------------------------------------------------------------- body kernels
+// The delta support of a point in cell (bx, by) (1-based, body_idx of structure-ops.jl:48-49)
+// covers qx[i, j], qy[i, j] for i ∈ bx-6..bx+6, j ∈ by-6..by+6; it stays inside both flux
+// arrays (qx (nx+1)×ny, qy nx×(ny+1)) when 1 ≤ i ≤ nx and 1 ≤ j ≤ ny.  The reference raises
+// BoundsError otherwise (structure-ops.jl:70-72, 97-99).
+constexpr int DELTA_SUPP = 6;
+__host__ __device__ inline bool support_inside(const Dom &d, int bx, int by) {
+    return bx >= 1 + DELTA_SUPP && bx <= d.nx - DELTA_SUPP && by >= 1 + DELTA_SUPP &&
+           by <= d.ny - DELTA_SUPP;
+}
+
 struct RegDev {
     int nb;
     int *bidx;     // 2·nb  (x index block, then y)
@@ -194,7 +204,9 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
                    const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
                    hipStream_t s);
 
-// CG state lives on the device; see body.hip
+// CG state lives on the device; see body.hip.  The `done` word: 0 running, otherwise why it
+// stopped (every nonzero value makes the B-products of later iterations skip).
+constexpr int CG_CONVERGED = 1, CG_MAXITER = 2, CG_NONFINITE = 3;
 struct CGDev {
     double *x, *b, *r, *c, *u;  // 2nb each
     double *z;                  // preconditioned residual (PCG)

@@ -48,6 +48,11 @@ class IbhError(RuntimeError):
         self.code = code
 
 
+class CGNotConverged(RuntimeWarning):
+    """The force solve's CG stopped at cg_maxiter.  The step completed with the last iterate,
+    as IterativeSolvers' cg! does when the reference calls it (cnab/ops.jl:110)."""
+
+
 class CaseDesc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nlev", C.c_int32),

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 from dataclasses import dataclass
 from typing import Any, Callable
 
@@ -98,9 +99,15 @@ class DeviceCase:
             r, v = f64(r), f64(v)
             self._check(self.lib.ibh_set_body_points(self.h, dptr(r), dptr(v)))
         ux, uy = prob.fluid.freestream(t)
-        self._check(self.lib.ibh_advance(self.h, float(t), float(ux), float(uy)))
+        rc = self.lib.ibh_advance(self.h, float(t), float(ux), float(uy))
+        if rc == _lib.ENOCONV:  # the step completed; cg! returns its last iterate silently
+            warnings.warn(self.lib.ibh_last_error(self.h).decode(), _lib.CGNotConverged,
+                          stacklevel=3)
+            return
+        self._check(rc)
 
     def advance_n(self, t0: float, n: int):
+        """n steps at t0 + k·dt with the freestream of t0 (fixed bodies only)."""
         ux, uy = self.prob.fluid.freestream(t0)
         self._check(self.lib.ibh_advance_n(self.h, float(t0), int(n), float(ux), float(uy)))
 

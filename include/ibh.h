@@ -30,7 +30,8 @@ enum ibh_status {
     IBH_EINVAL = 1,     /* bad sizes / arguments (e.g. nx % 4 != 0)                        */
     IBH_EBOUNDS = 2,    /* a body point's 13×13 delta support leaves the finest grid       */
     IBH_ENONFINITE = 3, /* non-finite spread flux ("infinite flux", structure-ops.jl:80-82) */
-    IBH_ENOCONV = 4,    /* CG reached cg_maxiter without meeting the tolerance              */
+    IBH_ENOCONV = 4,    /* CG reached cg_maxiter without meeting the tolerance; the step   */
+                        /* completed with the last iterate (cg! returns it, cnab/ops.jl:110) */
     IBH_EDEVICE = 5,    /* HIP runtime error or missing GPU                                 */
     IBH_ESINGULAR = 6   /* zero pivot in the dense Schur-complement inverse                 */
 };
@@ -114,20 +115,27 @@ int ibh_init_state(ibh_ctx *ctx, double t0, double ux, double uy);
  * (src/ImmersedBoundary.jl:52 → cnab/timestepping.jl:193-249). */
 int ibh_setup(ibh_ctx *ctx);
 /* update_bodies! output for moving bodies (src/bodies/body-points.jl:47-76): stacked points
- * and velocities at the coming step's t.  NULL keeps the current array. */
+ * and velocities at the coming step's t.  NULL keeps the current array.  Points that are not
+ * finite or whose 13×13 delta support leaves the finest grid are rejected with IBH_EBOUNDS
+ * before anything is uploaded (the reference's BoundsError, structure-ops.jl:70-72, 97-99). */
 int ibh_set_body_points(ibh_ctx *ctx, const double *r, const double *v);
 /* advance!(state, solver, t) (cnab/timestepping.jl:253-290), with the freestream (ux, uy)
- * = fluid.freestream(t) evaluated by the caller (fluid-ops.jl:403-417).  Enqueues the step
- * on the context stream and returns; getters synchronize. */
+ * = fluid.freestream(t) evaluated by the caller (fluid-ops.jl:403-417).  Returns once the
+ * step's device error flags are read: IBH_ENONFINITE ("infinite flux",
+ * structure-ops.jl:80-82), IBH_EBOUNDS, or IBH_ENOCONV from the CG force solve. */
 int ibh_advance(ibh_ctx *ctx, double t, double ux, double uy);
 /* advance! for nsteps steps at t0 + k·dt (k = 0..nsteps-1) with a constant freestream and
- * fixed bodies: the inner loop of solve! (src/solve.jl:198-212) without callbacks. */
+ * fixed bodies (IBH_EINVAL if any body moves): the inner loop of solve!
+ * (src/solve.jl:198-212) without callbacks.  Device error flags raised by any of the steps
+ * are returned at the end. */
 int ibh_advance_n(ibh_ctx *ctx, double t0, int32_t nsteps, double ux, double uy);
 int ibh_sync(ibh_ctx *ctx);
 
 /* ---- quantities / checkpoint ------------------------------------------------------------ */
 /* Copy a field to / from host memory (copy!/similar of streamfcn/states.jl:90-112 and
- * cnab/states.jl:23-27).  Sizes follow enum ibh_field. */
+ * cnab/states.jl:23-27).  Sizes follow enum ibh_field.  ibh_get_field copies the field and
+ * then returns any device error flag still set (and clears it).  ibh_set_field of
+ * IBH_F_POINTS_R checks the points as ibh_set_body_points does. */
 int ibh_get_field(ibh_ctx *ctx, int32_t which, double *host_out);
 int ibh_set_field(ibh_ctx *ctx, int32_t which, const double *host_in);
 /* out8 = {nx, ny, nlev, nq, nΓ, nb, nbc, nbodies} */

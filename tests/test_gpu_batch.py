@@ -79,8 +79,9 @@ def test_batch_matches_single_cases(precision):
         got = fields(st.case)
         assert len(got) == len(alone[i])
         for name, a, b in zip(NAMES, got, alone[i]):
-            # same per-element arithmetic; only the order of the FP64 atomics of the
-            # spreading operator differs between runs.  CL is held to the drag scale (it is
+            # the grouped kernels run each case's per-element arithmetic in the same order as
+            # the single-case kernels, and the spreading is deterministic (fixed point), so
+            # the cases agree to roundoff.  CL is held to the drag scale (it is
             # roundoff-seeded for the symmetric cylinder).
             if name == "CL":
                 err = np.abs(a - b).max() / np.abs(got[6]).max()

@@ -199,15 +199,21 @@ def run_pair(ibhip, prob, nsteps, t0=0.0):
 
 def assert_state_close(case, orc, tol):
     L = case.nlev
+    worst = {}
     for name, a, b in [("Γ", case.gamma(), orc.gamma()), ("ψ", case.psi(), orc.psi()),
                        ("q", case.q(), orc.q()), ("N1", case.nonlin(0), orc.nonlin(0)),
                        ("N2", case.nonlin(1), orc.nonlin(1))]:
         for lev in range(L):
             e = rel(a[:, lev], b[:, lev])
+            worst[name] = max(worst.get(name, 0.0), e)
             assert e < tol, f"{name} level {lev + 1}: {e:.3e}"
-    assert rel(case.Fb(), orc.Fb()) < tol
-    assert rel(case.CD(), orc.CD()) < tol
-    assert np.abs(case.CL() - orc.CL()).max() <= tol * np.abs(orc.CD()).max()
+    if case.nb:
+        worst["Fb"] = rel(case.Fb(), orc.Fb())
+        worst["CD"] = rel(case.CD(), orc.CD())
+        worst["CL"] = np.abs(case.CL() - orc.CL()).max() / np.abs(orc.CD()).max()
+    print("max rel err vs oracle: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+    if case.nb:
+        assert worst["Fb"] < tol and worst["CD"] < tol and worst["CL"] <= tol
 
 
 def test_step_parity_dense_cylinder(ibhip):
@@ -230,13 +236,13 @@ def test_step_parity_cg_stationary(ibhip):
     prob = cylinder_case(n=64, nlev=2, force_solver="cg", cg_reltol=1e-15, spacing=1.0)
     state, orc, hist = run_pair(ibhip, prob, 4)
     assert state.case.cg_iters() > 0
-    assert_state_close(state.case, orc, 1e-9)
+    assert_state_close(state.case, orc, 1e-10)
 
 
 def test_step_parity_moving_plate(ibhip):
     prob = cylinder_plate_case(n=64, nlev=2, cg_reltol=1e-15, cyl_spacing=1.0)
     state, orc, hist = run_pair(ibhip, prob, 3)
-    assert_state_close(state.case, orc, 1e-8)
+    assert_state_close(state.case, orc, 1e-10)
 
 
 def test_step_parity_translating_plate_moves_body_box(ibhip, monkeypatch):
@@ -261,7 +267,7 @@ def test_step_parity_translating_plate_moves_body_box(ibhip, monkeypatch):
     nsteps = 40  # 1.56 units = 19.5 cells of travel
     state, orc, hist = run_pair(ibhip, prob, nsteps)
     assert np.all(np.abs(state.case.CD()) < 100.0)  # the flow stays bounded
-    assert_state_close(state.case, orc, 1e-8)
+    assert_state_close(state.case, orc, 1e-10)
     monkeypatch.setenv("IBH_BODY_BOX", "0")
     st2 = m.initstate(prob)
     sv2 = m.initsolver(prob, st2)
@@ -279,7 +285,7 @@ def test_step_parity_moving_plate_pcg(ibhip):
                                force_solver="pcg", pc_refresh=2)
     state, orc, hist = run_pair(ibhip, prob, 5)
     assert state.case.cg_iters() < orc.cg_iters()
-    assert_state_close(state.case, orc, 1e-8)
+    assert_state_close(state.case, orc, 1e-10)
 
 
 def test_step_parity_moving_plate_pcg_cholesky(ibhip, monkeypatch):
@@ -289,7 +295,7 @@ def test_step_parity_moving_plate_pcg_cholesky(ibhip, monkeypatch):
                                force_solver="pcg", pc_refresh=2)
     state, orc, hist = run_pair(ibhip, prob, 4)
     assert state.case.cg_iters() < orc.cg_iters()
-    assert_state_close(state.case, orc, 1e-8)
+    assert_state_close(state.case, orc, 1e-10)
 
 
 def test_step_parity_moving_plate_pcg_truncated_columns(ibhip):
@@ -299,7 +305,7 @@ def test_step_parity_moving_plate_pcg_truncated_columns(ibhip):
                                force_solver="pcg", pc_refresh=2)
     state, orc, hist = run_pair(ibhip, prob, 4)
     assert state.case.cg_iters() < orc.cg_iters()
-    assert_state_close(state.case, orc, 1e-8)
+    assert_state_close(state.case, orc, 1e-10)
 
 
 def test_pcg_matches_dense_on_fixed_bodies(ibhip):

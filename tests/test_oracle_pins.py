@@ -15,10 +15,23 @@ transposed operand fails:
                            −u·∂ω/∂x (and −v·∂ω/∂y) pattern; Γ = 0 gives 0
   δh (Yang3)               zeroth/first moments, symmetry, continuity, support 3h (to the
                            1e-9 that the reference's 10-digit literals allow)
-  E / Eᵀ                   adjointness and partition of unity
+  E / Eᵀ                   adjointness and partition of unity; E of a linear velocity
+                           sampled at the staggered faces returns it at every body point
+                           (pins the support position and each component's half-cell offset)
+  avg_flux!                linear fluxes average to their midpoint values
+  direct_product!          on a sub-grid whose Γbc is the larger grid's Γ, the edge branches
+                           reproduce the larger grid's interior products (L/R/B/T blocks)
+  Vort2Flux                per level: coarsify! chain, −L₅ψ_ℓ = Γ_ℓ + edges(get_bc!(ψ_{ℓ+1})),
+                           q_ℓ = curl!(ψ_ℓ, bc) divergence-free
+  update_bodies!           r = r0 + R(θ)xb (= r0 + xb at θ = 0), v = dr/dt by central
+                           differences, fixed bodies unchanged
   B, Binv                  B·Binv = I; B·z equals the composition of the pinned operators
   full step                no-slip after projection: E(q₁ + q₀) = 0 for fixed bodies
                            (= v·h for moving ones); uniform flow without bodies stays exact
+
+Each pin was checked to fail under an injected mistake in a scratch copy of the oracle: a
+one-cell shift of the delta support, a dropped half-cell offset, B↔T and L↔R swaps of Γbc, a
+j+1 averaging index, a skipped Vort2Flux level, v := r and a flipped angular velocity.
 """
 import math
 
@@ -299,3 +312,162 @@ def test_step_enforces_body_velocity_moving_plate():
         r, v = prob.body_points(t)
         slip = o.E(o.q()[:, 0] + o.q0()[:, 0]) - v * prob.fluid.domain.base.h
         assert np.abs(slip).max() < 1e-8 * np.abs(o.E(o.q0()[:, 0])).max()
+
+
+# ------------------------------------------------------------------ pins of index patterns
+def _face_coords(prob):
+    """Coordinates of the flux unknowns (structure-ops.jl:48-57): qx[i, j] (1-based) at
+    (x0 + (i−1)h, y0 + (j−½)h), qy[i, j] at (x0 + (i−½)h, y0 + (j−1)h)."""
+    fl = prob.fluid
+    base = fl.domain.base
+    nx, ny, h, x0, y0 = fl.nx, fl.ny, base.h, base.x0, base.y0
+    ix, jx = np.meshgrid(np.arange(nx + 1), np.arange(ny), indexing="xy")  # x fastest
+    iy, jy = np.meshgrid(np.arange(nx), np.arange(ny + 1), indexing="xy")
+    ux = (x0 + ix * h, y0 + (jx + 0.5) * h)
+    vy = (x0 + (iy + 0.5) * h, y0 + jy * h)
+    return ux, vy
+
+
+def test_E_reproduces_linear_velocity_at_the_body_points():
+    """E of q sampled from a linear field at the staggered face positions returns the field
+    at each body point (zeroth and first moments of δh).  A one-cell shift of the support or
+    of a component's half-cell offset moves the result by b·h or c·h/2."""
+    prob = cylinder_case(n=64, nlev=2)
+    o = ibref.OracleRun(prob, setup=False)
+    rng = np.random.default_rng(11)
+    nb = o.nb
+    # points anywhere in the interior (not just the circle), all cell offsets
+    r = np.concatenate([rng.uniform(-0.4, 2.4, nb), rng.uniform(-1.4, 1.4, nb)])
+    o.reg_update(r)
+    (xu, yu), (xv, yv) = _face_coords(prob)
+    fu = lambda x, y: 0.7 + 1.9 * x - 2.3 * y  # noqa: E731
+    fv = lambda x, y: -0.4 - 1.1 * x + 0.8 * y  # noqa: E731
+    q = np.concatenate([fu(xu, yu).reshape(-1), fv(xv, yv).reshape(-1)])
+    got = o.E(q)
+    X, Y = r[:nb], r[nb:]
+    scale = np.abs(q).max()
+    assert np.abs(got[:nb] - fu(X, Y)).max() < 2e-8 * scale
+    assert np.abs(got[nb:] - fv(X, Y)).max() < 2e-8 * scale
+
+
+def test_avg_flux_of_linear_flux_is_the_midpoint_value():
+    """avg_flux! (fluid-ops.jl:277-297): Qx[i, j] = (qx + q0x)(i, j − ½), Qy[i, j] =
+    (qy + q0y)(i − ½, j) for linear fields of the face index (the /2 of four terms)."""
+    nx, ny = 20, 16
+    nu = (nx + 1) * ny
+    i, j = np.meshgrid(np.arange(1, nx + 2), np.arange(1, ny + 1), indexing="xy")
+    a, b = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 2), indexing="xy")
+    fx = lambda i, j: 0.3 + 1.2 * i - 0.7 * j  # noqa: E731
+    gx = lambda i, j: -1.1 + 0.4 * i + 2.1 * j  # noqa: E731
+    fy = lambda i, j: 2.2 - 0.9 * i + 0.5 * j  # noqa: E731
+    gy = lambda i, j: 0.6 + 1.7 * i + 0.2 * j  # noqa: E731
+    q = np.concatenate([fx(i, j).reshape(-1), fy(a, b).reshape(-1)])
+    q0 = np.concatenate([gx(i, j).reshape(-1), gy(a, b).reshape(-1)])
+    Q = ibref.avg_flux(nx, ny, q, q0)
+    Qx, Qy = split(Q, nx, ny)  # 0-based [i−1, j−1]
+    I, J = np.meshgrid(np.arange(1, nx + 2), np.arange(2, ny + 1), indexing="ij")
+    assert np.abs(Qx[:, 1:] - (fx(I, J - 0.5) + gx(I, J - 0.5))).max() < 1e-12
+    I, J = np.meshgrid(np.arange(2, nx + 1), np.arange(1, ny + 2), indexing="ij")
+    assert np.abs(Qy[1:, :] - (fy(I - 0.5, J) + gy(I - 0.5, J))).max() < 1e-12
+
+
+def _sub_fields(big, nx2, ny2, ox, oy, nx, ny):
+    """Restrict a flux vector, a vertex field and boundary values of an nx2×ny2 grid to the
+    nx×ny grid whose lower-left vertex is big vertex (ox, oy)."""
+    qx2, qy2 = split(big["q"], nx2, ny2)
+    q0x2, q0y2 = split(big["q0"], nx2, ny2)
+    sq = lambda ax, ay: np.concatenate([ax[ox:ox + nx + 1, oy:oy + ny].T.reshape(-1),  # noqa: E731
+                                        ay[ox:ox + nx, oy:oy + ny + 1].T.reshape(-1)])
+    V = big["V"]  # all vertices (nx2+1)×(ny2+1), including the big grid's boundary
+    Vs = V[ox:ox + nx + 1, oy:oy + ny + 1]
+    g = Vs[1:-1, 1:-1].T.reshape(-1)
+    bc = np.concatenate([Vs[0, :], Vs[nx, :], Vs[:, 0], Vs[:, ny]])  # L, R, B, T
+    return sq(qx2, qy2), sq(q0x2, q0y2), g, bc
+
+
+def test_direct_product_edges_extend_the_interior_formula():
+    """direct_product! (fluid-ops.jl:307-401): on a sub-grid whose Γbc holds the larger grid's
+    Γ on the sub-grid's boundary vertices, the product equals the larger grid's on every
+    face it computes, so each edge branch reads the right Γbc block (L, R, B, T) and index."""
+    nx2, ny2 = 32, 28
+    rng = np.random.default_rng(12)
+    nq2 = (nx2 + 1) * ny2 + nx2 * (ny2 + 1)
+    V = rng.standard_normal((nx2 + 1, ny2 + 1))
+    big = {"q": rng.standard_normal(nq2), "q0": rng.standard_normal(nq2), "V": V}
+    g2 = V[1:-1, 1:-1].T.reshape(-1)
+    bc2 = np.concatenate([V[0, :], V[nx2, :], V[:, 0], V[:, ny2]])
+    fq2 = ibref.direct_product(nx2, ny2, ibref.avg_flux(nx2, ny2, big["q"], big["q0"]), g2, bc2)
+    fqx2, fqy2 = split(fq2, nx2, ny2)
+    for ox, oy, nx, ny in ((6, 5, 16, 12), (1, 1, 20, 24), (12, 3, 20, 24)):
+        q, q0, g, bc = _sub_fields(big, nx2, ny2, ox, oy, nx, ny)
+        fq = ibref.direct_product(nx, ny, ibref.avg_flux(nx, ny, q, q0), g, bc)
+        fqx, fqy = split(fq, nx, ny)
+        # x fluxes i = 2..nx (all j), y fluxes j = 2..ny (all i), 1-based; the edge branches
+        # list the two products in the other order, which changes the rounding of a fused
+        # multiply-add, so equality holds to roundoff
+        assert rel(fqx[1:nx, :], fqx2[ox + 1:ox + nx, oy:oy + ny]) < 1e-15
+        assert rel(fqy[:, 1:ny], fqy2[ox:ox + nx, oy + 1:oy + ny]) < 1e-15
+        assert np.count_nonzero(fqx[[0, nx], :]) == 0 and np.count_nonzero(fqy[:, [0, ny]]) == 0
+
+
+def test_vort2flux_levels_telescope_by_poisson_residuals():
+    """Vort2Flux (fluid-ops.jl:440-472), level by level with independent operators: the coarse
+    centres hold the coarsify! chain of the finer Γ; the coarsest ψ solves −L₅ψ = Γ with zero
+    boundary values; every finer ψ_ℓ solves −L₅ψ_ℓ = Γ_ℓ + edges(get_bc!(ψ_{ℓ+1})); q_ℓ is
+    curl!(ψ_ℓ) with those boundary values and is divergence-free."""
+    nx = ny = 32
+    L = 4
+    prob = _nobody(nx, L)
+    o = ibref.OracleRun(prob, setup=False)
+    m = p = nx - 1
+    rng = np.random.default_rng(13)
+    G0 = rng.standard_normal((o.ng, L))
+    psi, q, G = o.vort2flux(G0)
+    # coarse centres: coarsify! of the (already modified) finer level, pinned separately
+    assert np.array_equal(G[:, 0], G0[:, 0])
+    for lev in range(1, L):
+        assert np.array_equal(G[:, lev], ibref.coarsify(nx, ny, G0[:, lev], G[:, lev - 1]))
+    zero = np.zeros(o.nbc)
+    for lev in range(L):
+        bc = zero if lev == L - 1 else ibref.get_bc(nx, ny, psi[:, lev + 1])
+        rhs = ibref.apply_bc(nx, ny, G[:, lev], bc, 1.0)
+        assert rel(neg_lap5(psi[:, lev], m, p), rhs) < 1e-12, lev
+        assert rel(q[:, lev], ibref.curl_bc(nx, ny, psi[:, lev], bc)) == 0, lev
+        qx, qy = split(q[:, lev], nx, ny)
+        div = qx[1:, :] - qx[:-1, :] + qy[:, 1:] - qy[:, :-1]
+        assert np.abs(div).max() < 1e-12 * np.abs(q[:, lev]).max()
+
+
+# ------------------------------------------------------------------ body kinematics
+def test_body_points_kinematics():
+    """update_bodies! (body-points.jl:47-76, with the frame's Ω and v from the kinematics):
+    a RelativeFrame body's points are r0 + R(θ)·xb, so at θ = 0 they are r0 + xb, and their
+    velocity is the time derivative of those positions (central difference, O(ε²)); bodies
+    fixed in the grid keep xb with zero velocity; bodies are stacked in order, x block first."""
+    prob = cylinder_plate_case(n=64, nlev=2)
+    cyl, plate = prob.bodies
+    nb, nc = prob.bodies.npoints, cyl.npoints
+    fr = plate.frame
+    # the plate's angle crosses zero nowhere (30° ± 10°): use a frame with θ(t0) = 0
+    from ibhip.problem import RelativeFrame, RigidBody, BodyGroup, Problem, GridFrame
+
+    om = 0.9
+    spin = RelativeFrame(r=lambda t: (0.3 * t + 1.0, -0.2 * t * t), v=lambda t: (0.3, -0.4 * t),
+                         theta=lambda t: math.sin(om * t), omega=lambda t: om * math.cos(om * t),
+                         base=GridFrame())
+    body = RigidBody(spin, plate.xb, plate.ds)
+    prob2 = Problem(prob.fluid, BodyGroup(cyl, body), prob.scheme)
+    r, v = prob2.body_points(0.0)
+    npl = body.npoints
+    assert np.array_equal(r[:nc], cyl.xb[:, 0]) and np.array_equal(r[nb:nb + nc], cyl.xb[:, 1])
+    assert np.count_nonzero(v[:nc]) == 0 and np.count_nonzero(v[nb:nb + nc]) == 0
+    assert np.abs(r[nc:nb] - (1.0 + plate.xb[:, 0])).max() < 1e-15
+    assert np.abs(r[nb + nc:] - plate.xb[:, 1]).max() < 1e-15
+    for pb in (prob, prob2):
+        for t in (0.0, 0.37, 2.1):
+            eps = 1e-5
+            rp, _ = pb.body_points(t + eps)
+            rm, _ = pb.body_points(t - eps)
+            _, vt = pb.body_points(t)
+            assert np.abs((rp - rm) / (2 * eps) - vt).max() < 1e-8
+    assert fr is plate.frame and npl == plate.npoints
