@@ -132,6 +132,8 @@ struct ibh_ctx {
     double *uv = nullptr;      // freestream (ux, uy) of the current step
     int *status = nullptr;     // [CG done, CG iterations, error flags, pad]
     bool use_graphs = true, capturing = false;
+    bool no_alloc = false;  // inside ibh_op_lapinv_noalloc: no tuning runs, no profiling
+    int dev = 0;            // the device current at ibh_create
     int parity = 0;
     long steps_done = 0;
     hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter = nullptr,

@@ -13,6 +13,7 @@
 #include <string>
 #include <utility>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -169,7 +170,10 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) return it->second;
     if (c->tune_fixed >= 0) return c->tuned[key] = c->tune_fixed;
-    if (c->capturing) return 0;  // never time inside a graph capture (shapes are tuned before)
+    // never time inside a graph capture (shapes are tuned before) or in the entry point that
+    // allocates nothing (the timing needs a scratch output): untuned shapes take variant 0,
+    // which gives the same bits as every other
+    if (c->capturing || c->no_alloc) return 0;
     const long ext = std::max((long)g0.M, (long)g0.N) * g0.ldc;
     const size_t need = (size_t)std::max(0L, (long)(g0.batch - 1)) * std::max(g0.sC, 0L) + ext;
     if (need > c->tune_cap) {
@@ -229,7 +233,7 @@ double gemm_flops(const GemmArgs &g, int kind) {
 
 void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
     const int v = tune(c, g, prec, kind, tout);
-    if (c->prof.on) {
+    if (c->prof.on && !c->no_alloc) {
         g.ts = prof_slot(c->prof);
         hipEvent_t e0;
         prof_begin(c, &e0);
@@ -1346,7 +1350,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     }
     c->cg_maxiter = D.cg_maxiter > 0 ? D.cg_maxiter : 5000;
     c->cg_reltol = D.cg_reltol > 0 ? D.cg_reltol : 1e-12;
-    if (hipStreamCreate(&c->s) != hipSuccess) {
+    if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->s) != hipSuccess) {
         delete c;
         return IBH_EDEVICE;
     }
@@ -1792,6 +1796,67 @@ int ibh_op_B(ibh_ctx *c, const double *z, double *x) {
     double *dz = B.up(z, nf), *dx = B.get(nf);
     B_times(c, dz, 0, dx, 0, 1, nullptr);
     return down(c, x, dx, nf);
+}
+
+// ---------------------------------------------------------------- allocation-free operator
+// The two intermediates of a LaplacianInv apply (T1, T2 of lapinv) for callers that cannot
+// allocate: one library-wide array per GPU, held until the process ends.  Its host-side name
+// is not a device address: kernels get the address hipGetSymbolAddress returns for the device
+// (looked up once per device on the first call there and kept in noalloc_ptr).
+constexpr long NOALLOC_MAX_NG = IBH_NOALLOC_MAX_NG;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_NG];
+static constexpr int NOALLOC_MAX_DEV = 64;
+static double *noalloc_ptr[NOALLOC_MAX_DEV];
+static std::mutex noalloc_mu;
+
+static int noalloc_scratch(ibh_ctx *c, double **out) {
+    if (c->dev < 0 || c->dev >= NOALLOC_MAX_DEV)
+        return fail(c, IBH_EDEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[c->dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, IBH_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+        noalloc_ptr[c->dev] = static_cast<double *>(p);
+    }
+    *out = noalloc_ptr[c->dev];
+    return IBH_OK;
+}
+
+int ibh_op_lapinv_noalloc(ibh_ctx *c, int32_t mode, int32_t lev, const double *in,
+                          double *out) {
+    if (!c || lev < 0 || lev >= c->L || mode < 0 || mode > 2 || !in || !out)
+        return IBH_EINVAL;
+    const long ng = c->d.ng;
+    if (ng > NOALLOC_MAX_NG)
+        return fail(c, IBH_EINVAL,
+                    "ibh_op_lapinv_noalloc: " + std::to_string(ng) + " interior points exceed "
+                    "the scratch limit of " + std::to_string(NOALLOC_MAX_NG) +
+                    " (use ibh_op_lapinv)");
+    if (in == out) return fail(c, IBH_EINVAL, "ibh_op_lapinv_noalloc: in and out must differ");
+    int cur = 0;
+    HIPCK(hipGetDevice(&cur));
+    if (cur != c->dev) HIPCK(hipSetDevice(c->dev));
+    double *S = nullptr;
+    int rc = noalloc_scratch(c, &S);
+    if (!rc) {
+        const double *sc = mode == 0 ? c->sD : (mode == 1 ? c->sA : c->sAinv) + lev * ng;
+        // lapinv's intermediates, pointed at the scratch array for this call only
+        double *T1 = c->T1, *T2 = c->T2;
+        c->T1 = S;
+        c->T2 = S + NOALLOC_MAX_NG;
+        c->no_alloc = true;
+        lapinv(c, in, ng, out, ng, sc, 0, 1, EPI_STORE, nullptr, nullptr);
+        c->no_alloc = false;
+        c->T1 = T1;
+        c->T2 = T2;
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(c, IBH_EDEVICE, hipGetErrorString(e));
+    }
+    if (cur != c->dev) hipSetDevice(cur);
+    return rc;
 }
 
 int ibh_tune_report(ibh_ctx *c, char *buf, int32_t len) {

@@ -14,6 +14,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "libibhip.so")
 
 OK, EINVAL, EBOUNDS, ENONFINITE, ENOCONV, EDEVICE, ESINGULAR = range(7)
+NOALLOC_MAX_NG = 1048576  # IBH_NOALLOC_MAX_NG of include/ibh.h
 STATUS_NAMES = {
     EINVAL: "invalid argument",
     EBOUNDS: "body outside the finest grid",
@@ -33,7 +34,7 @@ EXPORTED = [
     "ibh_op_lapinv", "ibh_op_dst2",
     "ibh_op_rot", "ibh_op_curl", "ibh_op_coarsify", "ibh_op_get_bc", "ibh_op_apply_bc",
     "ibh_op_nonlinear", "ibh_op_vort2flux", "ibh_op_reg_update", "ibh_op_E", "ibh_op_ET",
-    "ibh_op_B", "ibh_profile_enable", "ibh_profile_read", "ibh_profile_read_events",
+    "ibh_op_B", "ibh_op_lapinv_noalloc", "ibh_profile_enable", "ibh_profile_read", "ibh_profile_read_events",
     "ibh_tune_report",
     "ibh_batch_create", "ibh_batch_destroy", "ibh_batch_last_error", "ibh_batch_advance",
     "ibh_batch_advance_n", "ibh_batch_sync", "ibh_batch_launches_per_step",
@@ -115,6 +116,7 @@ def load():
     L.ibh_op_E.argtypes = [vp, dp, dp]
     L.ibh_op_ET.argtypes = [vp, dp, dp]
     L.ibh_op_B.argtypes = [vp, dp, dp]
+    L.ibh_op_lapinv_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]  # device pointers
     L.ibh_profile_enable.argtypes = [vp, C.c_int32]
     L.ibh_profile_read.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
     L.ibh_profile_read_events.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]

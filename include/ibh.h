@@ -180,6 +180,20 @@ int ibh_op_ET(ibh_ctx *ctx, const double *fb, double *q);
 /* B·z (B_Times, cnab/ops.jl:125-143): z (2nb) → x (2nb). */
 int ibh_op_B(ibh_ctx *ctx, const double *z, double *x);
 
+/* LaplacianInv apply as ibh_op_lapinv, on device arrays, allocating nothing.  No reference
+ * counterpart: for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  in and out are distinct device arrays of
+ * nΓ doubles; the apply is enqueued on the context's stream and the call returns without
+ * synchronizing (ibh_sync waits).  The two intermediates live in one library-wide device array
+ * per GPU of 2·IBH_NOALLOC_MAX_NG doubles (16 MiB), held until the process ends: grids with
+ * more than IBH_NOALLOC_MAX_NG interior points (1024², e.g. 1025×1025 cells) are rejected with
+ * IBH_EINVAL, and calls that overlap on one GPU (contexts on different streams, or threads)
+ * race on that array.  Contraction shapes not tuned before by the context run an untuned tile
+ * variant (same results). */
+#define IBH_NOALLOC_MAX_NG 1048576
+int ibh_op_lapinv_noalloc(ibh_ctx *ctx, int32_t mode, int32_t lev, const double *in_dev,
+                          double *out_dev);
+
 /* ---- batches of independent cases ------------------------------------------------------- */
 /* Many independent simulations (the dozens of cases a parameter sweep runs through
  * solve(prob, tspan) one after another, src/solve.jl:154-186) advanced together: every
