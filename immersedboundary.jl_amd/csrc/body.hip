@@ -309,6 +309,50 @@ void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s)
     hipLaunchKernelGGL(k_axpy_sub, dim3(nblk(n, 256)), dim3(256), 0, s, n, y, v, a);
 }
 
+// Rank-k coarse-level correction of a B-product (see ibh_api.hip): w = Vᵀ z with one
+// 256-thread workgroup per entry of w (fixed-order sums), then x += U w with one thread per
+// entry of x summing over the k columns in order.
+__global__ __launch_bounds__(256) void k_lr_dot(int n, int k, const double *__restrict__ V,
+                                                const double *__restrict__ z, long sz,
+                                                double *__restrict__ w, const int *skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[4];
+    const int j = blockIdx.x, b = blockIdx.y;
+    const double *Vj = V + (long)n * j, *zb = z + b * sz;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += Vj[i] * zb[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) w[(long)b * k + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void k_lr_axpy(int n, int k, const double *__restrict__ U,
+                          const double *__restrict__ w, double *__restrict__ x, long sx,
+                          const int *skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = blockIdx.y;
+    const double *wb = w + (long)b * k;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += U[i + (long)n * j] * wb[j];
+    x[b * sx + i] += s;
+}
+
+void launch_lr_dot(int n, int k, const double *V, const double *z, long sz, double *w, int nbatch,
+                   const int *skip, hipStream_t s) {
+    if (n == 0 || k == 0) return;
+    hipLaunchKernelGGL(k_lr_dot, dim3(k, nbatch), dim3(256), 0, s, n, k, V, z, sz, w, skip);
+}
+
+void launch_lr_axpy(int n, int k, const double *U, const double *w, double *x, long sx,
+                    int nbatch, const int *skip, hipStream_t s) {
+    if (n == 0 || k == 0) return;
+    hipLaunchKernelGGL(k_lr_axpy, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, n, k, U, w, x, sx,
+                       skip);
+}
+
 // ---------------------------------------------------------------- block reductions (FP64)
 template <int BS>
 __device__ double block_sum(double v, double *sh) {

@@ -97,6 +97,15 @@ struct ibh_ctx {
     // Its forward transforms sum only the box's pairs and its Poisson solve forms ψ only on it.
     bool box_B = true, box_on = false;
     int bxa = 0, bxb = 0, bya = 0, byb = 0;
+    // Coarse-level part of the B-products, kept at low rank for the CG iterations (see
+    // ibh_api.hip): T ≈ Q·Zᵀ on the body box, stored as the flux fields Q and V = curl Z
+    // (lr_k columns of nq each), with EQ = E·Q and EV = E·V (2nb × lr_k) at the current body
+    // positions; sAD = κ·sAinv₁ ⊙ sD, the level-1 Poisson spectrum of Ainv₁'s input.
+    bool lr_on = true, lr_valid = false, lr_failed = false, lr_bypass = false;
+    int lr_k = 0, lr_builds = 0;
+    double lr_err = 0, lr_build_s = 0;
+    double *lrQ = nullptr, *lrV = nullptr, *lrEQ = nullptr, *lrEV = nullptr, *lrW = nullptr;
+    double *sAD = nullptr, *zerobc = nullptr;
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
     // host copy of the points last given (nb×2, x then y) and whether every point's delta

@@ -984,4 +984,161 @@ GRP_LAUNCH(launch_grp_rhs, kg_rhs, RhsA)
 GRP_LAUNCH(launch_grp_resid, kg_resid, ResidA)
 GRP_LAUNCH(launch_grp_fill, kg_fill, FillA)
 
+
+// ------------------------------------------------------------------ adjoint transfers
+// Adjoints of the multilevel transfers (exact transposes of the maps above, same index sets),
+// for the coarse-level part of the B-product and its adjoint (see ibh_api.hip).
+
+// bc = edgesᵀ(f): the transpose of apply_bc!'s edge addition (fluid-ops.jl:249-275): each bc
+// entry that apply_bc! adds to a field entry reads that entry; the others are zero.
+__global__ void k_edges_adj(Dom d, const double *__restrict__ f, long sf,
+                            double *__restrict__ bc, long sbc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nbc) return;
+    const double *F = f + blockIdx.y * sf;
+    const int k = t + 1;  // 1-based bc index
+    double v = 0.0;
+    if (k > d.To) {
+        const int i = k - d.To - 1;  // BC((To + 1) + i), i in 1..m
+        if (i >= 1 && i <= d.m) v = GI(F, i, d.p);
+    } else if (k > d.Bo) {
+        const int i = k - d.Bo - 1;
+        if (i >= 1 && i <= d.m) v = GI(F, i, 1);
+    } else if (k > d.Ro) {
+        const int j = k - d.Ro - 1;
+        if (j >= 1 && j <= d.p) v = GI(F, d.m, j);
+    } else {
+        const int j = k - d.Lo - 1;
+        if (j >= 1 && j <= d.p) v = GI(F, 1, j);
+    }
+    bc[blockIdx.y * sbc + t] = v;
+}
+
+void launch_edges_adj(const Dom &d, const double *f, long sf, double *bc, long sbc, int nbatch,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_edges_adj, dim3(nblk(d.nbc, 256), nbatch), dim3(256), 0, s, d, f, sf, bc,
+                       sbc);
+}
+
+// a = get_bcᵀ(bc) on the whole coarse field (zero off the four lines get_bc! reads): a line
+// vertex t steps from the line's first vertex receives 0.25·bc(2t+1) from the copied value and
+// 0.125 from each averaged neighbour bc(2t), bc(2t+2) (fluid-ops.jl:210-239).
+__device__ inline double gbc_adj_line(const double *__restrict__ bc, int off, int t, int half) {
+    if (t < 0 || t > half) return 0.0;
+    double v = 0.25 * BC(bc, off + 2 * t + 1);
+    if (t >= 1) v += 0.125 * BC(bc, off + 2 * t);
+    if (t + 1 <= half) v += 0.125 * BC(bc, off + 2 * t + 2);
+    return v;
+}
+
+__global__ void k_get_bc_adj(Dom d, const double *__restrict__ bc, long sbc,
+                             double *__restrict__ a, long sa) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const double *B = bc + blockIdx.y * sbc;
+    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const int nx = d.nx, ny = d.ny;
+    double v = 0.0;
+    if (j == ny / 4) v += gbc_adj_line(B, d.Bo, i - nx / 4, nx / 2);
+    if (j == 3 * ny / 4) v += gbc_adj_line(B, d.To, i - nx / 4, nx / 2);
+    if (i == nx / 4) v += gbc_adj_line(B, d.Lo, j - ny / 4, ny / 2);
+    if (i == 3 * nx / 4) v += gbc_adj_line(B, d.Ro, j - ny / 4, ny / 2);
+    a[blockIdx.y * sa + t] = v;
+}
+
+void launch_get_bc_adj(const Dom &d, const double *bc, long sbc, double *a, long sa, int nbatch,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_get_bc_adj, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, bc, sbc,
+                       a, sa);
+}
+
+// out = base + coarsifyᵀ(y): fine vertex i receives the coarse centre values whose stencil
+// (weights 1, ½, ¼ as products of 1 and ½ per dimension) covers it.  base may be null.
+__device__ inline int coarsify_adj_taps(int i, int n, int *a, double *w) {
+    int c = 0;
+    if ((i & 1) == 0) {
+        const int k = (i - 2) / 2;
+        if (k >= 0 && k < n) { a[c] = k; w[c++] = 1.0; }
+    } else {
+        const int k0 = (i - 3) / 2, k1 = (i - 1) / 2;
+        if (i >= 3 && k0 < n) { a[c] = k0; w[c++] = 0.5; }
+        if (k1 >= 0 && k1 < n) { a[c] = k1; w[c++] = 0.5; }
+    }
+    return c;
+}
+
+__global__ void k_coarsify_adj(Dom d, const double *__restrict__ y, long sy,
+                               const double *__restrict__ base, long sbase,
+                               double *__restrict__ out, long sout) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.ng) return;
+    const double *Y = y + blockIdx.y * sy;
+    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    int ai[2], bj[2];
+    double wi[2], wj[2];
+    const int ni = coarsify_adj_taps(i, d.nx / 2 - 1, ai, wi);
+    const int nj = coarsify_adj_taps(j, d.ny / 2 - 1, bj, wj);
+    double v = 0.0;
+    for (int q = 0; q < nj; ++q)
+        for (int p = 0; p < ni; ++p)
+            v += (wi[p] * wj[q]) * GI(Y, d.nx / 4 + 1 + ai[p], d.ny / 4 + 1 + bj[q]);
+    out[blockIdx.y * sout + t] = (base ? base[blockIdx.y * sbase + t] : 0.0) + v;
+}
+
+void launch_coarsify_adj(const Dom &d, const double *y, long sy, const double *base, long sbase,
+                         double *out, long sout, int nbatch, hipStream_t s) {
+    hipLaunchKernelGGL(k_coarsify_adj, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, y, sy,
+                       base, sbase, out, sout);
+}
+
+// out = (a − b) on the flux faces with 1-based indices [i0, i1] × [j0, j1] (clipped to each
+// component's extent), zero elsewhere; b may be null
+__global__ void k_box_diff(Dom d, const double *__restrict__ a, const double *__restrict__ b,
+                           long sq, double *__restrict__ out, int i0, int i1, int j0, int j1) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= d.nq) return;
+    int i, j;
+    if (t < d.nu) {
+        i = (int)(t % (d.nx + 1)) + 1;
+        j = (int)(t / (d.nx + 1)) + 1;
+    } else {
+        i = (int)((t - d.nu) % d.nx) + 1;
+        j = (int)((t - d.nu) / d.nx) + 1;
+    }
+    const long o = blockIdx.y * sq + t;
+    const bool in = i >= i0 && i <= i1 && j >= j0 && j <= j1;
+    out[o] = in ? a[o] - (b ? b[o] : 0.0) : 0.0;
+}
+
+void launch_box_diff(const Dom &d, const double *a, const double *b, long sq, double *out,
+                     int nbatch, int i0, int i1, int j0, int j1, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_diff, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, d, a, b, sq,
+                       out, i0, i1, j0, j1);
+}
+
+// p[b·stride + i] for i < n: a uniform value in [−1, 1) from a counter-based hash of (seed, b, i)
+// (splitmix64), on the vertex box [xa, xb] × [ya, yb] (0-based, m-wide rows), zero elsewhere
+__global__ void k_fill_random_box(double *p, long stride, int m, int p_, unsigned long long seed,
+                                  int xa, int xb, int ya, int yb) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)m * p_) return;
+    const int i = (int)(t % m), j = (int)(t / m);
+    double v = 0.0;
+    if (i >= xa && i <= xb && j >= ya && j <= yb) {
+        unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (1ull + t + (unsigned long long)blockIdx.y * (unsigned long long)m * p_);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        v = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    p[blockIdx.y * stride + t] = v;
+}
+
+void launch_fill_random_box(const Dom &d, double *p, long stride, int count,
+                            unsigned long long seed, int xa, int xb, int ya, int yb,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_random_box, dim3(nblk(d.ng, 256), count), dim3(256), 0, s, p,
+                       stride, d.m, d.p, seed, xa, xb, ya, yb);
+}
+
 }  // namespace ibh

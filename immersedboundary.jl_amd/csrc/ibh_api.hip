@@ -539,11 +539,47 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
 // vort2flux; x = E q₁.  z, x: column stride sz, sx.  The transforms run in FP64 (the
 // Schur-complement side of the step) unless `prec` asks otherwise, and Vort2Flux runs on every
 // level unless `nlev` asks for fewer (both for the preconditioner's columns).
+// B·z with the coarse levels' part at low rank (valid factorization, FP64, every level):
+// x = E·curl ψ₁ + EQ·(EVᵀ z), with ψ₁ = Δinv Ainv₁ C' Eᵀ z on level 1 alone (zero boundary
+// values).  ψ₁ needs only the forward transform of C'Eᵀz, scaled by κ·sAinv₁ ⊙ sD (S·S =
+// (n/2)·I per dimension), and its inverse transform on the body box: four narrowed
+// contractions where the full product makes about fourteen and forty small launches.
+constexpr int LR_MAXNC = 64;
+void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
+                const int *skip) {
+    const Dom &d = c->d;
+    const int L = c->L, nf = 2 * c->nb;
+    const long ng = d.ng;
+    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
+    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
+    launch_lr_dot(nf, c->lr_k, c->lrEV, z, sz, c->lrW, nc, skip, c->s);
+    LapInvPlan P;
+    lapinv_plan(c, c->g1B, ng, c->psiB, L * ng, c->sAD, 0, nc, EPI_STORE, nullptr, skip, &P);
+    P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
+    P.g[0].khi = c->bxb / 2 + 1;
+    P.g[1].klo = c->bya / 2;
+    P.g[1].khi = c->byb / 2 + 1;
+    rows_for_pairs(P.g[0], P.g[1]);
+    pair_range(c->bxa, c->bxb, d.nx, &P.g[2].olo[0], &P.g[2].ohi[0]);  // ψ₁ on the box only
+    P.g[3].olo[0] = c->bxa;
+    P.g[3].ohi[0] = c->bxb + 1;
+    pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
+    for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
+    launch_curl_box(d, c->psiB, L * ng, c->zerobc, 0, c->qLB, d.nq, nc, skip, c->s, c->bxa + 1,
+                    c->bxb + 2, c->bya + 1, c->byb + 2);
+    launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
+    launch_lr_axpy(nf, c->lr_k, c->lrEQ, c->lrW, x, sx, nc, skip, c->s);
+}
+
 void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,
              int prec = IBH_FP64, int nlev = -1) {
     const Dom &d = c->d;
     const int L = c->L, Lv = nlev > 0 ? std::min(nlev, L) : L;
     const long ng = d.ng;
+    if (c->lr_valid && !c->lr_bypass && prec == IBH_FP64 && nlev <= 0 && nc <= LR_MAXNC) {
+        B_times_lr(c, z, sz, x, sx, nc, skip);
+        return;
+    }
     launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
     launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
     // Γ's coarse levels (zero here) need no clearing: their buffer is zeroed when allocated and
@@ -784,6 +820,188 @@ int release_B_work(ibh_ctx *c) {
     return IBH_OK;
 }
 
+// ---------------------------------------------------------------- low-rank coarse part
+// In a B-product (B_Times, cnab/ops.jl:125-143) the coarse levels of Vort2Flux reach the body
+// points only through level 1's boundary values: q₁ = curl(Δinv(Γ₁ + edges(bc₁)), bc₁) with
+// bc₁ = G(Γ₁) from the coarsify!/Poisson/get_bc! chain (fluid-ops.jl:448-469).  So B = B₁ + B_c
+// with B₁ the one-level product and B_c z = E·T·(C'Eᵀz), where T maps a vertex field on the
+// body box to curl Δinv edges G Ainv₁ of it on the box.  T passes through the 2(nx+ny)+4
+// boundary values and is smooth there, so its numerical rank is small (~60 at 1e-12 for the
+// cylinder + plate on 256², 5 levels, against its 30000 box unknowns).  A randomized range
+// finder (Y = T·Ω, Q = orth(Y), Z = Tᵀ·Q) gives T ≈ Q·Zᵀ; B_c z = (E Q)·((E V)ᵀ z) with
+// V = curl Z, since Zᵀ C' = (C Z)ᵀ.  The factorization depends on the grid and the body box
+// only; EQ and EV follow the body points.  It is accepted only if B-products through it match
+// the full ones to LR_TOL (relative, max norm) on random forces, else the rank doubles.
+constexpr double LR_TOL = 1e-12;
+constexpr int LR_K0 = 64, LR_KMAX = 256, LR_NC = 16;
+
+__global__ void k_scale_product(double *out, const double *a, const double *b, double kappa,
+                                long n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < n) out[t] = kappa * a[t] * b[t];
+}
+
+// Y[:, j] = T Γin[:, j] on the body box (zero elsewhere), nc columns (≤ ncB)
+void lr_T(ibh_ctx *c, const double *Gin, double *Y, int nc) {
+    const Dom &d = c->d;
+    const int L = c->L;
+    const long ng = d.ng;
+    lapinv(c, Gin, ng, c->gB, L * ng, c->sAinv, 0, nc, EPI_STORE, nullptr, nullptr, IBH_FP64);
+    // q₁ of the full multilevel solve and of level 1 alone, both formed on the box
+    vort2flux(c, c->gB, c->psiB, c->qLB, c->bcB, nc, false, nullptr, IBH_FP64, 0, -1, true,
+              nullptr, true);
+    vort2flux(c, c->gB, c->psiB, c->qB, c->bcB, nc, false, nullptr, IBH_FP64, 0, 1, true,
+              nullptr, true);
+    launch_box_diff(d, c->qLB, c->qB, d.nq, Y, nc, c->bxa + 1, c->bxb + 2, c->bya + 1,
+                    c->byb + 2, c->s);
+}
+
+// Zout[:, j] = Tᵀ W[:, j] (W: flux fields zero off the box): Ainv₁ Gᵀ edgesᵀ Δinv C'.  Gᵀ runs
+// the coarse chain backwards: aℓ = get_bc!ᵀ βℓ₋₁, yℓ = Δinv aℓ, βℓ = edgesᵀ yℓ, and
+// Gᵀβ₁ = coarsifyᵀ(y₂ + coarsifyᵀ(y₃ + …)).  A holds nc·L·nΓ doubles of work.
+void lr_Tt(ibh_ctx *c, const double *W, double *Zout, int nc, double *A) {
+    const Dom &d = c->d;
+    const int L = c->L;
+    const long ng = d.ng, sL = L * ng, sB = (long)L * d.nbc;
+    launch_rot(d, W, d.nq, c->g1B, ng, nc, nullptr, c->s);
+    lapinv(c, c->g1B, ng, c->psiB, sL, c->sD, 0, nc, EPI_STORE, nullptr, nullptr, IBH_FP64);
+    launch_edges_adj(d, c->psiB, sL, c->bcB, sB, nc, c->s);
+    for (int lev = 1; lev < L; ++lev) {
+        launch_get_bc_adj(d, c->bcB + (lev - 1) * d.nbc, sB, A + lev * ng, sL, nc, c->s);
+        lapinv(c, A + lev * ng, sL, c->psiB + lev * ng, sL, c->sD, 0, nc, EPI_STORE, nullptr,
+               nullptr, IBH_FP64);
+        if (lev < L - 1) launch_edges_adj(d, c->psiB + lev * ng, sL, c->bcB + lev * d.nbc, sB, nc, c->s);
+    }
+    const double *acc = c->psiB + (L - 1) * ng;
+    for (int lev = L - 2; lev >= 1; --lev) {
+        launch_coarsify_adj(d, acc, sL, c->psiB + lev * ng, sL, A + lev * ng, sL, nc, c->s);
+        acc = A + lev * ng;
+    }
+    launch_coarsify_adj(d, acc, sL, nullptr, 0, c->g1B, ng, nc, c->s);
+    lapinv(c, c->g1B, ng, Zout, ng, c->sAinv, 0, nc, EPI_STORE, nullptr, nullptr, IBH_FP64);
+}
+
+// EQ = E·Q, EV = E·V at the regularization's current points
+void lr_points(ibh_ctx *c) {
+    const Dom &d = c->d;
+    launch_E(d, c->reg, c->lrQ, d.nq, 0.0, 0.0, c->lrEQ, 2 * c->nb, c->lr_k, nullptr, c->s);
+    launch_E(d, c->reg, c->lrV, d.nq, 0.0, 0.0, c->lrEV, 2 * c->nb, c->lr_k, nullptr, c->s);
+}
+
+void lr_release(ibh_ctx *c) {
+    for (double **p : {&c->lrQ, &c->lrV, &c->lrEQ, &c->lrEV, &c->lrW}) {
+        dfree(c, *p);
+        *p = nullptr;
+    }
+    c->lr_valid = false;
+    c->lr_k = 0;
+}
+
+// the factorization for the current body box, with the regularization of the current points
+int lr_build(ibh_ctx *c) {
+    const Dom &d = c->d;
+    const int L = c->L, nf = 2 * c->nb;
+    const long ng = d.ng, nq = d.nq;
+    const double t0 = now_s();
+    HIPCK(hipStreamSynchronize(c->s));
+    drop_graphs(c);
+    lr_release(c);
+    if (!c->sAD) {
+        ALLOC(c->sAD, ng);
+        hipLaunchKernelGGL(k_scale_product, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
+                           c->s, c->sAD, c->sAinv, c->sD, 0.25 * d.nx * d.ny, ng);
+    }
+    if (!c->zerobc) ALLOC(c->zerobc, d.nbc);
+    int rc = ensure_B_work(c, LR_NC);
+    if (rc) return rc;
+    // validation forces (host random, fixed seed) and their full products
+    std::vector<double> zh(3 * (size_t)nf), xf(3 * (size_t)nf), xl(3 * (size_t)nf);
+    unsigned long long st = 0x2545F4914F6CDD1Dull;
+    for (auto &v : zh) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        v = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    double *zd = nullptr, *xd = nullptr, *tau = nullptr, *Om = nullptr, *Z = nullptr, *A = nullptr;
+    ALLOC(zd, 3 * (size_t)nf);
+    ALLOC(xd, 3 * (size_t)nf);
+    HIPCK(hipMemcpyAsync(zd, zh.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, c->s));
+    c->lr_bypass = true;
+    for (int j = 0; j < 3; ++j) B_times(c, zd + (long)j * nf, 0, xd + (long)j * nf, 0, 1, nullptr);
+    c->lr_bypass = false;
+    HIPCK(hipMemcpyAsync(xf.data(), xd, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost, c->s));
+    HIPCK(hipStreamSynchronize(c->s));
+    ALLOC(A, (size_t)LR_NC * L * ng);
+    bool ok = false;
+    for (int k = LR_K0; k <= LR_KMAX && !ok; k *= 2) {
+        lr_release(c);
+        dfree(c, tau); dfree(c, Om); dfree(c, Z);
+        ALLOC(c->lrQ, (size_t)k * nq);
+        ALLOC(c->lrV, (size_t)k * nq);
+        ALLOC(c->lrEQ, (size_t)k * nf);
+        ALLOC(c->lrEV, (size_t)k * nf);
+        ALLOC(c->lrW, (size_t)k * LR_MAXNC);
+        ALLOC(tau, k);
+        ALLOC(Om, (size_t)k * ng);
+        ALLOC(Z, (size_t)k * ng);
+        c->lr_k = k;
+        launch_fill_random_box(d, Om, ng, k, 0x1234567ull + (unsigned long long)k, c->bxa, c->bxb,
+                               c->bya, c->byb, c->s);
+        for (int j = 0; j < k; j += LR_NC)
+            lr_T(c, Om + (long)j * ng, c->lrQ + (long)j * nq, std::min(LR_NC, k - j));
+        rocblas_handle rb = rb_handle(c);
+        if (!rb) return fail(c, IBH_EDEVICE, "rocBLAS handle");
+        if (rocsolver_dgeqrf(rb, (rocblas_int)nq, k, c->lrQ, (rocblas_int)nq, tau) !=
+                rocblas_status_success ||
+            rocsolver_dorgqr(rb, (rocblas_int)nq, k, k, c->lrQ, (rocblas_int)nq, tau) !=
+                rocblas_status_success)
+            return fail(c, IBH_EDEVICE, "rocSOLVER QR of the coarse-level range");
+        // the reflectors may leave Q nonzero off the box, where T's range is zero
+        launch_box_diff(d, c->lrQ, nullptr, nq, c->lrQ, k, c->bxa + 1, c->bxb + 2, c->bya + 1,
+                        c->byb + 2, c->s);
+        for (int j = 0; j < k; j += LR_NC)
+            lr_Tt(c, c->lrQ + (long)j * nq, Z + (long)j * ng, std::min(LR_NC, k - j), A);
+        launch_curl_bc(d, Z, ng, c->zerobc, 0, c->lrV, nq, k, nullptr, c->s);
+        lr_points(c);
+        c->lr_valid = true;
+        for (int j = 0; j < 3; ++j)
+            B_times(c, zd + (long)j * nf, 0, xd + (long)j * nf, 0, 1, nullptr);
+        HIPCK(hipMemcpyAsync(xl.data(), xd, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost, c->s));
+        HIPCK(hipStreamSynchronize(c->s));
+        double em = 0.0, xm = 0.0;
+        for (size_t i = 0; i < xf.size(); ++i) {
+            em = std::max(em, std::fabs(xl[i] - xf[i]));
+            xm = std::max(xm, std::fabs(xf[i]));
+        }
+        c->lr_err = em / std::max(xm, 1e-300);
+        ok = c->lr_err <= LR_TOL && std::isfinite(c->lr_err);
+        char line[160];
+        std::snprintf(line, sizeof line,
+                      "low-rank coarse levels: rank %d, B-product rel. err %.2e (%s), %.3f s", k,
+                      c->lr_err, ok ? "accepted" : "rejected", now_s() - t0);
+        c->tune_log.push_back(line);
+        if (!ok) c->lr_valid = false;
+    }
+    dfree(c, tau); dfree(c, Om); dfree(c, Z); dfree(c, A); dfree(c, zd); dfree(c, xd);
+    if (!ok) {
+        lr_release(c);
+        c->lr_failed = true;
+    }
+    ++c->lr_builds;
+    HIPCK(hipStreamSynchronize(c->s));
+    c->lr_build_s += now_s() - t0;
+    return IBH_OK;
+}
+
+// A low-rank coarse part is used by CG force solves with the body box on, over every level,
+// when the box's flux faces all take curl!'s interior formula (fluid-ops.jl:138-182): the
+// faces next to the domain edge read the boundary values directly, a path T leaves out.
+bool lr_wanted(const ibh_ctx *c) {
+    const bool inner = c->bxa >= 1 && c->bxb <= c->d.m - 2 && c->bya >= 1 &&
+                       c->byb <= c->d.p - 2;
+    return c->lr_on && !c->lr_failed && c->use_cg && c->box_on && inner && c->use_eo &&
+           c->sparse_B && c->lines_B && c->L >= 2 && c->nb > 0;
+}
+
 int precompute_binv(ibh_ctx *c) {
     const int nf = 2 * c->nb;
     if (nf == 0) return IBH_OK;
@@ -881,6 +1099,7 @@ int update_box(ibh_ctx *c, const double *r) {
     if (!ok && !c->box_on) return IBH_OK;
     HIPCK(hipStreamSynchronize(c->s));
     drop_graphs(c);
+    c->lr_valid = false;  // the low-rank coarse part lives on the box: rebuilt at the next step
     c->box_on = ok;
     if (ok) {
         c->bxa = std::max(0, xa - BOX_REACH - BOX_SLACK);
@@ -968,7 +1187,10 @@ void seg_trial(ibh_ctx *c) {
     const int L = c->L;
     const long ng = d.ng;
     // update!(reg) for moving bodies (timestepping.jl:262); points came from set_body_points
-    if (c->anymove) launch_reg_update(d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+    if (c->anymove) {
+        launch_reg_update(d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+        if (c->lr_valid) lr_points(c);  // the low-rank coarse part at the new points
+    }
     // ---- GetTrialState (timestepping.jl:28-79), all levels batched
     if (L > 1) {
         launch_get_bc(d, c->gam + ng, ng, c->bcraw, d.nbc, L - 1, nullptr, c->s);
@@ -1133,6 +1355,11 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     // moving bodies: points that were never checked (the description's body-frame points,
     // when no ibh_set_body_points came first) are rejected before anything is enqueued
     if (c->anymove && c->nb && !c->pts_ok) return check_points(c, c->hpts.data());
+    if (lr_wanted(c) && !c->lr_valid) {  // first CG step, or the body box moved
+        if (c->anymove) launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+        int rc0 = lr_build(c);
+        if (rc0) return rc0;
+    }
     c->ux = ux;
     c->uy = uy;
     launch_fill(c->uv, 1, ux, c->s);
@@ -1335,6 +1562,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         if (const char *sb = std::getenv("IBH_SPARSE_B")) c->sparse_B = std::atoi(sb) != 0;
         if (const char *bb = std::getenv("IBH_BODY_BOX")) c->box_B = std::atoi(bb) != 0;
         if (const char *sl = std::getenv("IBH_SPARSE_LINES")) c->lines_B = std::atoi(sl) != 0;
+        if (const char *lr = std::getenv("IBH_LOWRANK_B")) c->lr_on = std::atoi(lr) != 0;
     }
     if (const char *rl = std::getenv("IBH_REFINE_LEVELS"))
         c->refine_levels = std::min(std::max(1, std::atoi(rl)), c->L);
@@ -1629,6 +1857,15 @@ double ibh_time(ibh_ctx *c) { return c ? c->t : 0.0; }
 
 int ibh_cg_iterations(ibh_ctx *c) { return c ? c->cg_last : -1; }
 
+int ibh_lowrank_stats(ibh_ctx *c, double *out4) {
+    if (!c || !out4) return IBH_EINVAL;
+    out4[0] = c->lr_valid ? (double)c->lr_k : 0.0;
+    out4[1] = c->lr_err;
+    out4[2] = (double)c->lr_builds;
+    out4[3] = c->lr_build_s;
+    return IBH_OK;
+}
+
 int ibh_solver_stats(ibh_ctx *c, double *out6) {
     if (!c || !out6) return IBH_EINVAL;
     const double v[6] = {(double)c->pc_rebuilds, c->pc_time_total, c->pc_time_cols,
@@ -1766,6 +2003,7 @@ int ibh_op_reg_update(ibh_ctx *c, const double *r) {
     int rc = upload_points(c, r, nullptr);
     if (rc) return rc;
     launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+    if (c->lr_valid) lr_points(c);
     return check_flags(c);
 }
 
@@ -1789,7 +2027,9 @@ int ibh_op_ET(ibh_ctx *c, const double *fb, double *q) {
 
 int ibh_op_B(ibh_ctx *c, const double *z, double *x) {
     if (!c) return IBH_EINVAL;
-    int rc = ensure_B_work(c, 1);
+    int rc = lr_wanted(c) && !c->lr_valid ? lr_build(c) : IBH_OK;
+    if (rc) return rc;
+    rc = ensure_B_work(c, 1);
     if (rc) return rc;
     DevBuf B{c};
     const int nf = 2 * c->nb;

@@ -166,6 +166,24 @@ inline bool sine_ready(const GemmArgs &g, int prec) {
 void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
                          double dt, double Re, hipStream_t s);
 
+// adjoints of the multilevel transfers (the coarse-level part of the B-product, transposed)
+// bc[b] = edgesᵀ(f[b]): the transpose of apply_bc!'s edge addition
+void launch_edges_adj(const Dom &d, const double *f, long sf, double *bc, long sbc, int nbatch,
+                      hipStream_t s);
+// a[b] = get_bcᵀ(bc[b]) over the whole field (zero off get_bc!'s four lines)
+void launch_get_bc_adj(const Dom &d, const double *bc, long sbc, double *a, long sa, int nbatch,
+                       hipStream_t s);
+// out[b] = base[b] + coarsifyᵀ(y[b]) (base may be null)
+void launch_coarsify_adj(const Dom &d, const double *y, long sy, const double *base, long sbase,
+                         double *out, long sout, int nbatch, hipStream_t s);
+// out[b] = a[b] − b[b] on the flux faces [i0, i1] × [j0, j1] (1-based), zero elsewhere
+void launch_box_diff(const Dom &d, const double *a, const double *b, long sq, double *out,
+                     int nbatch, int i0, int i1, int j0, int j1, hipStream_t s);
+// deterministic uniform values in [−1, 1) on the vertex box [xa, xb] × [ya, yb], zero elsewhere
+void launch_fill_random_box(const Dom &d, double *p, long stride, int count,
+                            unsigned long long seed, int xa, int xb, int ya, int yb,
+                            hipStream_t s);
+
 // ---------------------------------------------------------------------------- body kernels
 // The delta support of a point in cell (bx, by) (1-based, body_idx of structure-ops.jl:48-49)
 // covers qx[i, j], qy[i, j] for i ∈ bx-6..bx+6, j ∈ by-6..by+6; it stays inside both flux
@@ -200,6 +218,11 @@ void launch_gemv(int n, const double *A, const double *x, double *y, double *ws,
                  hipStream_t s);
 long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
 void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s);  // y -= a v
+// rank-k correction of a B-product: w[b] = Vᵀ z[b] (V n×k column-major), then x[b] += U w[b]
+void launch_lr_dot(int n, int k, const double *V, const double *z, long sz, double *w, int nbatch,
+                   const int *skip, hipStream_t s);
+void launch_lr_axpy(int n, int k, const double *U, const double *w, double *x, long sx,
+                    int nbatch, const int *skip, hipStream_t s);
 void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
                    const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
                    hipStream_t s);

@@ -30,7 +30,7 @@ STATUS_NAMES = {
 EXPORTED = [
     "ibh_version", "ibh_create", "ibh_destroy", "ibh_last_error", "ibh_init_state", "ibh_setup",
     "ibh_set_body_points", "ibh_advance", "ibh_advance_n", "ibh_sync", "ibh_get_field",
-    "ibh_set_field", "ibh_dims", "ibh_time", "ibh_cg_iterations", "ibh_solver_stats",
+    "ibh_set_field", "ibh_dims", "ibh_time", "ibh_cg_iterations", "ibh_solver_stats", "ibh_lowrank_stats",
     "ibh_op_lapinv", "ibh_op_dst2",
     "ibh_op_rot", "ibh_op_curl", "ibh_op_coarsify", "ibh_op_get_bc", "ibh_op_apply_bc",
     "ibh_op_nonlinear", "ibh_op_vort2flux", "ibh_op_reg_update", "ibh_op_E", "ibh_op_ET",
@@ -103,6 +103,7 @@ def load():
     L.ibh_time.restype = C.c_double
     L.ibh_cg_iterations.argtypes = [vp]
     L.ibh_solver_stats.argtypes = [vp, dp]
+    L.ibh_lowrank_stats.argtypes = [vp, dp]
     L.ibh_op_lapinv.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
     L.ibh_op_dst2.argtypes = [vp, dp, dp]
     L.ibh_op_rot.argtypes = [vp, dp, dp]

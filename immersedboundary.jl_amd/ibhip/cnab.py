@@ -163,6 +163,14 @@ class DeviceCase:
         return {"pc_rebuilds": int(out[0]), "pc_rebuild_s": out[1], "pc_columns_s": out[2],
                 "pc_inverse_s": out[3], "iteration_s": out[4], "rebuild_s_weighed": out[5]}
 
+    def lowrank_stats(self) -> dict:
+        """The low-rank coarse-level part of the CG B-products: rank in use (0 when the full
+        product runs), its B-product error at the last build, builds and build seconds."""
+        out = np.zeros(4)
+        self._check(self.lib.ibh_lowrank_stats(self.h, dptr(out)))
+        return {"rank": int(out[0]), "rel_err": out[1], "builds": int(out[2]),
+                "build_s": out[3]}
+
     def time(self) -> float:
         return self.lib.ibh_time(self.h)
 

@@ -148,6 +148,12 @@ int ibh_cg_iterations(ibh_ctx *ctx);
  * last measured seconds per iteration, last rebuild s as the adaptive rule weighs it} */
 int ibh_solver_stats(ibh_ctx *ctx, double *out6);
 
+/* Low-rank coarse-level part of the CG B-products (no reference counterpart; a faster way to
+ * apply the same B of cnab/ops.jl:125-143): out4 = {rank in use (0: the full product runs),
+ * relative max-norm difference of B-products through it from full ones at its last build,
+ * builds so far, total build seconds}.  IBH_LOWRANK_B=0 in the environment turns it off. */
+int ibh_lowrank_stats(ibh_ctx *ctx, double *out4);
+
 /* ---- operator-level entry points (host arrays in/out; each runs the device kernels) ---- */
 /* LaplacianInv apply (fluid-ops.jl:42-56): mode 0 Δinv (Λ), 1 A_ℓ (Λexpl), 2 Ainv_ℓ (Λimpl),
  * cnab/ops.jl:14-33; lev is 0-based.  in/out: nΓ. */

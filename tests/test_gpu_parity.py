@@ -181,6 +181,75 @@ def test_b_product_body_box(ibhip, monkeypatch):
     assert rel(with_box[1], orc.B_times(z)) < 1e-11
 
 
+@pytest.mark.parametrize("nlev", [2, 4])
+def test_b_product_low_rank_coarse_part(ibhip, monkeypatch, nlev):
+    # CG force solves apply B as the one-level product plus a low-rank coarse-level part
+    # (built per body box, checked against full products at 1e-12): it matches the oracle's
+    # full multilevel B, repeats bitwise, and is rebuilt when the points move the box.  256²
+    # cells keep the box's faces off the domain edge, where the low-rank part applies.
+    prob = cylinder_case(n=256, nlev=nlev, force_solver="cg")
+    nb = prob.bodies.npoints
+    r = prob.stacked_points.T.reshape(-1).copy()
+    moved = r.copy()
+    moved[nb:] += 0.4  # 25.6 cells in y: the box is recomputed
+    z = np.random.default_rng(14).standard_normal(2 * nb)
+
+    def run():
+        c = ibhip.DeviceCase(prob)
+        out, stats = [], []
+        for pts in (r, moved):
+            c.reg_update(pts)
+            a = c.B_times(z)
+            assert np.array_equal(a, c.B_times(z))
+            out.append(a)
+            stats.append(c.lowrank_stats())
+        c.close()
+        return out, stats
+
+    lr, stats = run()
+    print(f"low-rank coarse part, {nlev} levels: {stats}")
+    assert all(s["rank"] > 0 and s["rel_err"] <= 1e-12 for s in stats)
+    assert stats[1]["builds"] == 2  # the move changed the box
+    monkeypatch.setenv("IBH_LOWRANK_B", "0")
+    full, stats0 = run()
+    assert stats0[0]["rank"] == 0
+    orc = ibref.OracleRun(prob, setup=False)
+    for a, b, pts in zip(lr, full, (r, moved)):
+        orc.reg_update(pts)
+        want = orc.B_times(z)
+        print(f"  vs full {rel(a, b):.1e}, vs oracle {rel(a, want):.1e} (full: {rel(b, want):.1e})")
+        assert rel(a, b) < 1e-12 and rel(a, want) < 1e-11
+
+
+def test_step_low_rank_coarse_part_matches_full_products(ibhip, monkeypatch):
+    # the cylinder + pitching plate on 256² cells, 3 levels, PCG to 1e-14: steps with the
+    # low-rank coarse part in every B-product agree with steps through full products
+    import ibhip as m
+
+    prob = cylinder_plate_case(n=256, nlev=3, cg_reltol=1e-14, force_solver="pcg")
+
+    def run():
+        st = m.initstate(prob)
+        sv = m.initsolver(prob, st)
+        for k in range(4):
+            m.advance_(st, sv, k * prob.scheme.dt)
+        out = [st.case.gamma(), st.case.psi(), st.case.q(), st.case.Fb(), st.case.CD(),
+               st.case.CL()]
+        stats = st.case.lowrank_stats()
+        st.case.close()
+        return out, stats
+
+    lr, stats = run()
+    monkeypatch.setenv("IBH_LOWRANK_B", "0")
+    full, stats0 = run()
+    print(f"low-rank coarse part in the step: {stats}")
+    assert stats["rank"] > 0 and stats0["rank"] == 0
+    errs = [rel(a, b) for a, b in zip(lr[:5], full[:5])]
+    errs.append(np.abs(lr[5] - full[5]).max() / np.abs(full[4]).max())
+    print("  Γ ψ q Fb CD CL rel diff: " + " ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) < 1e-10
+
+
 def run_pair(ibhip, prob, nsteps, t0=0.0):
     import ibhip as m
 
@@ -265,6 +334,7 @@ def test_step_parity_translating_plate_moves_body_box(ibhip, monkeypatch):
                    BodyGroup(RigidBody(frame, xb, ds)),
                    SchemeCNAB(0.02, cg_reltol=1e-15, force_solver="pcg", pc_refresh=4))
     nsteps = 40  # 1.56 units = 19.5 cells of travel
+    monkeypatch.setenv("IBH_LOWRANK_B", "0")  # the full B-products, compared bit for bit
     state, orc, hist = run_pair(ibhip, prob, nsteps)
     assert np.all(np.abs(state.case.CD()) < 100.0)  # the flow stays bounded
     assert_state_close(state.case, orc, 1e-10)
