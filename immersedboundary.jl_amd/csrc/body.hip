@@ -105,27 +105,40 @@ __device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__r
     return wave_sum(acc);
 }
 
+// With lrk > 0 the wave also adds the low-rank term Σ_j U[t, j]·w[b, j] (U 2nb × lrk column-
+// major, w lrk per batch entry), summed over the lanes in a fixed order.
 __global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
                     double *__restrict__ fb, long sfb, const int *skip, const double *uv,
-                    double uvscale) {
+                    double uvscale, const double *__restrict__ U, const double *__restrict__ w,
+                    int lrk, int t0, int tn) {
     if (skip && *skip) return;
     if (uv) {  // q0 = freestream·h from the device parameter block
         u0 = uv[0] * uvscale;
         v0 = uv[1] * uvscale;
     }
-    const int t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (t >= 2 * r.nb) return;
+    const int tl = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (tl >= tn) return;
+    const int t = t0 + tl;
     const int b = blockIdx.y;
-    const double v = E_wave(d, r, q + b * sq, u0, v0, t, lane);
+    double v = E_wave(d, r, q + b * sq, u0, v0, t, lane);
+    if (lrk > 0) {
+        const long n2 = 2L * r.nb;
+        double s = 0.0;
+        for (int j = lane; j < lrk; j += 64) s += U[t + n2 * j] * w[(long)b * lrk + j];
+        s = wave_sum(s);
+        v += s;
+    }
     if (lane == 0) fb[b * sfb + t] = v;
 }
 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s, const double *uv,
-              double uvscale) {
+              double uvscale, const double *U, const double *w, int lrk, int t0, int tn) {
     if (r.nb == 0) return;
-    hipLaunchKernelGGL(k_E, dim3(nblk(2L * r.nb, 4), nbatch), dim3(256), 0, s, d, r, q, sq, u0,
-                       v0, fb, sfb, skip, uv, uvscale);
+    if (tn < 0) tn = 2 * r.nb - t0;
+    if (tn <= 0) return;
+    hipLaunchKernelGGL(k_E, dim3(nblk(tn, 4), nbatch), dim3(256), 0, s, d, r, q, sq, u0, v0, fb,
+                       sfb, skip, uv, uvscale, U, w, lrk, t0, tn);
 }
 
 // Eᵀ (Reg), structure-ops.jl:63-86: q .= 0 then scatter-add the 13×13 stencils of every point.
@@ -208,14 +221,31 @@ __global__ void k_et_fix(double *__restrict__ q, long n, long sq, const double *
     et_fix_at(q + blockIdx.y * sq, scale[blockIdx.y], t);
 }
 
-// zeroing of the fixed-point image (blocks before the last) and the column's scale (the last)
+// zeroing of the fixed-point image (the first ceil(n/256) blocks), the column's scale (the
+// next one) and, with lrk > 0, the low-rank dots w[b, j] = V[:, j]·fb[b] (one block per j;
+// V n2 × lrk column-major)
 __global__ __launch_bounds__(256) void k_et_zero_scale(double *__restrict__ q, long n, long sq,
                                                        const double *__restrict__ fb, long sfb,
-                                                       int n2, double *scale, const int *skip) {
+                                                       int n2, double *scale, const int *skip,
+                                                       const double *__restrict__ V,
+                                                       double *__restrict__ w, int lrk) {
     if (skip && *skip) return;
     const int b = blockIdx.y;
-    if (blockIdx.x == gridDim.x - 1) {
+    const long nz = nblk(n, 256);
+    if (blockIdx.x == nz) {
         et_scale_body(fb + b * sfb, n2, scale + b);
+        return;
+    }
+    if (blockIdx.x > nz) {
+        __shared__ double sh[4];
+        const int j = blockIdx.x - (int)nz - 1;
+        const double *Vj = V + (long)n2 * j, *f = fb + b * sfb;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < n2; i += 256) s += Vj[i] * f[i];
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) w[(long)b * lrk + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
         return;
     }
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -223,9 +253,10 @@ __global__ __launch_bounds__(256) void k_et_zero_scale(double *__restrict__ q, l
 }
 
 void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
-                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(d.nq, 256) + 1, nbatch), dim3(256), 0, s, img,
-                       d.nq, sq, fb, sfb, 2 * r.nb, scale, skip);
+                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
+                     const double *V, double *w, int lrk) {
+    hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(d.nq, 256) + 1 + lrk, nbatch), dim3(256), 0, s,
+                       img, d.nq, sq, fb, sfb, 2 * r.nb, scale, skip, V, w, lrk);
     if (r.nb == 0) return;
     long n = 2L * r.nb * NS * NS;
     hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, img, sq,
@@ -307,50 +338,6 @@ __global__ void k_axpy_sub(int n, double *y, const double *v, double a) {
 void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_axpy_sub, dim3(nblk(n, 256)), dim3(256), 0, s, n, y, v, a);
-}
-
-// Rank-k coarse-level correction of a B-product (see ibh_api.hip): w = Vᵀ z with one
-// 256-thread workgroup per entry of w (fixed-order sums), then x += U w with one thread per
-// entry of x summing over the k columns in order.
-__global__ __launch_bounds__(256) void k_lr_dot(int n, int k, const double *__restrict__ V,
-                                                const double *__restrict__ z, long sz,
-                                                double *__restrict__ w, const int *skip) {
-    if (skip && *skip) return;
-    __shared__ double sh[4];
-    const int j = blockIdx.x, b = blockIdx.y;
-    const double *Vj = V + (long)n * j, *zb = z + b * sz;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += Vj[i] * zb[i];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) w[(long)b * k + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-}
-
-__global__ void k_lr_axpy(int n, int k, const double *__restrict__ U,
-                          const double *__restrict__ w, double *__restrict__ x, long sx,
-                          const int *skip) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int b = blockIdx.y;
-    const double *wb = w + (long)b * k;
-    double s = 0.0;
-    for (int j = 0; j < k; ++j) s += U[i + (long)n * j] * wb[j];
-    x[b * sx + i] += s;
-}
-
-void launch_lr_dot(int n, int k, const double *V, const double *z, long sz, double *w, int nbatch,
-                   const int *skip, hipStream_t s) {
-    if (n == 0 || k == 0) return;
-    hipLaunchKernelGGL(k_lr_dot, dim3(k, nbatch), dim3(256), 0, s, n, k, V, z, sz, w, skip);
-}
-
-void launch_lr_axpy(int n, int k, const double *U, const double *w, double *x, long sx,
-                    int nbatch, const int *skip, hipStream_t s) {
-    if (n == 0 || k == 0) return;
-    hipLaunchKernelGGL(k_lr_axpy, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, n, k, U, w, x, sx,
-                       skip);
 }
 
 // ---------------------------------------------------------------- block reductions (FP64)

@@ -548,11 +548,11 @@ constexpr int LR_MAXNC = 64;
 void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
                 const int *skip) {
     const Dom &d = c->d;
-    const int L = c->L, nf = 2 * c->nb;
+    const int L = c->L;
     const long ng = d.ng;
-    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
+    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, c->lrEV,
+                    c->lrW, c->lr_k);  // w = EVᵀ z in the same launch
     launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
-    launch_lr_dot(nf, c->lr_k, c->lrEV, z, sz, c->lrW, nc, skip, c->s);
     LapInvPlan P;
     lapinv_plan(c, c->g1B, ng, c->psiB, L * ng, c->sAD, 0, nc, EPI_STORE, nullptr, skip, &P);
     P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
@@ -567,8 +567,8 @@ void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc
     for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
     launch_curl_box(d, c->psiB, L * ng, c->zerobc, 0, c->qLB, d.nq, nc, skip, c->s, c->bxa + 1,
                     c->bxb + 2, c->bya + 1, c->byb + 2);
-    launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s);
-    launch_lr_axpy(nf, c->lr_k, c->lrEQ, c->lrW, x, sx, nc, skip, c->s);
+    launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, c->lrEQ,
+             c->lrW, c->lr_k);  // + EQ·w in E's wave sums
 }
 
 void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,
@@ -881,11 +881,22 @@ void lr_Tt(ibh_ctx *c, const double *W, double *Zout, int nc, double *A) {
     lapinv(c, c->g1B, ng, Zout, ng, c->sAinv, 0, nc, EPI_STORE, nullptr, nullptr, IBH_FP64);
 }
 
-// EQ = E·Q, EV = E·V at the regularization's current points
-void lr_points(ibh_ctx *c) {
+// EQ = E·Q, EV = E·V at the regularization's current points; with moving_only the rows of
+// the moving bodies' unknowns (those of fixed bodies keep the values of the last full pass)
+void lr_points(ibh_ctx *c, bool moving_only = false) {
     const Dom &d = c->d;
-    launch_E(d, c->reg, c->lrQ, d.nq, 0.0, 0.0, c->lrEQ, 2 * c->nb, c->lr_k, nullptr, c->s);
-    launch_E(d, c->reg, c->lrV, d.nq, 0.0, 0.0, c->lrEV, 2 * c->nb, c->lr_k, nullptr, c->s);
+    const int nf = 2 * c->nb;
+    auto rows = [&](int t0, int tn) {
+        launch_E(d, c->reg, c->lrQ, d.nq, 0.0, 0.0, c->lrEQ, nf, c->lr_k, nullptr, c->s, nullptr,
+                 1.0, nullptr, nullptr, 0, t0, tn);
+        launch_E(d, c->reg, c->lrV, d.nq, 0.0, 0.0, c->lrEV, nf, c->lr_k, nullptr, c->s, nullptr,
+                 1.0, nullptr, nullptr, 0, t0, tn);
+    };
+    if (!moving_only) {
+        rows(0, nf);
+        return;
+    }
+    for (const auto &run : c->Jrange) rows(run.first, run.second);
 }
 
 void lr_release(ibh_ctx *c) {
@@ -1189,7 +1200,7 @@ void seg_trial(ibh_ctx *c) {
     // update!(reg) for moving bodies (timestepping.jl:262); points came from set_body_points
     if (c->anymove) {
         launch_reg_update(d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
-        if (c->lr_valid) lr_points(c);  // the low-rank coarse part at the new points
+        if (c->lr_valid) lr_points(c, true);  // the low-rank coarse part at the new points
     }
     // ---- GetTrialState (timestepping.jl:28-79), all levels batched
     if (L > 1) {

@@ -204,25 +204,28 @@ struct RegDev {
 void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double h, double x0,
                        double y0, hipStream_t s);
 // fb = E·(q + q0) where q0 = (u0, v0) constant (pass 0, 0 for plain E)
+// With lrk > 0 also fb[b] += U·w[b] (U 2nb × lrk, w lrk per batch entry): the low-rank
+// coarse-level part of a B-product, summed inside E's wave reduction.  Only the entries
+// [t0, t0 + tn) of each fb[b] are formed (tn < 0: through the end).
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s,
-              const double *uv = nullptr, double uvscale = 1.0);
+              const double *uv = nullptr, double uvscale = 1.0, const double *U = nullptr,
+              const double *w = nullptr, int lrk = 0, int t0 = 0, int tn = -1);
 // q = Eᵀ fb (zero, scale, fixed-point scatter, conversion); scale: nbatch doubles of work
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, double *scale, hipStream_t s);
 // Eᵀ up to its fixed-point image (zeroing and the scales in one launch, then the scatter);
-// launch_rot_image reads that image directly, converting as launch_ET's last pass would
+// launch_rot_image reads that image directly, converting as launch_ET's last pass would.
+// With lrk > 0 the first launch also forms w[b] = Vᵀ fb[b] (V 2nb × lrk), the low-rank
+// coarse-level part's input
 void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
-                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s);
+                     long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
+                     const double *V = nullptr, double *w = nullptr, int lrk = 0);
 void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
                  hipStream_t s);
 long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
 void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s);  // y -= a v
-// rank-k correction of a B-product: w[b] = Vᵀ z[b] (V n×k column-major), then x[b] += U w[b]
-void launch_lr_dot(int n, int k, const double *V, const double *z, long sz, double *w, int nbatch,
-                   const int *skip, hipStream_t s);
-void launch_lr_axpy(int n, int k, const double *U, const double *w, double *x, long sx,
-                    int nbatch, const int *skip, hipStream_t s);
+
 void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
                    const double *ds, double h, double dt, double *CD, double *CL, double *fbs,
                    hipStream_t s);
