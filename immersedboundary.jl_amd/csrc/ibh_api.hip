@@ -544,7 +544,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
 // values).  ψ₁ needs only the forward transform of C'Eᵀz, scaled by κ·sAinv₁ ⊙ sD (S·S =
 // (n/2)·I per dimension), and its inverse transform on the body box: four narrowed
 // contractions where the full product makes about fourteen and forty small launches.
-constexpr int LR_MAXNC = 64;
+constexpr int LR_MAXNC = 128;
 void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
                 const int *skip) {
     const Dom &d = c->d;
