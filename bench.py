@@ -46,14 +46,16 @@ def measured_traffic(workload, precision):
         return None
 
 
-def cpu_baseline(make_problem, n_ref_iters, cap=8):
+def cpu_baseline(make_problem, n_ref_iters, cap=8, threads=None):
     """Time the oracle on one step of the workload with cg! capped at `cap` iterations and on
     two lone B-products; extrapolate the step to `n_ref_iters` iterations of the reference's
-    unpreconditioned cg!."""
+    unpreconditioned cg! (the reference's force solve for moving bodies: it has no
+    preconditioner).  `threads`: OpenMP threads of the oracle (default OMP_NUM_THREADS)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ibref
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ibref.lib().ibr_set_threads(threads)
     prob = make_problem(force_solver="cg", cg_maxiter=cap)
     o = ibref.OracleRun(prob)
@@ -349,6 +351,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched (64-case) FP32/FP64 comparison of the default run")
     ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--steady-steps", type=int, default=300,
+                    help="moving bodies: steps of the same run after the timed window, timed "
+                         "with the preconditioner rebuilds they need (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one elementwise copy of 2^25 doubles first (256 MiB read + 256 MiB "
                          "written) so that profiler byte counters can be scaled")
@@ -427,6 +432,28 @@ def main():
     ev_ms, ev_launches, _ = case.profile_read(kind, events=True)
     case.profile(False)
     log(f"bench: profiled pass {args.profile_steps} steps")
+    steady = None
+    if prob.any_moving and args.steady_steps > 0:
+        # the same run continued for steady_steps more steps: the preconditioner is rebuilt
+        # as often as the bodies' motion asks, which a short timed window may not show
+        s0 = case.solver_stats()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for s in range(args.steady_steps):
+            ibhip.advance_(state, solver, k * dt)
+            k += 1
+            if (s + 1) % 100 == 0:
+                log(f"bench: steady state {s + 1}/{args.steady_steps} steps")
+        case.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        s1 = case.solver_stats()
+        steady = {"steps": args.steady_steps, "ms_per_step": 1e3 * el / args.steady_steps,
+                  "cell_steps_per_s": cells * args.steady_steps / el,
+                  "pc_rebuilds": s1["pc_rebuilds"] - s0["pc_rebuilds"],
+                  "pc_rebuild_ms_per_step": 1e3 * (s1["pc_rebuild_s"] - s0["pc_rebuild_s"])
+                  / args.steady_steps}
+        log(f"bench: steady state {steady}")
     accuracy = None
     if args.precision != "fp64":
         accuracy = lift_drag_error(ibhip, make, args.warmup + args.steps, dt)
@@ -467,7 +494,9 @@ def main():
                    / args.steps,
                    "pc_columns_ms_total": 1e3 * (st1["pc_columns_s"] - st0["pc_columns_s"]),
                    "pc_inverse_ms_total": 1e3 * (st1["pc_inverse_s"] - st0["pc_inverse_s"]),
-                   "iteration_ms": 1e3 * st1["iteration_s"]})
+                   "iteration_ms": 1e3 * st1["iteration_s"],
+                   "low_rank_coarse_part": case.lowrank_stats(),
+                   "steady_state": steady})
     out = {
         "metric": "cell-steps/s: FP64 vs mixed-precision batched, at ≤1e-6 rel lift/drag err",
         "value": value,
@@ -503,6 +532,8 @@ def main():
         rs.case.close()
         log(f"bench: reference cg! iterations on step 1: {n_ref}")
         out["cpu_baseline"] = cpu_baseline(make, n_ref)
+        one = cpu_baseline(make, n_ref, threads=1)
+        out["cpu_baseline"]["one_core"] = {"value": one["value"], "sample": one["sample"]}
         log("bench: cpu baseline done")
     if not args.no_batch and args.workload == "cylinder_plate" and args.precision == "fp64":
         out["config"]["mixed_precision_batched"] = batched_summary(torch, ibhip, args, value)
