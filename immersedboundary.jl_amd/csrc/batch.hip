@@ -30,7 +30,7 @@ namespace {
 enum Op {
     OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
     OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS, OP_RESID,
-    OP_ETZS, OP_ROTI
+    OP_ETZS, OP_ROTI, OP_EDGE, OP_SPEC, OP_LNP, OP_LNC, OP_LNO, OP_AXPY
 };
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -97,33 +97,109 @@ void add(std::vector<Stage> &v, int op, const std::vector<A> &a, const std::vect
     v.push_back(std::move(S));
 }
 
-// the four contractions of one LaplacianInv apply per case, as four grouped stages; the
-// second left/right pair reuses the first pair's tuned tile variant (same shapes)
-void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int prec) {
+// One grouped contraction stage over the listed split contractions (all of one kind and
+// epilogue).  `tune_ref` names an earlier stage of the same shapes whose tile variant it reuses.
+void add_gemm(std::vector<Stage> &v, const std::vector<GemmArgs> &gs, int kind, int epi, int prec,
+              int tune_ref = -1) {
+    if (gs.empty()) return;
+    Stage S;
+    S.op = OP_GEMM;
+    S.prec = prec;
+    S.kind = kind;
+    S.side = kind - 1;
+    S.epi = epi;
+    S.n = (int)gs.size();
+    for (const GemmArgs &g : gs) {
+        if (!sine_ready(g, prec)) {
+            std::fprintf(stderr, "libibhip: grouped contraction without its sine parts\n");
+            std::abort();
+        }
+        S.gargs.push_back(g);
+        S.flops += gemm_flops(g, kind);
+    }
+    S.tune_ref = tune_ref;
+    v.push_back(std::move(S));
+}
+
+// the four contractions of one LaplacianInv apply per case, as four grouped stages; with
+// `share` the second left/right pair reuses the first pair's tuned tile variant (same shapes).
+// The last contraction always takes the second one's variant: its epilogue may read its own
+// output (EPI_SUBFROM), which the tuner cannot re-run.
+void add_lapinv(std::vector<Stage> &v, const std::vector<LapInvPlan> &plans, int prec,
+                bool share = true) {
     if (plans.empty()) return;
     const int base = (int)v.size();
     for (int k = 0; k < 4; ++k) {
-        Stage S;
-        S.op = OP_GEMM;
-        S.prec = prec;
-        S.kind = plans[0].kind[k];
-        S.side = S.kind - 1;
-        S.epi = plans[0].epi[k];
-        S.n = (int)plans.size();
-        for (const LapInvPlan &P : plans) {
-            const GemmArgs &g = P.g[k];
-            if (!sine_ready(g, prec)) {
-                std::fprintf(stderr, "libibhip: grouped contraction without its sine parts\n");
-                std::abort();
-            }
-            S.gargs.push_back(g);
-            const double rows = S.kind == 1 ? g.N : g.M;
-            S.flops += 4.0 * g.h * (double)g.h * rows * g.batch;
-        }
-        S.tune_ref = k >= 2 ? base + k - 2 : -1;
-        v.push_back(std::move(S));
+        std::vector<GemmArgs> gs;
+        for (const LapInvPlan &P : plans) gs.push_back(P.g[k]);
+        add_gemm(v, gs, plans[0].kind[k], plans[0].epi[k], prec,
+                 k == 3 || (share && k == 2) ? base + k - 2 : -1);
     }
 }
+
+// One Poisson solve ψ_ℓ = Δinv(rhs) per listed case (each on its own level), in one of three
+// forms that agree up to roundoff:
+// * SV_FULL: the four contractions of the LaplacianInv apply;
+// * SV_LINES: the forward pair, then ψ only on the rows and columns get_bc! reads (a coarse
+//   level whose ψ feeds nothing but the next finer level's boundary values);
+// * SV_SPEC: level 1 from the kept scaled spectrum F of its Γ = S F S (S·S = (n/2)·I per
+//   dimension): the inverse pair of (nx·ny/4)·F ⊙ sD plus the boundary terms in the sine basis.
+enum { SV_FULL, SV_LINES, SV_SPEC };
+struct SolveItem {
+    int i, mode;
+    const double *in;    // FULL, LINES: right-hand side with its boundary values applied
+    double *psi;         // the level's ψ
+    const double *bc;    // SPEC: the level's boundary values (null: zero)
+    const double *spec;  // SPEC: the kept spectrum
+};
+
+void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem> &items) {
+    std::vector<EdgeA> ed;
+    std::vector<SpecA> sp;
+    std::vector<LinesA> ln;
+    std::vector<int> edb, spb, lb[3];
+    std::vector<GemmArgs> left1, right1, left2, right2;
+    for (const SolveItem &it : items) {
+        ibh_ctx *c = b->cs[it.i];
+        const Dom &d = c->d;
+        LapInvPlan P;
+        lapinv_plan(c, it.in, d.ng, it.psi, d.ng, c->sD, 0, 1, EPI_STORE, nullptr, nullptr, &P);
+        if (it.mode == SV_SPEC) {
+            if (it.bc) {
+                ed.push_back({d, c->Sm, c->Sp, it.bc, c->edgeS});
+                edb.push_back(d.m + d.p);
+            }
+            sp.push_back({d, it.spec, c->sD, 0.25 * d.nx * d.ny, c->Sm, c->Sp,
+                          it.bc ? c->edgeS : nullptr, c->T2});
+            spb.push_back(nblk(d.ng, GRP_BS));
+            left1.push_back(P.g[2]);  // reads T2
+            right2.push_back(P.g[3]);
+            continue;
+        }
+        left1.push_back(P.g[0]);
+        right1.push_back(P.g[1]);
+        if (it.mode == SV_LINES) {
+            ln.push_back({d, c->T2, c->Sm, c->Sp, nullptr, c->sD, c->lineS, it.psi});
+            for (int k = 0; k < 3; ++k) lb[k].push_back((int)grp_lines_blocks(d, k));
+        } else {
+            left2.push_back(P.g[2]);
+            right2.push_back(P.g[3]);
+        }
+    }
+    add(v, OP_EDGE, ed, edb);
+    add(v, OP_SPEC, sp, spb);
+    add_gemm(v, left1, 1, EPI_STORE, b->prec);
+    add_gemm(v, right1, 2, EPI_SCALE, b->prec);
+    add(v, OP_LNP, ln, lb[0]);
+    add(v, OP_LNC, ln, lb[1]);
+    add(v, OP_LNO, ln, lb[2]);
+    add_gemm(v, left2, 1, EPI_STORE, b->prec);
+    add_gemm(v, right2, 2, EPI_STORE, b->prec);
+}
+
+// cases whose trial transforms are unrefined keep the trial spectrum of every level (specT)
+// and the projection's (specP), so that level 1's solves in both Vort2Flux passes start from it
+bool keeps_spectrum(const ibh_ctx *c) { return c->use_eo && c->refine == 0; }
 
 // FP64-residual refinement of the solves on nlev consecutive levels per listed case
 // (x -= Solve(M x − rhs), the correction in the batch precision), as many sweeps as each
@@ -164,10 +240,17 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
 // Vort2Flux (fluid-ops.jl:440-472) of every case: coarsify up, then the Poisson solves from
 // each case's coarsest level down; stage j of the descent is level L-1-j of a case.  With
 // `refine`, the solves on each case's finest refine_levels levels are refined before their curl.
+// Cases that keep their spectrum solve level 1 from it (SV_SPEC), and without `curl_all` their
+// coarser levels on get_bc!'s lines only (SV_LINES).
 void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
                    const std::vector<double *> &Psi, const std::vector<double *> &Q,
                    bool curl_all, bool refine = false) {
     const int n = (int)b->cs.size();
+    auto mode = [&](const ibh_ctx *c, int lev) {
+        if (!keeps_spectrum(c)) return (int)SV_FULL;
+        if (lev == 0) return (int)SV_SPEC;
+        return !curl_all && c->lines_B ? (int)SV_LINES : (int)SV_FULL;
+    };
     for (int j = 1; j < b->Lmax; ++j) {
         std::vector<CoarsA> a;
         std::vector<int> bl;
@@ -183,7 +266,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
     {
         std::vector<FillA> f;
         std::vector<int> fb;
-        std::vector<LapInvPlan> plans;
+        std::vector<SolveItem> sv;
         std::vector<CurlA> cu;
         std::vector<int> cb;
         std::vector<RefineItem> rf;
@@ -194,10 +277,9 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             double *bcL = c->psibc + (long)(L - 1) * d.nbc;
             f.push_back({bcL, d.nbc, 0.0});
             fb.push_back(nblk(d.nbc, GRP_BS));
-            LapInvPlan P;
-            lapinv_plan(c, G[i] + (L - 1) * d.ng, d.ng, Psi[i] + (L - 1) * d.ng, d.ng, c->sD, 0, 1,
-                        EPI_STORE, nullptr, nullptr, &P);
-            plans.push_back(P);
+            // ψ_L = Δinv Γ_L with ψbc = 0
+            sv.push_back({i, mode(c, L - 1), G[i] + (L - 1) * d.ng, Psi[i] + (L - 1) * d.ng,
+                          nullptr, c->specT + (L - 1) * d.ng});
             if (refine && L - 1 < c->refine_levels)
                 rf.push_back({i, Psi[i] + (L - 1) * d.ng, G[i] + (L - 1) * d.ng, -1.0, c->sD, 0,
                               1});
@@ -208,7 +290,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             }
         }
         add(v, OP_FILL, f, fb);
-        add_lapinv(v, plans, b->prec);
+        add_solves(b, v, sv);
         add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
     }
@@ -216,7 +298,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         std::vector<GetBcA> gb;
         std::vector<CopyBcA> cp;
         std::vector<int> b1, b2, cb;
-        std::vector<LapInvPlan> plans;
+        std::vector<SolveItem> sv;
         std::vector<CurlA> cu;
         std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
@@ -227,12 +309,12 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             double *bcl = c->psibc + (long)lev * d.nbc;
             gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
             b1.push_back(nblk(d.nbc, GRP_BS));
-            cp.push_back({d, G[i] + lev * d.ng, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
-            b2.push_back((int)grp_blocks(d.ng));
-            LapInvPlan P;
-            lapinv_plan(c, c->gtmp, d.ng, Psi[i] + lev * d.ng, d.ng, c->sD, 0, 1, EPI_STORE,
-                        nullptr, nullptr, &P);
-            plans.push_back(P);
+            const int md = mode(c, lev);
+            if (md != SV_SPEC) {
+                cp.push_back({d, G[i] + lev * d.ng, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
+                b2.push_back((int)grp_blocks(d.ng));
+            }
+            sv.push_back({i, md, c->gtmp, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
             if (refine && lev < c->refine_levels)
                 rf.push_back({i, Psi[i] + lev * d.ng, c->gtmp, -1.0, c->sD, 0, 1});
             if (curl_all || lev == 0) {
@@ -243,7 +325,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         }
         add(v, OP_GETBC, gb, b1);
         add(v, OP_COPYBC, cp, b2);
-        add_lapinv(v, plans, b->prec);
+        add_solves(b, v, sv);
         add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
     }
@@ -299,6 +381,10 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             LapInvPlan P2;
             lapinv_plan(c, c->rhsb, d.ng, gs[i], d.ng, c->sAinv, d.ng, c->L, EPI_STORE, nullptr,
                         nullptr, &P2);
+            if (keeps_spectrum(c)) {  // Γs = S F S: keep F of every level
+                P2.g[1].C = c->specT;
+                P2.g[2].B = c->specT;
+            }
             pI.push_back(P2);
             rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, d.ng,
                           std::min(c->refine_levels, c->L)});
@@ -334,7 +420,8 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         std::vector<StressA> sa;
         std::vector<ETA> et;
         std::vector<RotIA> ro;
-        std::vector<int> b1, b2, b3, b4;
+        std::vector<AxpyA> ax;
+        std::vector<int> b1, b2, b3, b4, b5;
         std::vector<LapInvPlan> plans;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
@@ -351,13 +438,27 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             b4.push_back(nblk(d.ng, GRP_BS));
             LapInvPlan P;
             lapinv_plan(c, c->g1, d.ng, gs[i], d.ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr, &P);
+            if (keeps_spectrum(c)) {  // Γ₁ = S(F_trial − F_proj)S: keep F_proj and subtract it
+                P.g[1].C = c->specP;
+                P.g[2].B = c->specP;
+                ax.push_back({d.ng, c->specT, c->specP, 1.0});
+                b5.push_back(nblk(d.ng, GRP_BS));
+            }
+            if (c->box_on) {  // rot Eᵀ Fb vanishes outside the body box
+                P.g[0].klo = c->bxa / 2;
+                P.g[0].khi = c->bxb / 2 + 1;
+                P.g[1].klo = c->bya / 2;
+                P.g[1].khi = c->byb / 2 + 1;
+                rows_for_pairs(P.g[0], P.g[1]);
+            }
             plans.push_back(P);
         }
         add(v, OP_STRESS, sa, b1);
         add(v, OP_ETZS, et, b2);
         add(v, OP_ET, et, b3);
         add(v, OP_ROTI, ro, b4);
-        add_lapinv(v, plans, IBH_FP64);  // the projection stays FP64
+        add_lapinv(v, plans, IBH_FP64, false);  // the projection stays FP64
+        add(v, OP_AXPY, ax, b5);
     }
     add_vort2flux(b, v, gs, psi, q, true, true);
 }
@@ -443,6 +544,12 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
     case OP_ETF: launch_grp_ET_fix(g, s); break;
     case OP_ETZS: launch_grp_ET_zero_scale(g, s); break;
     case OP_ROTI: launch_grp_rot_image(g, s); break;
+    case OP_EDGE: launch_grp_edge_gemv(g, s); break;
+    case OP_SPEC: launch_grp_spectral_scale(g, s); break;
+    case OP_LNP: launch_grp_lines_part(g, s); break;
+    case OP_LNC: launch_grp_lines_colsum(g, s); break;
+    case OP_LNO: launch_grp_lines_out(g, s); break;
+    case OP_AXPY: launch_grp_axpy_sub(g, s); break;
     case OP_GEMVP: launch_grp_gemv_part(g, s); break;
     case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
     case OP_STRESS: launch_grp_stress(g, s); break;
@@ -491,7 +598,7 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
             if (b->tune_fixed >= 0 && b->tune_fixed < gemm_eo_grp_variant_count(S.prec))
                 S.variant = b->tune_fixed;
             else if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
-            else if (may_tune) {
+            else if (may_tune && S.epi != EPI_SUBFROM) {  // re-running it would subtract again
                 int rc = tune_stage(b, S);
                 if (rc) return rc;
             }

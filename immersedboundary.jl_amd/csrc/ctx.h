@@ -179,4 +179,11 @@ void lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long sout,
                  const double *scale, long sscale, int nbatch, int epi, const RhsExtras *rx,
                  const int *skip, LapInvPlan *plan);
 
+// Algorithmic flops of one split contraction (kind 1 left, 2 right), narrowing included
+double gemm_flops(const GemmArgs &g, int kind);
+
+// A forward pair whose second contraction sums only the pairs [klo, khi): the first forms only
+// the (128-aligned) rows those K tiles read
+void rows_for_pairs(GemmArgs &g0, const GemmArgs &g1);
+
 }  // namespace ibh

@@ -611,8 +611,12 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo(const GemmArgs *g
     const GemmArgs g = gs[p];
     const int local = T - off[p];
     const int rows = SIDE == 0 ? g.h : g.M, cols = SIDE == 0 ? g.N : g.h;
-    const int gx = (rows + BM - 1) / BM, gy = (cols + BN - 1) / BN;
-    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, local % gx, (local / gx) % gy,
+    // output tiles [olo, ohi) per dimension where ohi > 0 (as k_gemm_eo's grid)
+    const int x0 = g.ohi[0] > 0 ? g.olo[0] / BM : 0, y0 = g.ohi[1] > 0 ? g.olo[1] / BN : 0;
+    const int gx = ((g.ohi[0] > 0 ? min(g.ohi[0], rows) : rows) + BM - 1) / BM - x0;
+    const int gy = ((g.ohi[1] > 0 ? min(g.ohi[1], cols) : cols) + BN - 1) / BN - y0;
+    eo_tile<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, NB, WS>(g, x0 + local % gx,
+                                                            y0 + (local / gx) % gy,
                                                             local / (gx * gy));
     ts_mark(ts, true, blockIdx.x);
 }
@@ -836,7 +840,11 @@ long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side) {
     const VariantGEO *t = table_geo(prec, &n);
     const VariantGEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
     const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
-    return (long)((gm + V.bm - 1) / V.bm) * ((gn + V.bn - 1) / V.bn) * g.batch;
+    const int ex = g.ohi[0] > 0 ? std::min(g.ohi[0], gm) : gm;
+    const int ey = g.ohi[1] > 0 ? std::min(g.ohi[1], gn) : gn;
+    const long tx = (ex + V.bm - 1) / V.bm - (g.ohi[0] > 0 ? g.olo[0] / V.bm : 0);
+    const long ty = (ey + V.bn - 1) / V.bn - (g.ohi[1] > 0 ? g.olo[1] / V.bn : 0);
+    return std::max(0L, tx) * std::max(0L, ty) * g.batch;
 }
 
 void launch_gemm_eo_grp(const Grp &gr, int prec, int variant, int side, int epi, hipStream_t s,

@@ -491,15 +491,11 @@ __global__ void k_fill(double *p, long n, double v) {
 // (S symmetric).  k_edge_gemv forms W = [S_m cB | S_m cT | S_p cL | S_p cR] (2m + 2p per
 // column), one workgroup per output entry; k_edge_add adds the rank-4 term, times the
 // eigenvalue scale D, to the scaled spectral array F = (S_m X S_p) ⊙ D.
-__global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restrict__ Sm,
-                                                   const double *__restrict__ Sp,
-                                                   const double *__restrict__ bc, long sbc,
-                                                   double *__restrict__ W, long sW,
-                                                   const int *skip) {
-    SKIP_GUARD
+// one 256-thread workgroup: output entry x of one batch entry (boundary values r, W row Wb)
+__device__ inline void edge_gemv_row(const Dom &d, const double *__restrict__ Sm,
+                                     const double *__restrict__ Sp, const double *__restrict__ r,
+                                     double *__restrict__ Wb, int x) {
     __shared__ double sh[2][4];
-    const int x = blockIdx.x, b = blockIdx.y;
-    const double *r = bc + b * sbc;
     const bool xs = x < d.m;                    // S_m rows first, then S_p rows
     const int row = xs ? x : x - d.m, n = xs ? d.m : d.p;
     const double *S = (xs ? Sm : Sp) + (long)row * n;
@@ -521,7 +517,6 @@ __global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restri
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double *Wb = W + b * sW;
         const double t1 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
         const double t2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
         if (xs) {
@@ -532,6 +527,15 @@ __global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restri
             Wb[2 * d.m + d.p + row] = t2;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restrict__ Sm,
+                                                   const double *__restrict__ Sp,
+                                                   const double *__restrict__ bc, long sbc,
+                                                   double *__restrict__ W, long sW,
+                                                   const int *skip) {
+    SKIP_GUARD
+    edge_gemv_row(d, Sm, Sp, bc + blockIdx.y * sbc, W + blockIdx.y * sW, blockIdx.x);
 }
 
 // the rank-4 boundary term at spectral entry (a, c), from one batch entry's W
@@ -601,13 +605,10 @@ __device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F,
 }
 
 // one wave per entry a: lane l adds chunks l and l + 64 (coalesced reads), then a wave sum
-__global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int *skip) {
-    SKIP_GUARD
+__device__ inline void lines_colsum(const Dom &d, double *__restrict__ Wb, int a) {
     static_assert(LN_CH == 128, "two chunks per lane");
-    const int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), bt = blockIdx.y;
     const int lane = threadIdx.x & 63;
     if (a >= d.m) return;
-    double *Wb = W + bt * sW;
     const double *w = Wb + (long)a * LN_CH * 2;
     double s1 = w[lane * 2] + w[(lane + 64) * 2];
     double s2 = w[lane * 2 + 1] + w[(lane + 64) * 2 + 1];
@@ -617,6 +618,11 @@ __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int
         Wb[2L * LN_CH * d.m + a] = s1;          // v_J1
         Wb[(2L * LN_CH + 1) * d.m + a] = s2;    // v_J2
     }
+}
+
+__global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int *skip) {
+    SKIP_GUARD
+    lines_colsum(d, W + blockIdx.y * sW, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
 }
 
 // one 256-thread workgroup per column b: wave w sums a quarter of the rows lane-strided, the
@@ -674,17 +680,11 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
 
 // one 256-thread workgroup per output entry: wave w sums a quarter of the index range
 // lane-strided, the four wave sums are added in order (deterministic)
-__global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restrict__ Sm,
-                                                   const double *__restrict__ Sp,
-                                                   const double *__restrict__ W, long sW,
-                                                   double *__restrict__ psi, long spsi,
-                                                   const int *skip) {
-    SKIP_GUARD
+__device__ inline void lines_out(const Dom &d, const double *__restrict__ Sm,
+                                 const double *__restrict__ Sp, const double *__restrict__ Wb,
+                                 double *__restrict__ ps, int o) {
     __shared__ double sh[2][4];
-    const int o = blockIdx.x, bt = blockIdx.y;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double *Wb = W + bt * sW;
-    double *ps = psi + bt * spsi;
     const bool col = o < d.m;
     const int n = col ? d.m : d.p;
     const int k0 = (int)((long)w * n / 4), k1 = (int)((long)(w + 1) * n / 4);
@@ -727,6 +727,15 @@ __global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restri
             ps[(3 * d.nx / 4 - 1) + (long)d.m * c] = t2;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restrict__ Sm,
+                                                   const double *__restrict__ Sp,
+                                                   const double *__restrict__ W, long sW,
+                                                   double *__restrict__ psi, long spsi,
+                                                   const int *skip) {
+    SKIP_GUARD
+    lines_out(d, Sm, Sp, W + blockIdx.y * sW, psi + blockIdx.y * spsi, blockIdx.x);
 }
 
 long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d.p; }
@@ -983,6 +992,69 @@ GRP_LAUNCH(launch_grp_bc_prefix, kg_bc_prefix, BcPrefA)
 GRP_LAUNCH(launch_grp_rhs, kg_rhs, RhsA)
 GRP_LAUNCH(launch_grp_resid, kg_resid, ResidA)
 GRP_LAUNCH(launch_grp_fill, kg_fill, FillA)
+
+static_assert(GRP_BS == 256, "the edge and lines passes run 256-thread workgroups");
+
+// m + p workgroups per problem
+__global__ __launch_bounds__(GRP_BS) void kg_edge_gemv(const EdgeA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const EdgeA A = args[p];
+    edge_gemv_row(A.d, A.Sm, A.Sp, A.bc, A.W, blockIdx.x - off[p]);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, const int *off,
+                                                            int n) {
+    GRP_PROLOGUE(SpecA, A.d.ng)
+    (void)b;  // one batch entry per problem
+    if (t >= d.ng) return;
+    double v = A.kappa * A.A[t] * A.D[t];
+    if (A.EW) v += A.D[t] * edge_term(d, A.Sm, A.Sp, A.EW, (int)(t % d.m), (int)(t / d.m));
+    A.F[t] = v;
+}
+
+long grp_lines_blocks(const Dom &d, int pass) {
+    if (pass == 0) return nblk(d.m, 256) * (long)LN_CH + d.p;
+    if (pass == 1) return nblk(d.m, 4);
+    return d.m + d.p;
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const LinesA A = args[p];
+    const int blk = blockIdx.x - off[p], nca = (int)nblk(A.d.m, 256);
+    if (blk < nca * LN_CH)
+        lines_colpart(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0, (blk % nca) * 256 + threadIdx.x,
+                      blk / nca, 0);
+    else
+        lines_row(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0, blk - nca * LN_CH, 0);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, const int *off,
+                                                          int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const LinesA A = args[p];
+    lines_colsum(A.d, A.W, (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6));
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_lines_out(const LinesA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const LinesA A = args[p];
+    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, blockIdx.x - off[p]);
+}
+
+__global__ __launch_bounds__(GRP_BS) void kg_axpy_sub(const AxpyA *args, const int *off, int n) {
+    const int p = grp_find(off, n, blockIdx.x);
+    const AxpyA A = args[p];
+    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    if (t < A.n) A.y[t] -= A.x[t] * A.a;
+}
+
+GRP_LAUNCH(launch_grp_edge_gemv, kg_edge_gemv, EdgeA)
+GRP_LAUNCH(launch_grp_spectral_scale, kg_spectral_scale, SpecA)
+GRP_LAUNCH(launch_grp_lines_part, kg_lines_part, LinesA)
+GRP_LAUNCH(launch_grp_lines_colsum, kg_lines_colsum, LinesA)
+GRP_LAUNCH(launch_grp_lines_out, kg_lines_out, LinesA)
+GRP_LAUNCH(launch_grp_axpy_sub, kg_axpy_sub, AxpyA)
 
 
 // ------------------------------------------------------------------ adjoint transfers

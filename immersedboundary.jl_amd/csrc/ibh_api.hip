@@ -218,9 +218,11 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     return c->tuned[key] = best;
 }
 
+}  // namespace
+
 // Algorithmic flops of a launch: 2MNK for the full contraction, 2·2·h·k·(data rows) for the
 // split one (two h×k parts; k = h pairs, or the khi - klo pairs a narrowed launch sums)
-double gemm_flops(const GemmArgs &g, int kind) {
+double ibh::gemm_flops(const GemmArgs &g, int kind) {
     if (kind == 0) return 2.0 * g.M * (double)g.N * g.K * g.batch;
     const int ext[2] = {kind == 1 ? g.h : g.M, kind == 1 ? g.N : g.h};
     double e[2];
@@ -230,6 +232,8 @@ double gemm_flops(const GemmArgs &g, int kind) {
     const double kp = g.khi > 0 ? (double)(g.khi - g.klo) : (double)g.h;
     return 4.0 * pairs * kp * rows * g.batch;
 }
+
+namespace {
 
 void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind = 0) {
     const int v = tune(c, g, prec, kind, tout);
@@ -383,15 +387,19 @@ void refine_sweeps(ibh_ctx *c, double *x, const double *rhs, double a, const dou
     }
 }
 
+}  // namespace
+
 // A forward pair whose second contraction sums only the pairs [klo, khi) of the first one's
 // output rows: the first contraction forms only the rows those K tiles read (128-aligned, so
 // every tile variant, BK ≤ 64 pairs, reads rows inside the range); other rows of its output keep
 // stale values that nothing reads.
-void rows_for_pairs(GemmArgs &g0, const GemmArgs &g1) {
+void ibh::rows_for_pairs(GemmArgs &g0, const GemmArgs &g1) {
     if (g1.khi <= 0) return;
     g0.olo[1] = std::max(0, (2 * g1.klo / 128) * 128);
     g0.ohi[1] = std::min(g0.N, ((2 * g1.khi + 127) / 128) * 128);
 }
+
+namespace {
 
 // Poisson solve ψ = Δinv(G + edges(bc)) on level `lev` of a B-product, where G is zero
 // outside the rows/columns [bx_lo, bx_hi) × [by_lo, by_hi) of that level: the two forward

@@ -313,6 +313,16 @@ struct StressA {
     int nbodies; const int *boff, *npts; int nb; const double *Fb, *ds; double h, dt;
     double *CD, *CL, *fbs;
 };
+// The Poisson solve of the finest level from a kept spectrum and the coarse levels' ψ on
+// get_bc!'s lines (grouped forms of launch_edge_gemv, launch_spectral_scale and launch_lines,
+// one batch entry per problem; W of EdgeA holds 2(m + p), of LinesA lines_work_size(d)
+// doubles), and y -= a·x.
+struct EdgeA { Dom d; const double *Sm, *Sp, *bc; double *W; };
+struct SpecA { Dom d; const double *A, *D; double kappa; const double *Sm, *Sp, *EW; double *F; };
+struct LinesA { Dom d; const double *F, *Sm, *Sp, *EW, *D; double *W, *psi; };
+struct AxpyA { long n; double *y; const double *x; double a; };
+// workgroups per problem of the three lines passes
+long grp_lines_blocks(const Dom &d, int pass);
 
 // element counts per batch entry of each grouped op (host side, for the block offsets)
 __host__ __device__ inline long grp_elems_coarsify(const Dom &d) {
@@ -344,7 +354,13 @@ void launch_grp_nonlinear(const Grp &g, hipStream_t s);
 void launch_grp_bc_prefix(const Grp &g, hipStream_t s);   // blocks: ceil(nbc/256)
 void launch_grp_rhs(const Grp &g, hipStream_t s);         // blocks: ceil(nΓ/256)·levels
 void launch_grp_resid(const Grp &g, hipStream_t s);       // blocks: ceil(nΓ/256)
-void launch_grp_fill(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
+void launch_grp_fill(const Grp &g, hipStream_t s);
+void launch_grp_edge_gemv(const Grp &g, hipStream_t s);
+void launch_grp_spectral_scale(const Grp &g, hipStream_t s);
+void launch_grp_lines_part(const Grp &g, hipStream_t s);
+void launch_grp_lines_colsum(const Grp &g, hipStream_t s);
+void launch_grp_lines_out(const Grp &g, hipStream_t s);
+void launch_grp_axpy_sub(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
 void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
 void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)
 void launch_grp_ET_scale(const Grp &g, hipStream_t s);    // ETA; blocks: 1
