@@ -238,10 +238,11 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
 }
 
 // Vort2Flux (fluid-ops.jl:440-472) of every case: coarsify up, then the Poisson solves from
-// each case's coarsest level down; stage j of the descent is level L-1-j of a case.  With
-// `refine`, the solves on each case's finest refine_levels levels are refined before their curl.
-// Cases that keep their spectrum solve level 1 from it (SV_SPEC), and without `curl_all` their
-// coarser levels on get_bc!'s lines only (SV_LINES).
+// each case's coarsest level down.  Stage j of the descent solves level Lmax-1-j of every case
+// that has it: cases with fewer levels join later, and every case solves level 1 in the last
+// stage.  With `refine`, the solves on each case's finest refine_levels levels are refined
+// before their curl.  Cases that keep their spectrum solve level 1 from it (SV_SPEC), and
+// without `curl_all` their coarser levels on get_bc!'s lines only (SV_LINES).
 void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
                    const std::vector<double *> &Psi, const std::vector<double *> &Q,
                    bool curl_all, bool refine = false) {
@@ -263,38 +264,17 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         }
         add(v, OP_COARS, a, bl);
     }
-    {
+    {   // ψbc = 0 on every case's coarsest level
         std::vector<FillA> f;
         std::vector<int> fb;
-        std::vector<SolveItem> sv;
-        std::vector<CurlA> cu;
-        std::vector<int> cb;
-        std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
-            const Dom &d = c->d;
-            const int L = c->L;
-            double *bcL = c->psibc + (long)(L - 1) * d.nbc;
-            f.push_back({bcL, d.nbc, 0.0});
-            fb.push_back(nblk(d.nbc, GRP_BS));
-            // ψ_L = Δinv Γ_L with ψbc = 0
-            sv.push_back({i, mode(c, L - 1), G[i] + (L - 1) * d.ng, Psi[i] + (L - 1) * d.ng,
-                          nullptr, c->specT + (L - 1) * d.ng});
-            if (refine && L - 1 < c->refine_levels)
-                rf.push_back({i, Psi[i] + (L - 1) * d.ng, G[i] + (L - 1) * d.ng, -1.0, c->sD, 0,
-                              1});
-            if (curl_all || L == 1) {
-                cu.push_back({d, Psi[i] + (L - 1) * d.ng, 0, bcL, 0,
-                              Q[i] + (curl_all ? (L - 1) * d.nq : 0), 0, 1});
-                cb.push_back((int)grp_blocks(d.nq));
-            }
+            f.push_back({c->psibc + (long)(c->L - 1) * c->d.nbc, c->d.nbc, 0.0});
+            fb.push_back(nblk(c->d.nbc, GRP_BS));
         }
         add(v, OP_FILL, f, fb);
-        add_solves(b, v, sv);
-        add_refine(b, v, rf);
-        add(v, OP_CURL, cu, cb);
     }
-    for (int j = 1; j < b->Lmax; ++j) {
+    for (int lev = b->Lmax - 1; lev >= 0; --lev) {
         std::vector<GetBcA> gb;
         std::vector<CopyBcA> cp;
         std::vector<int> b1, b2, cb;
@@ -303,20 +283,25 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         std::vector<RefineItem> rf;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
-            if (c->L <= j) continue;
+            if (c->L <= lev) continue;
             const Dom &d = c->d;
-            const int lev = c->L - 1 - j;
             double *bcl = c->psibc + (long)lev * d.nbc;
-            gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
-            b1.push_back(nblk(d.nbc, GRP_BS));
             const int md = mode(c, lev);
-            if (md != SV_SPEC) {
-                cp.push_back({d, G[i] + lev * d.ng, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
-                b2.push_back((int)grp_blocks(d.ng));
+            const double *rhs = G[i] + lev * d.ng;
+            if (lev == c->L - 1) {  // the coarsest level: ψ_L = Δinv Γ_L
+                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, nullptr, c->specT + lev * d.ng});
+            } else {
+                gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
+                b1.push_back(nblk(d.nbc, GRP_BS));
+                if (md != SV_SPEC) {
+                    cp.push_back({d, rhs, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
+                    b2.push_back((int)grp_blocks(d.ng));
+                }
+                rhs = c->gtmp;
+                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
             }
-            sv.push_back({i, md, c->gtmp, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
             if (refine && lev < c->refine_levels)
-                rf.push_back({i, Psi[i] + lev * d.ng, c->gtmp, -1.0, c->sD, 0, 1});
+                rf.push_back({i, Psi[i] + lev * d.ng, rhs, -1.0, c->sD, 0, 1});
             if (curl_all || lev == 0) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
