@@ -39,6 +39,10 @@ struct Stage {
     int op = 0, n = 0;
     std::vector<char> args;    // n argument structs
     std::vector<int> blocks;   // per problem (non-GEMM ops)
+    // E and Eᵀ: workgroups per problem of each block schedule (the variants the tuner times);
+    // Eᵀ's tuning re-runs the zeroing stage tune_pre before each timed scatter
+    std::vector<std::vector<int>> vblocks;
+    int tune_pre = -1;
     // contractions
     std::vector<GemmArgs> gargs;
     int side = 0, epi = 0, kind = 0, prec = 0;
@@ -66,6 +70,7 @@ struct ibh_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool use_graphs = true, tuned = false;
     int tune_fixed = -1;  // grouped contraction variant forced for every stage (IBH_GRP_VARIANT)
+    int e_fixed = -1, et_fixed = -1;  // E / Eᵀ block schedules forced (IBH_REG_SCHED="e,et")
     std::vector<std::string> tune_log;
     ibh::Prof prof;
     std::string err;
@@ -396,6 +401,14 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             bs.push_back(nblk(nf, GRP_BS / 64));  // one wave per row
         }
         add(v, OP_E, e, be);
+        if (!e.empty()) {
+            Stage &S = v.back();
+            for (int k = 0; k < e_sched_count(); ++k) {
+                std::vector<int> vb;
+                for (const EA &a : e) vb.push_back((int)grp_E_blocks(a.r.nb, k));
+                S.vblocks.push_back(vb);
+            }
+        }
         add(v, OP_GEMVP, g, bp);
         add(v, OP_GEMVS, g, bs);
     }
@@ -441,6 +454,15 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         add(v, OP_STRESS, sa, b1);
         add(v, OP_ETZS, et, b2);
         add(v, OP_ET, et, b3);
+        if (!et.empty()) {
+            Stage &S = v.back();
+            S.tune_pre = (int)v.size() - 2;  // the zeroing + scale stage
+            for (int k = 0; k < et_sched_count(); ++k) {
+                std::vector<int> vb;
+                for (const ETA &a : et) vb.push_back((int)grp_ET_blocks(a.r.nb, k));
+                S.vblocks.push_back(vb);
+            }
+        }
         add(v, OP_ROTI, ro, b4);
         add_lapinv(v, plans, IBH_FP64, false);  // the projection stays FP64
         add(v, OP_AXPY, ax, b5);
@@ -456,7 +478,7 @@ int upload(ibh_batch *b) {
         for (Stage &S : b->st[P]) {
             const int nv = gemm_eo_grp_variant_count(S.prec);
             if (S.op == OP_GEMM) bytes += al(S.gargs.size() * sizeof(GemmArgs)) + al(sizeof(int) * (S.n + 1) * nv);
-            else bytes += al(S.args.size()) + al(sizeof(int) * (S.n + 1));
+            else bytes += al(S.args.size()) + al(sizeof(int) * (S.n + 1) * std::max<size_t>(1, S.vblocks.size()));
         }
     BCK(hipMalloc(&b->arena, bytes + 256));
     std::vector<char> host(bytes + 256, 0);
@@ -488,14 +510,20 @@ int upload(ibh_batch *b) {
                 std::memcpy(host.data() + o, S.args.data(), S.args.size());
                 S.d_args = base + o;
                 o += al(S.args.size());
-                std::vector<int> off(S.n + 1);
-                int acc = 0;
-                for (int i = 0; i < S.n; ++i) {
-                    off[i] = acc;
-                    acc += S.blocks[i];
+                // one offset row per block schedule (a single row for the other ops)
+                const std::vector<std::vector<int>> rows =
+                    S.vblocks.empty() ? std::vector<std::vector<int>>{S.blocks} : S.vblocks;
+                std::vector<int> off((size_t)(S.n + 1) * rows.size());
+                S.total.assign(rows.size(), 0);
+                for (size_t r = 0; r < rows.size(); ++r) {
+                    int acc = 0;
+                    for (int i = 0; i < S.n; ++i) {
+                        off[r * (S.n + 1) + i] = acc;
+                        acc += rows[r][i];
+                    }
+                    off[r * (S.n + 1) + S.n] = acc;
+                    S.total[r] = acc;
                 }
-                off[S.n] = acc;
-                S.total.assign(1, acc);
                 std::memcpy(host.data() + o, off.data(), off.size() * sizeof(int));
                 S.d_off = reinterpret_cast<const int *>(base + o);
                 o += al(off.size() * sizeof(int));
@@ -522,10 +550,18 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
     case OP_NONLIN: launch_grp_nonlinear(g, s); break;
     case OP_BCPREF: launch_grp_bc_prefix(g, s); break;
     case OP_FILL: launch_grp_fill(g, s); break;
-    case OP_E: launch_grp_E(g, s); break;
+    case OP_E:
+        g.off = S.d_off + (size_t)variant * (S.n + 1);
+        g.blocks = S.total[variant];
+        launch_grp_E(g, variant, s);
+        break;
     case OP_ETZ: launch_grp_ET_zero(g, s); break;
     case OP_ETS: launch_grp_ET_scale(g, s); break;
-    case OP_ET: launch_grp_ET(g, s); break;
+    case OP_ET:
+        g.off = S.d_off + (size_t)variant * (S.n + 1);
+        g.blocks = S.total[variant];
+        launch_grp_ET(g, variant, s);
+        break;
     case OP_ETF: launch_grp_ET_fix(g, s); break;
     case OP_ETZS: launch_grp_ET_zero_scale(g, s); break;
     case OP_ROTI: launch_grp_rot_image(g, s); break;
@@ -546,17 +582,24 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
 // Time every tile variant of a contraction stage on its real operands (each grouped
 // contraction reads operands it does not write, so re-running it leaves the same result) and
 // keep the fastest.  Variants of one precision sum k in the same order: identical results.
-int tune_stage(ibh_batch *b, Stage &S) {
-    const int nv = gemm_eo_grp_variant_count(S.prec);
+int tune_stage(ibh_batch *b, std::vector<Stage> &st, size_t k) {
+    Stage &S = st[k];
+    const bool gemm = S.op == OP_GEMM;
+    const int nv = gemm ? gemm_eo_grp_variant_count(S.prec) : (int)S.vblocks.size();
+    // a stage that accumulates into its output (Eᵀ) re-runs the stage that clears it first
+    auto run = [&](int v) {
+        if (S.tune_pre >= 0) launch_stage(b, st[S.tune_pre], st[S.tune_pre].variant);
+        launch_stage(b, S, v);
+    };
     hipEvent_t e0, e1;
     BCK(hipEventCreate(&e0));
     BCK(hipEventCreate(&e1));
     int best = 0;
     float best_ms = 1e30f;
     for (int v = 0; v < nv; ++v) {
-        launch_stage(b, S, v);
+        run(v);
         BCK(hipEventRecord(e0, b->s));
-        for (int r = 0; r < 3; ++r) launch_stage(b, S, v);
+        for (int r = 0; r < 3; ++r) run(v);
         BCK(hipEventRecord(e1, b->s));
         BCK(hipEventSynchronize(e1));
         float ms = 0;
@@ -566,12 +609,21 @@ int tune_stage(ibh_batch *b, Stage &S) {
             best = v;
         }
     }
+    // every variant leaves the same output (the same bits); a stage that accumulates has its
+    // output cleared once more, for the launch that follows the tuning in run_stages
+    if (S.tune_pre >= 0) launch_stage(b, st[S.tune_pre], st[S.tune_pre].variant);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     S.variant = best;
     char line[256];
-    std::snprintf(line, sizeof line, "%s %s problems=%d %.1fus", gemm_eo_grp_variant_name(S.prec, best),
-                  S.kind == 1 ? "left" : "right", S.n, best_ms / 3 * 1e3);
+    if (gemm)
+        std::snprintf(line, sizeof line, "%s %s problems=%d %.1fus",
+                      gemm_eo_grp_variant_name(S.prec, best), S.kind == 1 ? "left" : "right", S.n,
+                      best_ms / 3 * 1e3);
+    else
+        std::snprintf(line, sizeof line, "%s schedule %s problems=%d %.1fus%s",
+                      S.op == OP_E ? "E" : "Eᵀ", S.op == OP_E ? e_sched_name(best) : et_sched_name(best),
+                      S.n, best_ms / 3 * 1e3, S.tune_pre >= 0 ? " (with its zeroing)" : "");
     b->tune_log.push_back(line);
     return IBH_OK;
 }
@@ -584,7 +636,16 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
                 S.variant = b->tune_fixed;
             else if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
             else if (may_tune && S.epi != EPI_SUBFROM) {  // re-running it would subtract again
-                int rc = tune_stage(b, S);
+                int rc = tune_stage(b, v, k);
+                if (rc) return rc;
+            }
+            if (S.variant < 0) S.variant = 0;
+        }
+        if (!S.vblocks.empty() && S.variant < 0) {  // E and Eᵀ block schedules
+            const int fixed = S.op == OP_E ? b->e_fixed : b->et_fixed;
+            if (fixed >= 0 && fixed < (int)S.vblocks.size()) S.variant = fixed;
+            else if (may_tune) {
+                int rc = tune_stage(b, v, k);
                 if (rc) return rc;
             }
             if (S.variant < 0) S.variant = 0;
@@ -698,6 +759,7 @@ int ibh_batch_create(int32_t n, ibh_ctx *const *cases, ibh_batch **out) {
     b->prec = cases[0]->precision;
     if (const char *ge = std::getenv("IBH_GRAPH")) b->use_graphs = std::atoi(ge) != 0;
     if (const char *gv = std::getenv("IBH_GRP_VARIANT")) b->tune_fixed = std::atoi(gv);
+    if (const char *rs = std::getenv("IBH_REG_SCHED")) parse_reg_sched(rs, &b->e_fixed, &b->et_fixed);
     if (hipDeviceSynchronize() != hipSuccess || hipStreamCreate(&b->s) != hipSuccess ||
         hipMalloc(&b->uv, 2 * sizeof(double)) != hipSuccess) {
         ibh_batch_destroy(b);

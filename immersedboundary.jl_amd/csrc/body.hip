@@ -1,6 +1,8 @@
 // Body coupling: Yang3 regularized delta (structure-ops.jl), interpolation E, spreading Eᵀ,
 // surface-force solvers (dense inverse GEMV and CG, cnab/ops.jl:79-114) and the per-body
 // force coefficients (streamfcn/states.jl:53-84).  Reductions are FP64.
+#include <cstdio>
+
 #include "ibh_internal.h"
 
 namespace ibh {
@@ -78,6 +80,51 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
                        pts, h, x0, y0);
 }
 
+// ---- block schedules (see e_sched_name): E's waves per workgroup; Eᵀ's workgroup size and
+// entry order
+constexpr int E_WAVES[] = {4, 1, 2, 8, 16};
+constexpr int E_NSCHED = sizeof(E_WAVES) / sizeof(E_WAVES[0]);
+struct ETSched {
+    int bs;
+    bool pm;  // point-major
+};
+constexpr ETSched ET_SCHED[] = {{256, false}, {64, false},  {128, false}, {512, false},
+                                {1024, false}, {64, true},  {128, true},  {256, true},
+                                {512, true},  {1024, true}};
+constexpr int ET_NSCHED = sizeof(ET_SCHED) / sizeof(ET_SCHED[0]);
+
+int e_sched_count() { return E_NSCHED; }
+int et_sched_count() { return ET_NSCHED; }
+const char *e_sched_name(int v) {
+    static const char *names[] = {"4 waves/workgroup", "1 wave/workgroup", "2 waves/workgroup",
+                                  "8 waves/workgroup", "16 waves/workgroup"};
+    return v >= 0 && v < E_NSCHED ? names[v] : "?";
+}
+const char *et_sched_name(int v) {
+    static const char *names[] = {"256 threads entry-major", "64 threads entry-major",
+                                  "128 threads entry-major", "512 threads entry-major",
+                                  "1024 threads entry-major", "64 threads point-major",
+                                  "128 threads point-major", "256 threads point-major",
+                                  "512 threads point-major", "1024 threads point-major"};
+    return v >= 0 && v < ET_NSCHED ? names[v] : "?";
+}
+void parse_reg_sched(const char *s, int *e, int *et) {
+    int a = -1, b = -1;
+    if (s && std::sscanf(s, "%d,%d", &a, &b) >= 1) {
+        *e = a;
+        *et = b;
+    }
+}
+
+static inline int et_sched_index(int v) { return v >= 0 && v < ET_NSCHED ? v : 0; }
+static inline int e_sched_index(int v) { return v >= 0 && v < E_NSCHED ? v : 0; }
+
+// stencil entry (point-component kc, then tt, s) of thread t: entry-major walks the 169 entries
+// of one point-component together, point-major one entry of the n2 point-components together
+__device__ inline int et_entry(int t, int n2, bool pm) {
+    return pm ? (t % n2) * (NS * NS) + t / n2 : t;
+}
+
 // E (RegT), structure-ops.jl:88-105: fb[k] = Σ_t Σ_s q[i, j]·w over the 13×13 stencil of point
 // k, with q0 (constant per component) added to q first, as in Q = qs + q0.  One wave per output:
 // lane l sums the entries l, l + 64, l + 128 (column-major, s fastest), then a fixed-order wave
@@ -107,16 +154,21 @@ __device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__r
 
 // With lrk > 0 the wave also adds the low-rank term Σ_j U[t, j]·w[b, j] (U 2nb × lrk column-
 // major, w lrk per batch entry), summed over the lanes in a fixed order.
-__global__ void k_E(Dom d, RegDev r, const double *__restrict__ q, long sq, double u0, double v0,
-                    double *__restrict__ fb, long sfb, const int *skip, const double *uv,
-                    double uvscale, const double *__restrict__ U, const double *__restrict__ w,
-                    int lrk, int t0, int tn) {
+// W waves (outputs) per workgroup.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__restrict__ q,
+                                              long sq, double u0, double v0,
+                                              double *__restrict__ fb, long sfb, const int *skip,
+                                              const double *uv, double uvscale,
+                                              const double *__restrict__ U,
+                                              const double *__restrict__ w, int lrk, int t0,
+                                              int tn) {
     if (skip && *skip) return;
     if (uv) {  // q0 = freestream·h from the device parameter block
         u0 = uv[0] * uvscale;
         v0 = uv[1] * uvscale;
     }
-    const int tl = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int tl = blockIdx.x * W + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tl >= tn) return;
     const int t = t0 + tl;
     const int b = blockIdx.y;
@@ -137,8 +189,17 @@ void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0
     if (r.nb == 0) return;
     if (tn < 0) tn = 2 * r.nb - t0;
     if (tn <= 0) return;
-    hipLaunchKernelGGL(k_E, dim3(nblk(tn, 4), nbatch), dim3(256), 0, s, d, r, q, sq, u0, v0, fb,
-                       sfb, skip, uv, uvscale, U, w, lrk, t0, tn);
+#define LAUNCH_E(W)                                                                           \
+    hipLaunchKernelGGL(k_E<W>, dim3(nblk(tn, W), nbatch), dim3(64 * W), 0, s, d, r, q, sq, u0, \
+                       v0, fb, sfb, skip, uv, uvscale, U, w, lrk, t0, tn)
+    switch (E_WAVES[(r.esched >= 0 && r.esched < E_NSCHED) ? r.esched : 0]) {
+    case 1: LAUNCH_E(1); break;
+    case 2: LAUNCH_E(2); break;
+    case 8: LAUNCH_E(8); break;
+    case 16: LAUNCH_E(16); break;
+    default: LAUNCH_E(4); break;
+    }
+#undef LAUNCH_E
 }
 
 // Eᵀ (Reg), structure-ops.jl:63-86: q .= 0 then scatter-add the 13×13 stencils of every point.
@@ -197,14 +258,44 @@ __device__ inline void ET_at(const Dom &d, const RegDev &r, const double *__rest
     atomicAdd(reinterpret_cast<unsigned long long *>(dst), (unsigned long long)llrint(v * sc));
 }
 
-__global__ void k_ET(Dom d, RegDev r, const double *__restrict__ fb, long sfb,
-                     double *__restrict__ q, long sq, const double *__restrict__ scale,
-                     const int *skip) {
+template <int BS, bool PM>
+__global__ __launch_bounds__(BS) void k_ET(Dom d, RegDev r, const double *__restrict__ fb,
+                                           long sfb, double *__restrict__ q, long sq,
+                                           const double *__restrict__ scale, const int *skip) {
     if (skip && *skip) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= 2 * r.nb * NS * NS) return;
     const int b = blockIdx.y;
-    ET_at(d, r, fb + b * sfb, q + b * sq, scale[b], t);
+    ET_at(d, r, fb + b * sfb, q + b * sq, scale[b], et_entry(t, 2 * r.nb, PM));
+}
+
+// the scatter-add of Eᵀ (the image zeroed and the scales set before) in r's schedule
+static void launch_ET_scatter(const Dom &d, const RegDev &r, const double *fb, long sfb,
+                              double *q, long sq, int nbatch, const int *skip,
+                              const double *scale, hipStream_t s) {
+    const long n = 2L * r.nb * NS * NS;
+    const ETSched E = ET_SCHED[et_sched_index(r.etsched)];
+#define LAUNCH_ET(BS, PM)                                                                     \
+    hipLaunchKernelGGL((k_ET<BS, PM>), dim3(nblk(n, BS), nbatch), dim3(BS), 0, s, d, r, fb,   \
+                       sfb, q, sq, scale, skip)
+    if (E.pm) {
+        switch (E.bs) {
+        case 64: LAUNCH_ET(64, true); break;
+        case 128: LAUNCH_ET(128, true); break;
+        case 512: LAUNCH_ET(512, true); break;
+        case 1024: LAUNCH_ET(1024, true); break;
+        default: LAUNCH_ET(256, true); break;
+        }
+    } else {
+        switch (E.bs) {
+        case 64: LAUNCH_ET(64, false); break;
+        case 128: LAUNCH_ET(128, false); break;
+        case 512: LAUNCH_ET(512, false); break;
+        case 1024: LAUNCH_ET(1024, false); break;
+        default: LAUNCH_ET(256, false); break;
+        }
+    }
+#undef LAUNCH_ET
 }
 
 // fixed point → FP64 in place (exact power-of-two rescale after the int64 → double rounding)
@@ -258,9 +349,7 @@ void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, 
     hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(d.nq, 256) + 1 + lrk, nbatch), dim3(256), 0, s,
                        img, d.nq, sq, fb, sfb, 2 * r.nb, scale, skip, V, w, lrk);
     if (r.nb == 0) return;
-    long n = 2L * r.nb * NS * NS;
-    hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, img, sq,
-                       (const double *)scale, skip);
+    launch_ET_scatter(d, r, fb, sfb, img, sq, nbatch, skip, scale, s);
 }
 
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
@@ -268,9 +357,7 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
     hipLaunchKernelGGL(k_zero, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq, skip);
     if (r.nb == 0) return;
     hipLaunchKernelGGL(k_et_scale, dim3(nbatch), dim3(256), 0, s, fb, sfb, 2 * r.nb, scale, skip);
-    long n = 2L * r.nb * NS * NS;
-    hipLaunchKernelGGL(k_ET, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, d, r, fb, sfb, q, sq,
-                       (const double *)scale, skip);
+    launch_ET_scatter(d, r, fb, sfb, q, sq, nbatch, skip, scale, s);
     hipLaunchKernelGGL(k_et_fix, dim3(nblk(d.nq, 256), nbatch), dim3(256), 0, s, q, d.nq, sq,
                        (const double *)scale, skip);
 }
@@ -607,10 +694,11 @@ void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err
 // case; see Grp).  Same per-element bodies as the single-case kernels.
 static_assert(GRP_NS == NS && GV_CHUNK == GV_CH, "grouped constants");
 
-__global__ __launch_bounds__(GRP_BS) void kg_E(const EA *args, const int *off, int n) {
+template <int W>
+__global__ __launch_bounds__(64 * W) void kg_E(const EA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const EA A = args[p];
-    const int t = (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6);
+    const int t = (blockIdx.x - off[p]) * W + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (t >= 2 * A.r.nb) return;
     const double v = E_wave(A.d, A.r, A.q, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t, lane);
@@ -646,12 +734,58 @@ __global__ __launch_bounds__(GRP_BS) void kg_ET_scale(const ETA *args, const int
     et_scale_body(A.fb, 2 * A.r.nb, A.scale);
 }
 
-__global__ __launch_bounds__(GRP_BS) void kg_ET(const ETA *args, const int *off, int n) {
+template <int BS, bool PM>
+__global__ __launch_bounds__(BS) void kg_ET(const ETA *args, const int *off, int n) {
     const int p = grp_find(off, n, blockIdx.x);
     const ETA A = args[p];
-    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const int t = (blockIdx.x - off[p]) * BS + threadIdx.x;
     if (t >= 2 * A.r.nb * NS * NS) return;
-    ET_at(A.d, A.r, A.fb, A.q, *A.scale, t);
+    ET_at(A.d, A.r, A.fb, A.q, *A.scale, et_entry(t, 2 * A.r.nb, PM));
+}
+
+long grp_E_blocks(int nb, int sched) { return nblk(2L * nb, E_WAVES[e_sched_index(sched)]); }
+long grp_ET_blocks(int nb, int sched) {
+    return nblk(2L * nb * NS * NS, ET_SCHED[et_sched_index(sched)].bs);
+}
+
+void launch_grp_E(const Grp &g, int sched, hipStream_t s) {
+    if (g.n == 0 || g.blocks == 0) return;
+    const EA *a = static_cast<const EA *>(g.args);
+#define LAUNCH_GE(W) hipLaunchKernelGGL(kg_E<W>, dim3(g.blocks), dim3(64 * W), 0, s, a, g.off, g.n)
+    switch (E_WAVES[e_sched_index(sched)]) {
+    case 1: LAUNCH_GE(1); break;
+    case 2: LAUNCH_GE(2); break;
+    case 8: LAUNCH_GE(8); break;
+    case 16: LAUNCH_GE(16); break;
+    default: LAUNCH_GE(4); break;
+    }
+#undef LAUNCH_GE
+}
+
+void launch_grp_ET(const Grp &g, int sched, hipStream_t s) {
+    if (g.n == 0 || g.blocks == 0) return;
+    const ETA *a = static_cast<const ETA *>(g.args);
+    const ETSched E = ET_SCHED[et_sched_index(sched)];
+#define LAUNCH_GET(BS, PM) \
+    hipLaunchKernelGGL((kg_ET<BS, PM>), dim3(g.blocks), dim3(BS), 0, s, a, g.off, g.n)
+    if (E.pm) {
+        switch (E.bs) {
+        case 64: LAUNCH_GET(64, true); break;
+        case 128: LAUNCH_GET(128, true); break;
+        case 512: LAUNCH_GET(512, true); break;
+        case 1024: LAUNCH_GET(1024, true); break;
+        default: LAUNCH_GET(256, true); break;
+        }
+    } else {
+        switch (E.bs) {
+        case 64: LAUNCH_GET(64, false); break;
+        case 128: LAUNCH_GET(128, false); break;
+        case 512: LAUNCH_GET(512, false); break;
+        case 1024: LAUNCH_GET(1024, false); break;
+        default: LAUNCH_GET(256, false); break;
+        }
+    }
+#undef LAUNCH_GET
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_ET_fix(const ETA *args, const int *off, int n) {
@@ -691,11 +825,9 @@ __global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const i
         hipLaunchKernelGGL(KERNEL, dim3(g.blocks), dim3(GRP_BS), 0, s,                       \
                            static_cast<const TYPE *>(g.args), g.off, g.n);                   \
     }
-GRP_LAUNCH(launch_grp_E, kg_E, EA)
 GRP_LAUNCH(launch_grp_ET_zero, kg_ET_zero, ETA)
 GRP_LAUNCH(launch_grp_ET_scale, kg_ET_scale, ETA)
 GRP_LAUNCH(launch_grp_ET_zero_scale, kg_ET_zero_scale, ETA)
-GRP_LAUNCH(launch_grp_ET, kg_ET, ETA)
 GRP_LAUNCH(launch_grp_ET_fix, kg_ET_fix, ETA)
 GRP_LAUNCH(launch_grp_gemv_part, kg_gemv_part, GemvA)
 GRP_LAUNCH(launch_grp_gemv_sum, kg_gemv_sum, GemvA)

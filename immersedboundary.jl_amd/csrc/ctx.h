@@ -153,6 +153,7 @@ struct ibh_ctx {
     // GEMM autotuner: chosen variant per shape, scratch output, log of the choices
     std::map<ibh::TuneKey, int> tuned;
     int tune_fixed = -1;
+    bool reg_tuned = false;  // E / Eᵀ block schedules picked (reg.esched, reg.etsched)
     bool tune_verbose = false;  // log every candidate's time (IBH_TUNE_VERBOSE)
     double *tune_buf = nullptr;
     size_t tune_cap = 0;

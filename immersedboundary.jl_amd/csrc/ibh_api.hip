@@ -220,6 +220,74 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
 
 }  // namespace
 
+// Block schedules of E and Eᵀ for this case's points, picked once with valid points (outside a
+// graph capture): each schedule runs on the real operands (E's output and Eᵀ's image into the
+// tuning scratch, Eᵀ spreading the current forces) and the fastest is kept.  Every schedule
+// gives the same bits.  IBH_REG_SCHED="e,et" forces them.
+void tune_reg(ibh_ctx *c) {
+    if (c->reg_tuned || c->nb == 0 || c->capturing || c->no_alloc) return;
+    c->reg_tuned = true;
+    int fe = -1, fet = -1;
+    if (const char *rs = std::getenv("IBH_REG_SCHED")) parse_reg_sched(rs, &fe, &fet);
+    const Dom &d = c->d;
+    const int nf = 2 * c->nb;
+    const size_t need = (size_t)std::max((long)d.nq, (long)nf);
+    if (need > c->tune_cap) {
+        if (c->tune_buf) hipFree(c->tune_buf);
+        c->tune_buf = nullptr;
+        c->tune_cap = 0;
+        if (hipMalloc(&c->tune_buf, need * sizeof(double)) != hipSuccess) return;
+        c->tune_cap = need;
+    }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    auto time = [&](auto &&run) {
+        run();
+        hipEventRecord(e0, c->s);
+        for (int r = 0; r < 5; ++r) run();
+        hipEventRecord(e1, c->s);
+        hipEventSynchronize(e1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        return ms / 5;
+    };
+    RegDev r = c->reg;
+    float best_e = 1e30f, best_et = 1e30f;
+    int be = 0, bet = 0;
+    if (fe >= 0 && fe < e_sched_count()) {
+        be = fe;
+    } else {
+        for (int v = 0; v < e_sched_count(); ++v) {
+            r.esched = v;
+            const float ms = time([&] {
+                launch_E(d, r, c->qs, d.nq, 0.0, 0.0, c->tune_buf, nf, 1, nullptr, c->s);
+            });
+            if (ms < best_e) { best_e = ms; be = v; }
+        }
+    }
+    if (fet >= 0 && fet < et_sched_count()) {
+        bet = fet;
+    } else {
+        for (int v = 0; v < et_sched_count(); ++v) {
+            r.etsched = v;
+            const float ms = time([&] {
+                launch_ET_image(d, r, c->Fb, nf, c->tune_buf, d.nq, 1, nullptr, c->etscale, c->s);
+            });
+            if (ms < best_et) { best_et = ms; bet = v; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->reg.esched = be;
+    c->reg.etsched = bet;
+    char line[200];
+    std::snprintf(line, sizeof line, "E schedule %s %.1fus; Eᵀ schedule %s %.1fus (with its zeroing), %d points",
+                  e_sched_name(be), best_e < 1e29f ? best_e * 1e3 : 0.0, et_sched_name(bet),
+                  best_et < 1e29f ? best_et * 1e3 : 0.0, c->nb);
+    c->tune_log.push_back(line);
+}
+
 // Algorithmic flops of a launch: 2MNK for the full contraction, 2·2·h·k·(data rows) for the
 // split one (two h×k parts; k = h pairs, or the khi - klo pairs a narrowed launch sums)
 double ibh::gemm_flops(const GemmArgs &g, int kind) {
@@ -1374,6 +1442,10 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     // moving bodies: points that were never checked (the description's body-frame points,
     // when no ibh_set_body_points came first) are rejected before anything is enqueued
     if (c->anymove && c->nb && !c->pts_ok) return check_points(c, c->hpts.data());
+    if (!c->reg_tuned && c->nb) {  // first step: the points are valid now
+        if (c->anymove) launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
+        tune_reg(c);
+    }
     if (lr_wanted(c) && !c->lr_valid) {  // first CG step, or the body box moved
         if (c->anymove) launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
         int rc0 = lr_build(c);
@@ -1674,6 +1746,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->reg.w, (size_t)nb * 4 * 13));
     c->reg.nb = nb;
     c->reg.err = c->derr;
+    c->reg.esched = c->reg.etsched = 0;
     CK(dalloc(c, &c->Fb, nf));
     CK(dalloc(c, &c->Ftmp, nf));
     CK(dalloc(c, &c->CD, D.nbodies + 1));
@@ -1777,6 +1850,7 @@ int ibh_setup(ibh_ctx *c) {
         launch_reg_update(c->d, c->reg, c->pts, c->h, c->x0, c->y0, c->s);
         int rc = check_flags(c);
         if (rc) return rc;
+        tune_reg(c);
         if (!c->use_cg) {
             rc = precompute_binv(c);
             if (rc) return rc;

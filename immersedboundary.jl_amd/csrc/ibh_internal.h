@@ -200,7 +200,20 @@ struct RegDev {
     int *bidx;     // 2·nb  (x index block, then y)
     double *w;     // nb × 4 × 13 : (wx_u, wy_u, wx_v, wy_v) 1-D delta factors
     int *err;      // device error flag (1: out of bounds, 2: non-finite flux)
+    int esched, etsched;  // block schedules of E and Eᵀ (host side; see e_sched_name)
 };
+// Block schedules of E and Eᵀ.  E: waves (outputs) per workgroup; Eᵀ: workgroup size and the
+// order in which a workgroup's threads walk the 2nb·169 stencil entries (entry-major: the 169
+// entries of one point together; point-major: one entry of consecutive points together).  Every
+// schedule gives the same bits: E forms each output in one wave in a fixed order, and Eᵀ's
+// fixed-point adds do not depend on their order.  Schedule 0 is the default.
+int e_sched_count();
+int et_sched_count();
+const char *e_sched_name(int v);
+const char *et_sched_name(int v);
+// "e,et" (IBH_REG_SCHED): schedules forced instead of timed; a missing or negative entry is left
+// to the tuner
+void parse_reg_sched(const char *s, int *e, int *et);
 void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double h, double x0,
                        double y0, hipStream_t s);
 // fb = E·(q + q0) where q0 = (u0, v0) constant (pass 0, 0 for plain E)
@@ -361,11 +374,13 @@ void launch_grp_lines_part(const Grp &g, hipStream_t s);
 void launch_grp_lines_colsum(const Grp &g, hipStream_t s);
 void launch_grp_lines_out(const Grp &g, hipStream_t s);
 void launch_grp_axpy_sub(const Grp &g, hipStream_t s);        // blocks: ceil(n/256)
-void launch_grp_E(const Grp &g, hipStream_t s);           // blocks: ceil(2nb/256)
+void launch_grp_E(const Grp &g, int sched, hipStream_t s);  // blocks: grp_E_blocks
 void launch_grp_ET_zero(const Grp &g, hipStream_t s);     // ETA; blocks: ceil(nq/256)
 void launch_grp_ET_scale(const Grp &g, hipStream_t s);    // ETA; blocks: 1
 void launch_grp_ET_zero_scale(const Grp &g, hipStream_t s); // ETA; blocks: ceil(nq/256) + 1
-void launch_grp_ET(const Grp &g, hipStream_t s);          // ETA; blocks: ceil(2nb·169/256)
+void launch_grp_ET(const Grp &g, int sched, hipStream_t s);  // ETA; blocks: grp_ET_blocks
+long grp_E_blocks(int nb, int sched);
+long grp_ET_blocks(int nb, int sched);
 void launch_grp_ET_fix(const Grp &g, hipStream_t s);      // ETA; blocks: ceil(nq/256)
 void launch_grp_gemv_part(const Grp &g, hipStream_t s);   // blocks: ceil(n/256)·ceil(n/64)
 void launch_grp_gemv_sum(const Grp &g, hipStream_t s);    // blocks: ceil(n/256)

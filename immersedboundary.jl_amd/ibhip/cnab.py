@@ -246,7 +246,8 @@ class DeviceCase:
         return out
 
     def tune_report(self) -> list:
-        """The GEMM autotuner's choices so far, one line per contraction shape."""
+        """The autotuner's choices so far: one line per contraction shape, and the block
+        schedules of E and Eᵀ."""
         buf = C.create_string_buffer(1 << 16)
         self._check(self.lib.ibh_tune_report(self.h, buf, len(buf)))
         return [ln for ln in buf.value.decode().splitlines() if ln]
