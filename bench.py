@@ -46,6 +46,26 @@ def measured_traffic(workload, precision):
         return None
 
 
+def by_shape(records, steps, peak, top=12):
+    """The profiled contraction launches grouped by shape (equal algorithmic flops and bytes):
+    launches per step, flops and bytes per launch, average device time, TFLOP/s and GB/s.
+    Summing flops·launches over the groups and dividing by the summed time gives `achieved`."""
+    us, fl, by = records
+    groups = {}
+    for u, f, b in zip(us, fl, by):
+        g = groups.setdefault((f, b), [0, 0.0])
+        g[0] += 1
+        g[1] += u
+    rows = []
+    for (f, b), (n, t) in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+        avg = t / n
+        rows.append({"launches_per_step": n / steps, "mflop_per_launch": f / 1e6,
+                     "mbyte_per_launch": b / 1e6, "avg_us": avg,
+                     "tflops": f / (avg * 1e-6) / 1e12, "frac_of_peak": f / (avg * 1e-6) / peak,
+                     "gbps": b / (avg * 1e-6) / 1e9, "share_of_time": t / max(sum(us), 1e-30)})
+    return rows[:top]
+
+
 def cpu_baseline(make_problem, n_ref_iters, cap=8, threads=None):
     """Time the oracle on one step of the workload with cg! capped at `cap` iterations and on
     two lone B-products; extrapolate the step to `n_ref_iters` iterations of the reference's
@@ -238,6 +258,7 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
     kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[precision]
     ms, launches, flops = batch.profile_read(kind)
     ev_ms, ev_launches, _ = batch.profile_read(kind, events=True)
+    records = batch.profile_launches(kind)
     batch.profile(False)
     launches_per_step = batch.launches_per_step()
     tune = batch.tune_report()
@@ -267,6 +288,7 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
             "event_avg_launch_us": 1e3 * ev_ms / max(ev_launches, 1),
             "launches": launches,
             "gemm_share_of_profiled_steps": (ms / 1e3) / prof_elapsed,
+            "by_shape": by_shape(records, profile_steps, PEAK[precision]),
             "timing": "device wall clock of each grouped contraction launch (first workgroup "
                       "start to last workgroup end) over a profiled pass of eager steps right "
                       "after the timed (graph) region; event_avg_launch_us: HIP events around "
@@ -430,6 +452,7 @@ def main():
     kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision]
     ms, launches, flops = case.profile_read(kind)
     ev_ms, ev_launches, _ = case.profile_read(kind, events=True)
+    records = case.profile_launches(kind)
     case.profile(False)
     log(f"bench: profiled pass {args.profile_steps} steps")
     steady = None
@@ -478,6 +501,7 @@ def main():
         "event_avg_launch_us": 1e3 * ev_ms / max(ev_launches, 1),
         "event_launches": ev_launches,
         "gemm_share_of_profiled_steps": (ms / 1e3) / prof_elapsed,
+        "by_shape": by_shape(records, args.profile_steps, peak),
         "timing": "device wall clock of each contraction launch (first workgroup start to last "
                   "workgroup end, as rocprofv3's kernel trace measures it) over a profiled pass "
                   "of eager steps right after the timed (graph) region; launches skipped on the "

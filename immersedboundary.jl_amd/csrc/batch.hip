@@ -47,7 +47,7 @@ struct Stage {
     std::vector<GemmArgs> gargs;
     int side = 0, epi = 0, kind = 0, prec = 0;
     int tune_ref = -1;         // index of the stage whose variant this one reuses
-    double flops = 0;
+    double flops = 0, bytes = 0;
     int variant = -1;
     // device copies
     const void *d_args = nullptr;
@@ -121,6 +121,7 @@ void add_gemm(std::vector<Stage> &v, const std::vector<GemmArgs> &gs, int kind, 
         }
         S.gargs.push_back(g);
         S.flops += gemm_flops(g, kind);
+        S.bytes += gemm_bytes(g, kind, epi, prec);
     }
     S.tune_ref = tune_ref;
     v.push_back(std::move(S));
@@ -663,7 +664,7 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
             launch_stage(b, S, S.variant, ts);
             hipEventRecord(P.pool[P.used + 1], b->s);
             P.used += 2;
-            P.rec.push_back({S.prec, S.flops});
+            P.rec.push_back({S.prec, S.flops, S.bytes});
         } else {
             launch_stage(b, S, S.variant);
         }
@@ -846,6 +847,12 @@ int ibh_batch_profile_read_events(ibh_batch *b, int32_t kind, double *ms, int64_
     BCK(hipStreamSynchronize(b->s));
     const int rc = prof_event_sum(b->prof, kind, ms, launches, flops);
     return rc ? bfail(b, rc, "reading the profiling events failed") : IBH_OK;
+}
+
+int ibh_batch_profile_launches(ibh_batch *b, int32_t kind, int32_t max, double *us,
+                               double *flops, double *bytes) {
+    if (!b || max < 0 || (max > 0 && (!us || !flops || !bytes))) return -IBH_EINVAL;
+    return prof_launches(b->prof, kind, b->s, max, us, flops, bytes);
 }
 
 int ibh_batch_tune_report(ibh_batch *b, char *buf, int32_t len) {

@@ -26,7 +26,12 @@ struct Prof {
     bool on = false;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
-    std::vector<std::pair<int, double>> rec;  // (kind, flops) per recorded launch
+    struct Rec {
+        int kind;       // precision of the launch
+        double flops;   // algorithmic flops
+        double bytes;   // algorithmic bytes: each operand read and each output written once
+    };
+    std::vector<Rec> rec;  // per recorded launch
     unsigned long long *ts = nullptr;  // device wall clock: PROF_SLOTS records of PROF_REC
 };
 
@@ -40,6 +45,10 @@ int prof_arm(Prof &P, hipStream_t s);
 unsigned long long *prof_slot(Prof &P);
 int prof_device_sum(const Prof &P, int kind, hipStream_t s, double *ms, int64_t *n, double *fl);
 int prof_event_sum(const Prof &P, int kind, double *ms, int64_t *n, double *fl);
+// per-launch records of one precision (device time in us, flops, bytes), skipped launches left
+// out; returns the number written (at most max)
+int prof_launches(const Prof &P, int kind, hipStream_t s, int max, double *us, double *fl,
+                  double *by);
 
 }  // namespace ibh
 
@@ -182,6 +191,10 @@ void lapinv_plan(ibh_ctx *c, const double *in, long sin, double *out, long sout,
 
 // Algorithmic flops of one split contraction (kind 1 left, 2 right), narrowing included
 double gemm_flops(const GemmArgs &g, int kind);
+// Algorithmic bytes of the same: the data operand over the summed pairs, the two sine parts
+// (in the operand precision prec), the output (both butterfly rows of each pair) and, for the
+// scaling and subtracting epilogues, the operand D; each once
+double gemm_bytes(const GemmArgs &g, int kind, int epi, int prec);
 
 // A forward pair whose second contraction sums only the pairs [klo, khi): the first forms only
 // the (128-aligned) rows those K tiles read

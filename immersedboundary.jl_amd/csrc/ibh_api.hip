@@ -99,14 +99,14 @@ int prof_device_sum(const Prof &P, int kind, hipStream_t s, double *ms, int64_t 
     double tot = 0, f = 0;
     int64_t cnt = 0;
     for (size_t k = 0; k < nr; ++k) {
-        if (P.rec[k].first != kind) continue;
+        if (P.rec[k].kind != kind) continue;
         const unsigned long long *r = h.data() + k * PROF_REC;
         unsigned long long b = 0;
         for (int j = 0; j < 16; ++j) b = std::max(b, r[16 + 16 * j]);
         const unsigned long long a = r[0];
         if (a == 0 || b == 0 || b < a) continue;  // skipped on the device
         tot += (double)(b - a) / khz;                  // ticks / kHz = ms
-        f += P.rec[k].second;
+        f += P.rec[k].flops;
         ++cnt;
     }
     if (ms) *ms = tot;
@@ -115,16 +115,46 @@ int prof_device_sum(const Prof &P, int kind, hipStream_t s, double *ms, int64_t 
     return IBH_OK;
 }
 
+int prof_launches(const Prof &P, int kind, hipStream_t s, int max, double *us, double *fl,
+                  double *by) {
+    const size_t nr = P.ts ? std::min(P.rec.size(), (size_t)PROF_SLOTS) : 0;
+    std::vector<unsigned long long> h(nr * PROF_REC + 1);
+    if (nr) {
+        if (hipMemcpyAsync(h.data(), P.ts, nr * PROF_REC * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return -IBH_EDEVICE;
+    }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        return -IBH_EDEVICE;
+    int cnt = 0;
+    for (size_t k = 0; k < nr && cnt < max; ++k) {
+        if (P.rec[k].kind != kind) continue;
+        const unsigned long long *r = h.data() + k * PROF_REC;
+        unsigned long long b = 0;
+        for (int j = 0; j < 16; ++j) b = std::max(b, r[16 + 16 * j]);
+        const unsigned long long a = r[0];
+        if (a == 0 || b == 0 || b < a) continue;  // skipped on the device
+        us[cnt] = (double)(b - a) * 1e3 / khz;
+        fl[cnt] = P.rec[k].flops;
+        by[cnt] = P.rec[k].bytes;
+        ++cnt;
+    }
+    return cnt;
+}
+
 int prof_event_sum(const Prof &P, int kind, double *ms, int64_t *n, double *fl) {
     double tot = 0, f = 0;
     int64_t cnt = 0;
     for (size_t k = 0; k < P.rec.size(); ++k) {
-        if (P.rec[k].first != kind) continue;
+        if (P.rec[k].kind != kind) continue;
         float e = 0;
         if (hipEventElapsedTime(&e, P.pool[2 * k], P.pool[2 * k + 1]) != hipSuccess)
             return IBH_EDEVICE;
         tot += e;
-        f += P.rec[k].second;
+        f += P.rec[k].flops;
         ++cnt;
     }
     if (ms) *ms = tot;
@@ -290,6 +320,20 @@ void tune_reg(ibh_ctx *c) {
 
 // Algorithmic flops of a launch: 2MNK for the full contraction, 2·2·h·k·(data rows) for the
 // split one (two h×k parts; k = h pairs, or the khi - klo pairs a narrowed launch sums)
+double ibh::gemm_bytes(const GemmArgs &g, int kind, int epi, int prec) {
+    if (kind == 0) return 8.0 * ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * g.batch;
+    const int ext[2] = {kind == 1 ? g.h : g.M, kind == 1 ? g.N : g.h};
+    double e[2];
+    for (int a = 0; a < 2; ++a)
+        e[a] = g.ohi[a] > 0 ? (double)(std::min(g.ohi[a], ext[a]) - g.olo[a]) : (double)ext[a];
+    const double pairs = kind == 1 ? e[0] : e[1], rows = kind == 1 ? e[1] : e[0];
+    const double kp = g.khi > 0 ? (double)(g.khi - g.klo) : (double)g.h;
+    const double sine = prec == 0 ? 8.0 : 4.0;  // FP64, or FP32 / packed bf16 pairs
+    const double out = 2.0 * pairs * rows * 8.0;
+    const double d = (kind == 2 && (epi == EPI_SCALE || epi == EPI_SUBFROM)) ? out : 0.0;
+    return (2.0 * kp * rows * 8.0 + out + d) * g.batch + 2.0 * pairs * kp * sine;
+}
+
 double ibh::gemm_flops(const GemmArgs &g, int kind) {
     if (kind == 0) return 2.0 * g.M * (double)g.N * g.K * g.batch;
     const int ext[2] = {kind == 1 ? g.h : g.M, kind == 1 ? g.N : g.h};
@@ -313,7 +357,7 @@ void gemm(ibh_ctx *c, GemmArgs g, int epi, int prec, bool tout = false, int kind
         hipEvent_t e1 = c->prof.pool[c->prof.used + 1];
         hipEventRecord(e1, c->s);
         c->prof.used += 2;
-        c->prof.rec.push_back({prec, gemm_flops(g, kind)});
+        c->prof.rec.push_back({prec, gemm_flops(g, kind), gemm_bytes(g, kind, epi, prec)});
     } else {
         launch_kind(g, prec, kind, v, epi, tout, c->s);
     }
@@ -2216,6 +2260,12 @@ int ibh_profile_read(ibh_ctx *c, int32_t kind, double *ms, int64_t *launches, do
     HIPCK(hipStreamSynchronize(c->s));
     const int rc = ibh::prof_device_sum(c->prof, kind, c->s, ms, launches, flops);
     return rc ? fail(c, rc, "reading the profiling timestamps failed") : IBH_OK;
+}
+
+int ibh_profile_launches(ibh_ctx *c, int32_t kind, int32_t max, double *us, double *flops,
+                         double *bytes) {
+    if (!c || max < 0 || (max > 0 && (!us || !flops || !bytes))) return -IBH_EINVAL;
+    return ibh::prof_launches(c->prof, kind, c->s, max, us, flops, bytes);
 }
 
 int ibh_profile_read_events(ibh_ctx *c, int32_t kind, double *ms, int64_t *launches,

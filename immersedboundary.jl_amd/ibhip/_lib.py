@@ -35,11 +35,11 @@ EXPORTED = [
     "ibh_op_rot", "ibh_op_curl", "ibh_op_coarsify", "ibh_op_get_bc", "ibh_op_apply_bc",
     "ibh_op_nonlinear", "ibh_op_vort2flux", "ibh_op_reg_update", "ibh_op_E", "ibh_op_ET",
     "ibh_op_B", "ibh_op_lapinv_noalloc", "ibh_profile_enable", "ibh_profile_read", "ibh_profile_read_events",
-    "ibh_tune_report",
+    "ibh_profile_launches", "ibh_tune_report",
     "ibh_batch_create", "ibh_batch_destroy", "ibh_batch_last_error", "ibh_batch_advance",
     "ibh_batch_advance_n", "ibh_batch_sync", "ibh_batch_launches_per_step",
     "ibh_batch_profile_enable", "ibh_batch_profile_read", "ibh_batch_profile_read_events",
-    "ibh_batch_tune_report",
+    "ibh_batch_profile_launches", "ibh_batch_tune_report",
 ]
 
 
@@ -121,6 +121,7 @@ def load():
     L.ibh_profile_enable.argtypes = [vp, C.c_int32]
     L.ibh_profile_read.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
     L.ibh_profile_read_events.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
+    L.ibh_profile_launches.argtypes = [vp, C.c_int32, C.c_int32, dp, dp, dp]
     L.ibh_tune_report.argtypes = [vp, C.c_char_p, C.c_int32]
     L.ibh_batch_create.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(vp)]
     L.ibh_batch_destroy.argtypes = [vp]
@@ -134,6 +135,7 @@ def load():
     L.ibh_batch_profile_enable.argtypes = [vp, C.c_int32]
     L.ibh_batch_profile_read.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
     L.ibh_batch_profile_read_events.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
+    L.ibh_batch_profile_launches.argtypes = [vp, C.c_int32, C.c_int32, dp, dp, dp]
     L.ibh_batch_tune_report.argtypes = [vp, C.c_char_p, C.c_int32]
     _lib = L
     return L

@@ -265,6 +265,20 @@ class DeviceCase:
         self._check(fn(self.h, kind, C.byref(ms), C.byref(n), C.byref(fl)))
         return ms.value, n.value, fl.value
 
+    def profile_launches(self, kind: int = 0):
+        """Per-launch device time (us), algorithmic flops and bytes of the profiled contraction
+        launches of one precision, in launch order (skipped launches left out)."""
+        return _launch_records(self.lib.ibh_profile_launches, self.h, kind, self._check)
+
+
+def _launch_records(fn, h, kind, check, max_launches=8192):
+    """Per-launch (us, flops, bytes) arrays of the profiled contraction launches."""
+    us, fl, by = (np.zeros(max_launches) for _ in range(3))
+    n = fn(h, kind, max_launches, dptr(us), dptr(fl), dptr(by))
+    if n < 0:
+        check(-n)
+    return us[:n], fl[:n], by[:n]
+
 
 # ------------------------------------------------------------------------------ state / solver
 class Quantities:
@@ -446,6 +460,10 @@ class Batch:
         fn = self.lib.ibh_batch_profile_read_events if events else self.lib.ibh_batch_profile_read
         self._check(fn(self.h, kind, C.byref(ms), C.byref(n), C.byref(fl)))
         return ms.value, n.value, fl.value
+
+    def profile_launches(self, kind: int = 0):
+        """Per-launch (us, flops, bytes) of the profiled grouped contraction launches."""
+        return _launch_records(self.lib.ibh_batch_profile_launches, self.h, kind, self._check)
 
     def tune_report(self) -> list:
         buf = C.create_string_buffer(1 << 16)

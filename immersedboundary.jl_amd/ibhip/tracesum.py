@@ -11,8 +11,8 @@ one are its profiled eager pass.
         per-kernel counter averages over the timed window.  With GRBM_GUI_ACTIVE and
         SQ_VALU_MFMA_BUSY_CYCLES: the MFMA-busy fraction, busy cycles over (GRBM_GUI_ACTIVE / 8
         XCDs) x 1024 SIMDs.  With FETCH_SIZE / WRITE_SIZE (KiB): HBM bytes per dispatch, each
-        scaled by what the counter reads for bench.py --calib-copy's 256 MiB copy (the largest
-        reading before the timed window) against the 256 MiB it moves.
+        scaled by what the counter reads for bench.py --calib-copy's 256 MiB copy (torch's
+        multiply kernel before the timed window) against the 256 MiB it moves.
 """
 import argparse
 import collections
@@ -82,14 +82,15 @@ def pmc(path, steps, match):
     order = sorted(disp)
     rows = [meta[d] for d in order]
     a, b = window(rows)
-    pre = order[:a]
+    # the calibration copy: torch's multiply kernel before the window (bench.py --calib-copy)
+    pre = [d for d in order[:a] if "MulFunctor" in meta[d]["Kernel_Name"]]
     scale = {}
     for cn in ("FETCH_SIZE", "WRITE_SIZE"):
         vals = [disp[d].get(cn, 0.0) for d in pre]
         if vals and max(vals) > 0:
             scale[cn] = CALIB_BYTES / 1024 / max(vals)
     if scale:
-        print("calibration (256 MiB copy before the window): " +
+        print("calibration (the 256 MiB copy before the window): " +
               ", ".join(f"{k} x{v:.3f}" for k, v in scale.items()) + "\n")
     agg = collections.defaultdict(lambda: collections.defaultdict(float))
     cnt = collections.Counter()

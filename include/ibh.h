@@ -227,6 +227,9 @@ int ibh_batch_profile_read(ibh_batch *batch, int32_t kind, double *ms, int64_t *
 int ibh_batch_profile_read_events(ibh_batch *batch, int32_t kind, double *ms, int64_t *launches,
                                   double *flops);
 int ibh_batch_tune_report(ibh_batch *batch, char *buf, int32_t len);
+/* per-launch records of the grouped contraction launches, as ibh_profile_launches */
+int ibh_batch_profile_launches(ibh_batch *batch, int32_t kind, int32_t max, double *us,
+                               double *flops, double *bytes);
 
 /* ---- profiling of the sine-transform contraction launches (no reference counterpart) --- */
 int ibh_profile_enable(ibh_ctx *ctx, int32_t on);
@@ -243,6 +246,12 @@ int ibh_profile_read(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches, 
  * skipped launches. */
 int ibh_profile_read_events(ibh_ctx *ctx, int32_t kind, double *ms, int64_t *launches,
                             double *flops);
+/* Per-launch records since the last enable, launches of precision `kind` in order (skipped
+ * ones left out): device time (us, as ibh_profile_read), algorithmic flops and algorithmic
+ * bytes (the data operand over the summed pairs, both sine parts, the output and an epilogue's
+ * operand, each once).  Returns the number written (at most max), or -IBH_E... on error. */
+int ibh_profile_launches(ibh_ctx *ctx, int32_t kind, int32_t max, double *us, double *flops,
+                         double *bytes);
 
 #ifdef __cplusplus
 }
