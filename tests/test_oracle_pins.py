@@ -25,6 +25,9 @@ transposed operand fails:
                            q_ℓ = curl!(ψ_ℓ, bc) divergence-free
   update_bodies!           r = r0 + R(θ)xb (= r0 + xb at θ = 0), v = dr/dt by central
                            differences, fixed bodies unchanged
+  update_stress!           two mirror-image ellipses (y ↦ −y) in the symmetric domain get
+                           equal CD and opposite CL, each away from zero (per-body slices of
+                           Fb and the x/y component order)
   B, Binv                  B·Binv = I; B·z equals the composition of the pinned operators
   full step                no-slip after projection: E(q₁ + q₀) = 0 for fixed bodies
                            (= v·h for moving ones); uniform flow without bodies stays exact
@@ -471,3 +474,26 @@ def test_body_points_kinematics():
             _, vt = pb.body_points(t)
             assert np.abs((rp - rm) / (2 * eps) - vt).max() < 1e-8
     assert fr is plate.frame and npl == plate.npoints
+
+
+def test_update_stress_mirror_bodies():
+    # update_stress! (streamfcn/states.jl:53-84) sums each body's slice of Fb: a mirror pair
+    # in the y-symmetric domain must get equal drag and opposite lift.  A wrong body offset or
+    # swapped force components breaks the symmetry.
+    from ibhip.problem import GridFrame, RigidBody
+    from ibhip.workloads import ellipse
+
+    n = 64
+    h = 4.0 / n
+    upper = ellipse((0.2, 0.7), 0.4, 0.15, 0.5, h)
+    lower = RigidBody(GridFrame(), upper.xb * np.array([1.0, -1.0]), upper.ds)
+    grid = UniformGrid(h, (-1.0, 3.0), (-2.0, 2.0))
+    prob = Problem(StreamFcnFluid(MultiLevel(grid, 2), (1.0, 0.0), 100.0),
+                   BodyGroup(upper, lower), SchemeCNAB(0.2 * h, force_solver="dense"))
+    o = ibref.OracleRun(prob)
+    for k in range(3):
+        o.advance(k * prob.scheme.dt)
+    cd, cl = np.asarray(o.CD()), np.asarray(o.CL())
+    assert cd.shape == (2,) and cd[0] > 0 and abs(cl[0]) > 1e-3 * cd[0]
+    assert abs(cd[0] - cd[1]) < 1e-9 * cd[0]
+    assert abs(cl[0] + cl[1]) < 1e-9 * cd[0]
