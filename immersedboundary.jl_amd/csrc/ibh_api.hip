@@ -802,6 +802,10 @@ int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64, int 
     const double per_col = 8.0 * (2.0 * d.nq + d.ng * (1.0 + 2.0 * c->L + 3.0));
     int nc = (int)std::max(1.0, std::min(512.0, 12.0e9 / per_col));
     nc = std::min(nc, cnt);
+    // the columns [j0, j0 + cnt) of the nf×nf matrix, and the unit-column grid of a batch
+    // (nf·nc entries) within 32-bit workgroup counts
+    if (j0 < 0 || cnt < 0 || j0 + cnt > nf || (long)nf * nc >= (1L << 31))
+        return fail(c, IBH_EINVAL, "B_columns: column range outside the force unknowns");
     int rc = ensure_B_work(c, nc);
     if (rc) return rc;
     for (int j = j0; j < j0 + cnt; j += nc) {
