@@ -47,6 +47,7 @@ struct Stage {
     std::vector<GemmArgs> gargs;
     int side = 0, epi = 0, kind = 0, prec = 0;
     int tune_ref = -1;         // index of the stage whose variant this one reuses
+    bool dma_ok = true;        // every problem fits the LDS-direct variants' 32-bit offsets
     double flops = 0, bytes = 0;
     int variant = -1;
     // device copies
@@ -120,6 +121,7 @@ void add_gemm(std::vector<Stage> &v, const std::vector<GemmArgs> &gs, int kind, 
             std::abort();
         }
         S.gargs.push_back(g);
+        S.dma_ok = S.dma_ok && dma_fits(g, kind - 1);
         S.flops += gemm_flops(g, kind);
         S.bytes += gemm_bytes(g, kind, epi, prec);
     }
@@ -598,6 +600,7 @@ int tune_stage(ibh_batch *b, std::vector<Stage> &st, size_t k) {
     int best = 0;
     float best_ms = 1e30f;
     for (int v = 0; v < nv; ++v) {
+        if (gemm && !S.dma_ok && gemm_eo_grp_variant_dma(S.prec, v)) continue;
         run(v);
         BCK(hipEventRecord(e0, b->s));
         for (int r = 0; r < 3; ++r) run(v);
@@ -633,7 +636,8 @@ int run_stages(ibh_batch *b, std::vector<Stage> &v, bool may_tune) {
     for (size_t k = 0; k < v.size(); ++k) {
         Stage &S = v[k];
         if (S.op == OP_GEMM && S.variant < 0) {
-            if (b->tune_fixed >= 0 && b->tune_fixed < gemm_eo_grp_variant_count(S.prec))
+            if (b->tune_fixed >= 0 && b->tune_fixed < gemm_eo_grp_variant_count(S.prec) &&
+                (S.dma_ok || !gemm_eo_grp_variant_dma(S.prec, b->tune_fixed)))
                 S.variant = b->tune_fixed;
             else if (S.tune_ref >= 0) S.variant = v[S.tune_ref].variant;
             else if (may_tune && S.epi != EPI_SUBFROM) {  // re-running it would subtract again

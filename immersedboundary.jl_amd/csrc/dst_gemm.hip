@@ -138,6 +138,13 @@ __global__ void k_sine_low(const double *S, float *f, unsigned *b, long n) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
+__global__ void k_sine_interleave(const double *So, const double *Se, double *Soe, long n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Soe[2 * t] = So[t];
+    Soe[2 * t + 1] = Se[t];
+}
+
 template <class P, int ROWS, int BK>
 __device__ inline void fload(Frag<P> &f, const Tile<P, ROWS, BK> &t, int row, int kk, int lq) {
     f.load(t.s[row], kk, lq);
@@ -621,6 +628,303 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo(const GemmArgs *g
     ts_mark(ts, true, blockIdx.x);
 }
 
+// ------------------------------------------------------------------------ LDS-direct split tiles
+// The split contraction of eo_tile (FP64) with its operand tiles staged by buffer loads that
+// write LDS directly (buffer_load ... lds): nothing passes through registers, two LDS stages
+// alternate, and the loads of K tile t+1 are in flight while tile t is multiplied.
+// * The sine parts come interleaved (GemmArgs::Soe, 16 bytes per pair) with 16-byte loads into
+//   XOR-swizzled slots: pair k of tile row r sits in slot k ^ (r & 15) of its row, so the 16 rows
+//   one fragment read touches fall in distinct LDS banks.
+// * Data rows have odd lengths (not 16-byte aligned), so they are staged by 4-byte loads, one
+//   256-byte run per wave instruction, into rows padded by 16 bytes; one 16-byte read gives
+//   (x[2k], x[2k+1]) for the O and the E part.
+// * Out-of-range elements (rows past the operand, pairs past h, and x[K], which the last pair
+//   would take from the next row) are loaded from an offset past the buffer's size, which the
+//   buffer load returns as zero: the zeros eo_tile stores.
+// * Fragments are read from LDS two K steps ahead of the products that use them.
+// Each part's products run over the same k groups in the same order as eo_tile, so the results
+// are bitwise those of every other split variant.  WS = 2 runs the O and E parts on separate
+// waves (twice the resident waves); the E waves hand their sums over through LDS.
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// buffer resource of `bytes` bytes at p (raw, stride 0; the third dword selects 32-bit data,
+// which raw loads need to return data at all): offsets at or past `bytes` read as zero.
+// Arguments are made wave-uniform, as the resource lives in scalar registers.
+__device__ inline __amdgpu_buffer_rsrc_t wave_rsrc(const void *p, long bytes) {
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const int nb = __builtin_amdgcn_readfirstlane((int)bytes);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), 0, nb,
+                                             0x00020000);
+}
+constexpr unsigned DMA_OOB = 0x80000000u;  // an offset past every operand's size
+
+// the split contraction's store of o ± e for pair index pi and data row di (eo_tile's epilogue)
+template <int EPI, bool TOUT, int SIDE>
+__device__ __forceinline__ void eo_store(const GemmArgs &g, double *__restrict__ C, int bz, int pi,
+                                         int di, double o, double e) {
+    const int Kd = g.K;
+    for (int side2 = 0; side2 < 2; ++side2) {
+        const int pr = side2 == 0 ? pi : Kd - 1 - pi;
+        if (side2 == 1 && pr == pi) break;
+        double v = side2 == 0 ? o + e : o - e;
+        const int m = SIDE == 0 ? pr : di, n = SIDE == 0 ? di : pr;
+        if (TOUT) {
+            C[n + g.ldc * (long)m] = v;
+            continue;
+        }
+        const long ci = m + g.ldc * (long)n;
+        if (EPI == EPI_STORE) {
+            C[ci] = v;
+        } else if (EPI == EPI_SCALE) {
+            C[ci] = v * g.D[bz * g.sD + m + g.ldd * (long)n];
+        } else if (EPI == EPI_SUBFROM) {
+            C[ci] = g.D[bz * g.sD + m + g.ldd * (long)n] - v;
+        } else {
+            const double *N1 = g.N1 + bz * g.sC, *N2 = g.N2 + bz * g.sC;
+            v = v + g.f1 * N1[ci];
+            v = v + g.f2 * N2[ci];
+            const double *bc = g.bcacc + (long)bz * g.nbc;
+            double eb = 0.0;
+            if (n == 0) eb += bc[g.Bo + 1 + m];
+            if (n == g.N - 1) eb += bc[g.To + 1 + m];
+            if (m == 0) eb += bc[g.Lo + 1 + n];
+            if (m == g.M - 1) eb += bc[g.Ro + 1 + n];
+            C[ci] = v + eb;
+        }
+    }
+}
+
+__device__ inline void dma_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+// workgroup barrier ordering LDS only (the staged loads are waited for by dma_wait_all)
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int SIDE, int WS>
+struct DmaCfg {
+    static constexpr int NW = WM * WN, NWT = NW * WS;            // waves per part, in all
+    static constexpr int RS = SIDE == 0 ? BM : BN;              // sine rows (pairs)
+    static constexpr int RD = SIDE == 0 ? BN : BM;              // data rows
+    static constexpr int TP = SIDE == 0 ? BM / WM : BN / WN;    // a wave's pairs
+    static constexpr int TD = SIDE == 0 ? BN / WN : BM / WM;    // a wave's data rows
+    static constexpr int FP = TP / 16, FD = TD / 16;
+    static constexpr int PITCH = 2 * BK * 8 + 16;               // data row pitch (bytes)
+    static constexpr int SBYTES = RS * BK * 16, DBYTES = RD * PITCH, STAGE = SBYTES + DBYTES;
+    static constexpr int SI = RS * BK / 64;                     // 16-byte wave loads (sine)
+    static constexpr int SEGS = 2 * BK / 32;                    // 256-byte runs per data row
+    static constexpr int DI = RD * SEGS;                        // 4-byte wave loads (data)
+    static constexpr int NSI = (SI + NWT - 1) / NWT, NDI = (DI + NWT - 1) / NWT;
+    static_assert(BK >= 16 && BK <= 64 && (BK & (BK - 1)) == 0, "K tile of 16..64 pairs");
+    static_assert(TP % 16 == 0 && TD % 16 == 0, "wave tile");
+    static_assert(SI * 64 == RS * BK && NWT % SEGS == 0, "load split");
+    static_assert(WS == 1 || NW * FP * FD * 4 * 64 * 8 <= STAGE, "hand-over fits one stage");
+};
+
+template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+__device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt, int bz) {
+    using Q = DmaCfg<BM, BN, BK, WM, WN, SIDE, WS>;
+    constexpr int FP = Q::FP, FD = Q::FD;
+    const int h = g.h, Kd = g.K;
+    const double *X = SIDE == 0 ? g.B + bz * g.sB : g.A + bz * g.sA;
+    const long ldx = SIDE == 0 ? g.ldb : g.lda;
+    double *__restrict__ C = g.C + bz * g.sC;
+    const int m0 = bxt * BM, n0 = byt * BN;
+    const int r0s = SIDE == 0 ? m0 : n0, r0d = SIDE == 0 ? n0 : m0;
+    const int nrd = SIDE == 0 ? g.N : g.M;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int part = WS == 2 ? wave / Q::NW : 0, wl = wave % Q::NW;
+    const int wm = wl % WM, wn = wl / WM;
+    const int wp = SIDE == 0 ? wm : wn, wd = SIDE == 0 ? wn : wm;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    __shared__ __attribute__((aligned(16))) char st0[Q::STAGE];
+    __shared__ __attribute__((aligned(16))) char st1[Q::STAGE];
+
+    const __amdgpu_buffer_rsrc_t rs = wave_rsrc(g.Soe, (long)h * h * 16);
+    const __amdgpu_buffer_rsrc_t rd = wave_rsrc(X, (ldx * (long)(nrd - 1) + Kd) * 8);
+    // byte offsets of this lane's elements in K tile 0 (DMA_OOB for rows past the operand)
+    int sbase[Q::NSI], sk[Q::NSI];
+#pragma unroll
+    for (int u = 0; u < Q::NSI; ++u) {
+        const int L = (u * Q::NWT + wave) * 64 + lane;
+        const int row = L / BK, k = (L % BK) ^ (row & 15);
+        sk[u] = k;
+        sbase[u] = r0s + row < h ? (k + h * (r0s + row)) * 16 : (int)DMA_OOB;
+    }
+    int dbase[Q::NDI];
+    const int de0 = (wave % Q::SEGS) * 32 + (lane >> 1);  // element of the tile row
+#pragma unroll
+    for (int v = 0; v < Q::NDI; ++v) {
+        const int row = (v * Q::NWT + wave) / Q::SEGS;
+        dbase[v] = r0d + row < nrd ? (int)((ldx * (r0d + row) + de0) * 8 + (lane & 1) * 4)
+                                   : (int)DMA_OOB;
+    }
+    auto issue = [&](char *stage, int kt) {
+        const int kp = kt * BK;  // first pair of the tile
+        const bool full = kp + BK <= h && 2 * (kp + BK) <= Kd;
+        const bool eok = 2 * kp + de0 < Kd;
+#pragma unroll
+        for (int u = 0; u < Q::NSI; ++u) {
+            const int inst = u * Q::NWT + wave;
+            if (Q::SI % Q::NWT != 0 && inst >= Q::SI) break;
+            const unsigned off = full || (sbase[u] != (int)DMA_OOB && kp + sk[u] < h)
+                                     ? (unsigned)sbase[u] + kp * 16 : DMA_OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(stage + inst * 1024), 16, off,
+                                                     0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < Q::NDI; ++v) {
+            const int inst = v * Q::NWT + wave;
+            if (Q::DI % Q::NWT != 0 && inst >= Q::DI) break;
+            const int row = inst / Q::SEGS, seg = inst % Q::SEGS;
+            const unsigned off = full || (dbase[v] != (int)DMA_OOB && eok)
+                                     ? (unsigned)dbase[v] + kp * 16 : DMA_OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rd, (lds_void *)(stage + Q::SBYTES + row * Q::PITCH + seg * 256), 4, off, 0, 0, 0);
+        }
+    };
+
+    // WS = 1: the O and E sums; WS = 2: this wave's part in accO (accE unused)
+    d4 accO[FP][FD], accE[FP][FD];
+#pragma unroll
+    for (int i = 0; i < FP; ++i)
+#pragma unroll
+        for (int j = 0; j < FD; ++j) accO[i][j] = accE[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // fragment of K step kk: the sine pair (So, Se) of row i, the data pair (x even, x odd)
+    auto ldf = [&](const char *stage, int kk, d2 *sf, d2 *df) {
+        const int k = 4 * kk + lq;
+#pragma unroll
+        for (int i = 0; i < FP; ++i) {
+            const int row = wp * Q::TP + i * 16 + l16;
+            sf[i] = *(const d2 *)(stage + (row * BK + (k ^ (row & 15))) * 16);
+        }
+#pragma unroll
+        for (int j = 0; j < FD; ++j) {
+            const int row = wd * Q::TD + j * 16 + l16;
+            df[j] = *(const d2 *)(stage + Q::SBYTES + row * Q::PITCH + k * 16);
+        }
+    };
+    auto mm = [&](const d2 *sf, const d2 *df) {
+#pragma unroll
+        for (int i = 0; i < FP; ++i)
+#pragma unroll
+            for (int j = 0; j < FD; ++j) {
+                if (WS == 2) {
+                    accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(part ? sf[i].y : sf[i].x,
+                                                                      part ? df[j].y : df[j].x,
+                                                                      accO[i][j], 0, 0, 0);
+                } else {
+                    accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[i].x, df[j].x, accO[i][j], 0, 0, 0);
+                    accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[i].y, df[j].y, accE[i][j], 0, 0, 0);
+                }
+            }
+    };
+    auto compute = [&](const char *stage) {
+        d2 sf[3][FP], df[3][FD];
+        ldf(stage, 0, sf[0], df[0]);
+        ldf(stage, 1, sf[1], df[1]);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            if (kk + 2 < BK / 4) ldf(stage, kk + 2, sf[(kk + 2) % 3], df[(kk + 2) % 3]);
+            mm(sf[kk % 3], df[kk % 3]);
+        }
+        // the reads of step kk+2 are issued ahead of the products of step kk
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 * (FP + FD), 0);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            if (kk + 2 < BK / 4) __builtin_amdgcn_sched_group_barrier(0x100, FP + FD, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, (WS == 1 ? 2 : 1) * FP * FD, 0);
+        }
+    };
+
+    const int nkall = (h + BK - 1) / BK;
+    const int kt0 = g.khi > 0 ? g.klo / BK : 0;
+    const int nk = g.khi > 0 ? min(nkall, (g.khi + BK - 1) / BK) - kt0 : nkall;
+    if (nk > 0) {
+        issue(st0, kt0);
+        for (int t = 0; t < nk; t += 2) {
+            dma_wait_all();  // tile t has landed (this wave's loads) ...
+            lds_barrier();   // ... for every wave, and every wave is done with tile t-1
+            if (t + 1 < nk) issue(st1, kt0 + t + 1);
+            compute(st0);
+            if (t + 1 >= nk) break;
+            dma_wait_all();
+            lds_barrier();
+            if (t + 2 < nk) issue(st0, kt0 + t + 2);
+            compute(st1);
+        }
+    }
+
+    double *xch = reinterpret_cast<double *>(st0);
+    if (WS == 2) {
+        // no load is in flight (the last tile's were waited for); every wave is past its products
+        lds_barrier();
+        if (part == 1) {
+#pragma unroll
+            for (int i = 0; i < FP; ++i)
+#pragma unroll
+                for (int j = 0; j < FD; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        xch[(((wl * FP + i) * FD + j) * 4 + r) * 64 + lane] = accO[i][j][r];
+        }
+        lds_barrier();
+        if (part == 1) return;
+    }
+#pragma unroll
+    for (int i = 0; i < FP; ++i)
+#pragma unroll
+        for (int j = 0; j < FD; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int pi = r0s + wp * Q::TP + i * 16 + lq + 4 * r;  // pair index a''
+                const int di = r0d + wd * Q::TD + j * 16 + l16;         // data row
+                if (pi >= h || di >= nrd) continue;
+                const double e = WS == 2 ? xch[(((wl * FP + i) * FD + j) * 4 + r) * 64 + lane]
+                                         : accE[i][j][r];
+                eo_store<EPI, TOUT, SIDE>(g, C, bz, pi, di, accO[i][j][r], e);
+            }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+__global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo_dma(GemmArgs g) {
+    if (g.skip && *g.skip) return;
+    ts_mark(g.ts, false, lin_block());
+    int bxt, byt, bz;
+    xcd_tile(bxt, byt, bz);
+    bxt += g.olo[0] / BM;
+    byt += g.olo[1] / BN;
+    eo_tile_dma<BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, bxt, byt, bz);
+    ts_mark(g.ts, true, lin_block());
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+__global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo_dma(const GemmArgs *gs, const int *off,
+                                                                   int n, unsigned long long *ts) {
+    ts_mark(ts, false, blockIdx.x);
+    const int total = gridDim.x, q = total / 8, r = total % 8;
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int T = xcd * q + min(xcd, r) + slot;
+    const int p = grp_find(off, n, T);
+    const GemmArgs g = gs[p];
+    const int local = T - off[p];
+    const int rows = SIDE == 0 ? g.h : g.M, cols = SIDE == 0 ? g.N : g.h;
+    const int x0 = g.ohi[0] > 0 ? g.olo[0] / BM : 0, y0 = g.ohi[1] > 0 ? g.olo[1] / BN : 0;
+    const int gx = ((g.ohi[0] > 0 ? min(g.ohi[0], rows) : rows) + BM - 1) / BM - x0;
+    const int gy = ((g.ohi[1] > 0 ? min(g.ohi[1], cols) : cols) + BN - 1) / BN - y0;
+    eo_tile_dma<BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, x0 + local % gx, y0 + (local / gx) % gy,
+                                                        local / (gx * gy));
+    ts_mark(ts, true, blockIdx.x);
+}
+
 // ------------------------------------------------------------------------ variant tables
 typedef void (*KFn)(GemmArgs);
 struct Variant {
@@ -685,6 +989,7 @@ struct VariantEO {
     int bm, bn, nthreads;
     KFn fn[2][4];
     const char *name;
+    int dma;  // operand tiles staged by LDS-direct loads (32-bit buffer offsets)
 };
 #define EOVW(P, BM, BN, BK, WM, WN, NB, WS, SUF)                                             \
     {BM, BN, WM * WN * 64 * WS,                                                                  \
@@ -693,10 +998,20 @@ struct VariantEO {
        k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB, WS>,                            \
        k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB, WS>,                          \
        k_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB, WS>}},                            \
-     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF}
+     "eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF, 0}
 #define EOV(P, BM, BN, BK, WM, WN, NB) EOVW(P, BM, BN, BK, WM, WN, NB, 1, "")
 // odd/even parts on separate waves (WS = 2)
 #define EOVS(P, BM, BN, BK, WM, WN, NB) EOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
+
+// LDS-direct FP64 variants (eo_tile_dma); WS = 2 puts the two parts on separate waves
+#define EODMA(BM, BN, BK, WM, WN, WS, SUF)                                                         \
+    {BM, BN, WM * WN * 64 * WS,                                                                     \
+     {{k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},       \
+      {k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                                  \
+       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                                  \
+       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                                \
+       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                                  \
+     "eo-dma PF64 " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
 
 static const VariantEO EF64[] = {
     EOV(PF64, 64, 64, 16, 2, 2, 2), EOV(PF64, 32, 64, 16, 1, 2, 2), EOV(PF64, 64, 32, 16, 2, 1, 2),
@@ -718,6 +1033,13 @@ static const VariantEO EF64[] = {
     // three and four register stages (loads issued three / four K tiles ahead)
     EOVS(PF64, 32, 32, 32, 2, 2, 4), EOVS(PF64, 32, 32, 32, 2, 2, 5), EOVS(PF64, 32, 32, 16, 2, 2, 4),
     EOVS(PF64, 32, 32, 16, 2, 2, 5), EOVS(PF64, 32, 64, 16, 2, 2, 4), EOVS(PF64, 64, 32, 16, 2, 2, 4),
+    // operand tiles staged by LDS-direct loads
+    EODMA(32, 32, 16, 2, 2, 1, ""), EODMA(64, 32, 16, 2, 2, 1, ""), EODMA(32, 64, 16, 2, 2, 1, ""),
+    EODMA(32, 32, 32, 2, 2, 1, ""), EODMA(32, 32, 16, 2, 2, 2, " oe-waves"),
+    EODMA(32, 32, 32, 2, 2, 2, " oe-waves"), EODMA(32, 16, 16, 2, 1, 2, " oe-waves"),
+    EODMA(16, 32, 16, 1, 2, 2, " oe-waves"), EODMA(32, 16, 32, 2, 1, 2, " oe-waves"),
+    EODMA(16, 32, 32, 1, 2, 2, " oe-waves"), EODMA(64, 32, 16, 4, 2, 2, " oe-waves"),
+    EODMA(32, 64, 16, 2, 4, 2, " oe-waves"),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
@@ -756,6 +1078,17 @@ void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t
     hipLaunchKernelGGL(k_sine_low, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, f, b, n);
 }
 
+bool dma_fits(const GemmArgs &g, int side) {
+    const long ldx = side == 0 ? g.ldb : g.lda, rows = side == 0 ? g.N : g.M;
+    const long data = (ldx * (rows - 1) + g.K) * 8, sine = (long)g.h * g.h * 16;
+    return g.Soe && data < (1L << 31) && sine < (1L << 31);
+}
+
+void launch_sine_interleave(const double *So, const double *Se, double *Soe, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_sine_interleave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, So, Se,
+                       Soe, n);
+}
+
 void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi, hipStream_t s) {
     if (!sine_ready(g, prec)) {
         std::fprintf(stderr, "libibhip: split contraction without its precision's sine parts\n");
@@ -763,7 +1096,8 @@ void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi,
     }
     int n;
     const VariantEO *t = table_eo(prec, &n);
-    const VariantEO &V = t[(variant >= 0 && variant < n) ? variant : 0];
+    if (variant < 0 || variant >= n || (t[variant].dma && !dma_fits(g, side))) variant = 0;
+    const VariantEO &V = t[variant];
     const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
     const int ex = g.ohi[0] > 0 ? std::min(g.ohi[0], gm) : gm;
     const int ey = g.ohi[1] > 0 ? std::min(g.ohi[1], gn) : gn;
@@ -783,6 +1117,7 @@ struct VariantGEO {
     int bm, bn, nthreads;
     KGFn fn[2][4];
     const char *name;
+    int dma;
 };
 #define GEOVW(P, BM, BN, BK, WM, WN, NB, WS, SUF)                                             \
     {BM, BN, WM * WN * 64 * WS,                                                                   \
@@ -791,9 +1126,18 @@ struct VariantGEO {
        kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, NB, WS>,                            \
        kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, NB, WS>,                          \
        kg_gemm_eo<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, NB, WS>}},                            \
-     "grouped eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF}
+     "grouped eo " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN " buf" #NB SUF, 0}
 #define GEOV(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 1, "")
 #define GEOVS(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
+
+#define GEODMA(BM, BN, BK, WM, WN, WS, SUF)                                                        \
+    {BM, BN, WM * WN * 64 * WS,                                                                     \
+     {{kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},      \
+      {kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                                 \
+       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                                 \
+       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                               \
+       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                                 \
+     "grouped eo-dma PF64 " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
 
 static const VariantGEO GF64[] = {
     GEOV(PF64, 64, 64, 16, 2, 2, 1), GEOV(PF64, 32, 64, 16, 2, 2, 1), GEOV(PF64, 64, 32, 16, 2, 2, 1),
@@ -802,6 +1146,10 @@ static const VariantGEO GF64[] = {
     GEOV(PF64, 32, 32, 64, 2, 2, 1), GEOV(PF64, 32, 64, 64, 2, 2, 1),
     GEOVS(PF64, 32, 32, 16, 2, 2, 1), GEOVS(PF64, 32, 64, 16, 2, 2, 1),
     GEOVS(PF64, 32, 32, 32, 2, 2, 4), GEOVS(PF64, 32, 64, 16, 2, 2, 4),
+    GEODMA(32, 32, 16, 2, 2, 1, ""), GEODMA(64, 32, 16, 2, 2, 1, ""), GEODMA(32, 64, 16, 2, 2, 1, ""),
+    GEODMA(32, 32, 16, 2, 2, 2, " oe-waves"), GEODMA(64, 32, 16, 4, 2, 2, " oe-waves"),
+    GEODMA(32, 64, 16, 2, 4, 2, " oe-waves"), GEODMA(32, 16, 32, 2, 1, 2, " oe-waves"),
+    GEODMA(16, 32, 32, 1, 2, 2, " oe-waves"),
 };
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),
@@ -821,6 +1169,12 @@ static const VariantGEO *table_geo(int prec, int *n) {
     if (prec == 2) { *n = sizeof(GBF3) / sizeof(GBF3[0]); return GBF3; }
     *n = sizeof(GF64) / sizeof(GF64[0]);
     return GF64;
+}
+
+bool gemm_eo_grp_variant_dma(int prec, int v) {
+    int n;
+    const VariantGEO *t = table_geo(prec, &n);
+    return v >= 0 && v < n && t[v].dma;
 }
 
 int gemm_eo_grp_variant_count(int prec) {

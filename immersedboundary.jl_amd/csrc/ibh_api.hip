@@ -380,6 +380,7 @@ GemmArgs gargs() {
 static void set_sine(GemmArgs &g, const ibh_ctx *c, bool y) {
     g.So = y ? c->Sop : c->Som;
     g.Se = y ? c->Sep : c->Sem;
+    g.Soe = y ? c->Spoe : c->Smoe;
     g.Sof = y ? c->Sopf : c->Somf;
     g.Sef = y ? c->Sepf : c->Semf;
     g.Sob = y ? c->Sopb : c->Somb;
@@ -1736,6 +1737,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         CK(dalloc(c, &c->Semf, hx * hx));
         CK(dalloc(c, &c->Somb, hx * hx));
         CK(dalloc(c, &c->Semb, hx * hx));
+        CK(dalloc(c, &c->Smoe, 2 * hx * hx));
+        c->Spoe = c->Smoe;
         c->Sop = c->Som;
         c->Sep = c->Sem;
         c->Sopf = c->Somf;
@@ -1749,6 +1752,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             CK(dalloc(c, &c->Sepf, hy * hy));
             CK(dalloc(c, &c->Sopb, hy * hy));
             CK(dalloc(c, &c->Sepb, hy * hy));
+            CK(dalloc(c, &c->Spoe, 2 * hy * hy));
         }
         if (const char *e = std::getenv("IBH_DST_SPLIT")) c->use_eo = std::atoi(e) != 0;
     }
@@ -1823,6 +1827,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         const long hx = D.nx / 2, hy = D.ny / 2;
         launch_sine_low(c->Som, c->Somf, c->Somb, hx * hx, c->s);
         launch_sine_low(c->Sem, c->Semf, c->Semb, hx * hx, c->s);
+        launch_sine_interleave(c->Som, c->Sem, c->Smoe, hx * hx, c->s);
+        if (c->Spoe != c->Smoe) launch_sine_interleave(c->Sop, c->Sep, c->Spoe, hy * hy, c->s);
         if (c->Sop != c->Som) {
             launch_sine_low(c->Sop, c->Sopf, c->Sopb, hy * hy, c->s);
             launch_sine_low(c->Sep, c->Sepf, c->Sepb, hy * hy, c->s);

@@ -54,6 +54,9 @@ struct GemmArgs {
     // pairs (Sob, Seb: the rounded value in the low 16 bits, the rounded remainder in the high)
     const float *Sof, *Sef;
     const unsigned *Sob, *Seb;
+    // the FP64 parts interleaved, Soe[2(k + h·a)] = So[a][k], Soe[2(k + h·a) + 1] = Se[a][k]
+    // (one 16-byte element per pair: the tiles staged by LDS-direct loads read both at once)
+    const double *Soe;
     int h;
     int batch;
     // even/odd split: only pair indices [klo, khi) of the data's contiguous dimension are
@@ -159,9 +162,11 @@ void launch_sine_matrix(double *S, int n, hipStream_t s);  // S[i,k] = sin(π(i+
 void launch_sine_split(double *So, double *Se, int n, hipStream_t s);
 // FP32 and bf16-pair copies of n sine elements, rounded as the contraction rounds FP64 operands
 void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t s);
+// Soe[2t] = So[t], Soe[2t + 1] = Se[t] for t < n (the interleaved parts of GemmArgs::Soe)
+void launch_sine_interleave(const double *So, const double *Se, double *Soe, long n, hipStream_t s);
 // a split contraction in precision prec (0 FP64, 1 FP32, 2 bf16x3) has its sine parts
 inline bool sine_ready(const GemmArgs &g, int prec) {
-    return prec == 0 ? g.So && g.Se : prec == 1 ? g.Sof && g.Sef : g.Sob && g.Seb;
+    return prec == 0 ? g.So && g.Se && g.Soe : prec == 1 ? g.Sof && g.Sef : g.Sob && g.Seb;
 }
 void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
                          double dt, double Re, hipStream_t s);
@@ -392,6 +397,11 @@ void launch_grp_stress(const Grp &g, hipStream_t s);      // blocks: nbodies
 void launch_gemm_eo_grp(const Grp &g, int prec, int variant, int side, int epi, hipStream_t s,
                         unsigned long long *ts = nullptr);
 int gemm_eo_grp_variant_count(int prec);
+// whether grouped variant v stages its tiles by LDS-direct loads (32-bit offsets: dma_fits)
+bool gemm_eo_grp_variant_dma(int prec, int v);
+// the LDS-direct split tiles address the sine parts and one batch entry's data operand by
+// 32-bit byte offsets; false if either exceeds them (or the interleaved parts are missing)
+bool dma_fits(const GemmArgs &g, int side);
 const char *gemm_eo_grp_variant_name(int prec, int v);
 long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side);
 
