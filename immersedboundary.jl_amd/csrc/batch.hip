@@ -121,7 +121,7 @@ void add_gemm(std::vector<Stage> &v, const std::vector<GemmArgs> &gs, int kind, 
             std::abort();
         }
         S.gargs.push_back(g);
-        S.dma_ok = S.dma_ok && dma_fits(g, kind - 1);
+        S.dma_ok = S.dma_ok && dma_fits(g, kind - 1, prec);
         S.flops += gemm_flops(g, kind);
         S.bytes += gemm_bytes(g, kind, epi, prec);
     }

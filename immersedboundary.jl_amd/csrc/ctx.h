@@ -72,6 +72,7 @@ struct ibh_ctx {
     // odd/even parts of the sine matrices for the split contraction (x and y dimensions)
     double *Som = nullptr, *Sem = nullptr, *Sop = nullptr, *Sep = nullptr;
     double *Smoe = nullptr, *Spoe = nullptr;  // the FP64 parts interleaved (GemmArgs::Soe)
+    float *Smoef = nullptr, *Spoef = nullptr;  // the FP32 parts interleaved (GemmArgs::Soef)
     // the split parts rounded for the FP32 and bf16x3 contractions
     float *Somf = nullptr, *Semf = nullptr, *Sopf = nullptr, *Sepf = nullptr;
     unsigned *Somb = nullptr, *Semb = nullptr, *Sopb = nullptr, *Sepb = nullptr;

@@ -138,6 +138,12 @@ __global__ void k_sine_low(const double *S, float *f, unsigned *b, long n) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
+__global__ void k_sine_interleave_f(const float *So, const float *Se, float *Soe, long n) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Soe[2 * t] = So[t];
+    Soe[2 * t + 1] = Se[t];
+}
 __global__ void k_sine_interleave(const double *So, const double *Se, double *Soe, long n) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -706,7 +712,26 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int SIDE, int WS>
+// the interleaved sine parts of a precision: SB bytes per pair, (So, Se) of one element type
+template <class P> struct DmaSine;
+template <> struct DmaSine<PF64> {
+    static constexpr int SB = 16;
+    typedef d2 T;
+    __device__ static const void *ptr(const GemmArgs &g) { return g.Soe; }
+};
+template <> struct DmaSine<PF32> {
+    static constexpr int SB = 8;
+    typedef float2 T;
+    __device__ static const void *ptr(const GemmArgs &g) { return g.Soef; }
+};
+__device__ inline d4 mfma16(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma16(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class P, int BM, int BN, int BK, int WM, int WN, int SIDE, int WS>
 struct DmaCfg {
     static constexpr int NW = WM * WN, NWT = NW * WS;            // waves per part, in all
     static constexpr int RS = SIDE == 0 ? BM : BN;              // sine rows (pairs)
@@ -714,22 +739,29 @@ struct DmaCfg {
     static constexpr int TP = SIDE == 0 ? BM / WM : BN / WN;    // a wave's pairs
     static constexpr int TD = SIDE == 0 ? BN / WN : BM / WM;    // a wave's data rows
     static constexpr int FP = TP / 16, FD = TD / 16;
+    static constexpr int SB = DmaSine<P>::SB;                   // sine bytes per pair
+    static constexpr int PPS = 16 / SB;                         // pairs per 16-byte slot
+    static constexpr int SPR = BK / PPS;                        // slots per sine row
+    static constexpr int SWM = (SPR < 16 ? SPR : 16) - 1;       // slot swizzle mask
     static constexpr int PITCH = 2 * BK * 8 + 16;               // data row pitch (bytes)
-    static constexpr int SBYTES = RS * BK * 16, DBYTES = RD * PITCH, STAGE = SBYTES + DBYTES;
-    static constexpr int SI = RS * BK / 64;                     // 16-byte wave loads (sine)
+    static constexpr int SBYTES = RS * SPR * 16, DBYTES = RD * PITCH, STAGE = SBYTES + DBYTES;
+    static constexpr int SI = RS * SPR / 64;                    // 16-byte wave loads (sine)
     static constexpr int SEGS = 2 * BK / 32;                    // 256-byte runs per data row
     static constexpr int DI = RD * SEGS;                        // 4-byte wave loads (data)
     static constexpr int NSI = (SI + NWT - 1) / NWT, NDI = (DI + NWT - 1) / NWT;
     static_assert(BK >= 16 && BK <= 64 && (BK & (BK - 1)) == 0, "K tile of 16..64 pairs");
+    static_assert(SPR >= 8, "eight slots per sine row for the swizzle");
     static_assert(TP % 16 == 0 && TD % 16 == 0, "wave tile");
-    static_assert(SI * 64 == RS * BK && NWT % SEGS == 0, "load split");
+    static_assert(SI * 64 == RS * SPR && NWT % SEGS == 0, "load split");
     static_assert(WS == 1 || NW * FP * FD * 4 * 64 * 8 <= STAGE, "hand-over fits one stage");
 };
 
-template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
 __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt, int bz) {
-    using Q = DmaCfg<BM, BN, BK, WM, WN, SIDE, WS>;
-    constexpr int FP = Q::FP, FD = Q::FD;
+    using Q = DmaCfg<P, BM, BN, BK, WM, WN, SIDE, WS>;
+    using ST = typename DmaSine<P>::T;
+    typedef typename P::E E;
+    constexpr int FP = Q::FP, FD = Q::FD, SB = Q::SB, PPS = Q::PPS;
     const int h = g.h, Kd = g.K;
     const double *X = SIDE == 0 ? g.B + bz * g.sB : g.A + bz * g.sA;
     const long ldx = SIDE == 0 ? g.ldb : g.lda;
@@ -747,16 +779,17 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
     __shared__ __attribute__((aligned(16))) char st0[Q::STAGE];
     __shared__ __attribute__((aligned(16))) char st1[Q::STAGE];
 
-    const __amdgpu_buffer_rsrc_t rs = wave_rsrc(g.Soe, (long)h * h * 16);
+    const __amdgpu_buffer_rsrc_t rs = wave_rsrc(DmaSine<P>::ptr(g), (long)h * h * SB);
     const __amdgpu_buffer_rsrc_t rd = wave_rsrc(X, (ldx * (long)(nrd - 1) + Kd) * 8);
-    // byte offsets of this lane's elements in K tile 0 (DMA_OOB for rows past the operand)
+    // byte offsets of this lane's elements in K tile 0 (DMA_OOB for rows past the operand):
+    // sine slot L of the tile holds pairs from k = PPS·((L % SPR) ^ (row & SWM)) of its row
     int sbase[Q::NSI], sk[Q::NSI];
 #pragma unroll
     for (int u = 0; u < Q::NSI; ++u) {
         const int L = (u * Q::NWT + wave) * 64 + lane;
-        const int row = L / BK, k = (L % BK) ^ (row & 15);
+        const int row = L / Q::SPR, k = PPS * ((L % Q::SPR) ^ (row & Q::SWM));
         sk[u] = k;
-        sbase[u] = r0s + row < h ? (k + h * (r0s + row)) * 16 : (int)DMA_OOB;
+        sbase[u] = r0s + row < h ? (k + h * (r0s + row)) * SB : (int)DMA_OOB;
     }
     int dbase[Q::NDI];
     const int de0 = (wave % Q::SEGS) * 32 + (lane >> 1);  // element of the tile row
@@ -774,8 +807,9 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
         for (int u = 0; u < Q::NSI; ++u) {
             const int inst = u * Q::NWT + wave;
             if (Q::SI % Q::NWT != 0 && inst >= Q::SI) break;
+            // h is even, so a slot's pairs are in range together
             const unsigned off = full || (sbase[u] != (int)DMA_OOB && kp + sk[u] < h)
-                                     ? (unsigned)sbase[u] + kp * 16 : DMA_OOB;
+                                     ? (unsigned)sbase[u] + kp * SB : DMA_OOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(stage + inst * 1024), 16, off,
                                                      0, 0, 0);
         }
@@ -792,43 +826,46 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
     };
 
     // WS = 1: the O and E sums; WS = 2: this wave's part in accO (accE unused)
-    d4 accO[FP][FD], accE[FP][FD];
+    typename P::Acc accO[FP][FD], accE[FP][FD];
 #pragma unroll
     for (int i = 0; i < FP; ++i)
 #pragma unroll
-        for (int j = 0; j < FD; ++j) accO[i][j] = accE[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FD; ++j) accO[i][j] = accE[i][j] = P::zero();
 
-    // fragment of K step kk: the sine pair (So, Se) of row i, the data pair (x even, x odd)
-    auto ldf = [&](const char *stage, int kk, d2 *sf, d2 *df) {
+    // fragment of K step kk: the sine pair (So, Se) of row i, the data pair (x even, x odd) in
+    // the operand type (FP32 rounds the data here, as eo_tile's tile store does)
+    auto ldf = [&](const char *stage, int kk, ST *sf, E (*df)[2]) {
         const int k = 4 * kk + lq;
 #pragma unroll
         for (int i = 0; i < FP; ++i) {
             const int row = wp * Q::TP + i * 16 + l16;
-            sf[i] = *(const d2 *)(stage + (row * BK + (k ^ (row & 15))) * 16);
+            sf[i] = *(const ST *)(stage + (row * Q::SPR + ((k / PPS) ^ (row & Q::SWM))) * 16 +
+                                  (k % PPS) * SB);
         }
 #pragma unroll
         for (int j = 0; j < FD; ++j) {
             const int row = wd * Q::TD + j * 16 + l16;
-            df[j] = *(const d2 *)(stage + Q::SBYTES + row * Q::PITCH + k * 16);
+            const d2 x = *(const d2 *)(stage + Q::SBYTES + row * Q::PITCH + k * 16);
+            df[j][0] = (E)x.x;
+            df[j][1] = (E)x.y;
         }
     };
-    auto mm = [&](const d2 *sf, const d2 *df) {
+    auto mm = [&](const ST *sf, const E (*df)[2]) {
 #pragma unroll
         for (int i = 0; i < FP; ++i)
 #pragma unroll
             for (int j = 0; j < FD; ++j) {
                 if (WS == 2) {
-                    accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(part ? sf[i].y : sf[i].x,
-                                                                      part ? df[j].y : df[j].x,
-                                                                      accO[i][j], 0, 0, 0);
+                    accO[i][j] = mfma16(part ? sf[i].y : sf[i].x, df[j][part], accO[i][j]);
                 } else {
-                    accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[i].x, df[j].x, accO[i][j], 0, 0, 0);
-                    accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[i].y, df[j].y, accE[i][j], 0, 0, 0);
+                    accO[i][j] = mfma16(sf[i].x, df[j][0], accO[i][j]);
+                    accE[i][j] = mfma16(sf[i].y, df[j][1], accE[i][j]);
                 }
             }
     };
     auto compute = [&](const char *stage) {
-        d2 sf[3][FP], df[3][FD];
+        ST sf[3][FP];
+        E df[3][FD][2];
         ldf(stage, 0, sf[0], df[0]);
         ldf(stage, 1, sf[1], df[1]);
 #pragma unroll
@@ -874,7 +911,7 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
                 for (int j = 0; j < FD; ++j)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        xch[(((wl * FP + i) * FD + j) * 4 + r) * 64 + lane] = accO[i][j][r];
+                        xch[(((wl * FP + i) * FD + j) * 4 + r) * 64 + lane] = (double)accO[i][j][r];
         }
         lds_barrier();
         if (part == 1) return;
@@ -885,16 +922,16 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
         for (int j = 0; j < FD; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int pi = r0s + wp * Q::TP + i * 16 + lq + 4 * r;  // pair index a''
-                const int di = r0d + wd * Q::TD + j * 16 + l16;         // data row
+                const int pi = r0s + wp * Q::TP + i * 16 + P::drow(lq, r);  // pair index a''
+                const int di = r0d + wd * Q::TD + j * 16 + l16;             // data row
                 if (pi >= h || di >= nrd) continue;
                 const double e = WS == 2 ? xch[(((wl * FP + i) * FD + j) * 4 + r) * 64 + lane]
-                                         : accE[i][j][r];
-                eo_store<EPI, TOUT, SIDE>(g, C, bz, pi, di, accO[i][j][r], e);
+                                         : (double)accE[i][j][r];
+                eo_store<EPI, TOUT, SIDE>(g, C, bz, pi, di, (double)accO[i][j][r], e);
             }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
 __global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo_dma(GemmArgs g) {
     if (g.skip && *g.skip) return;
     ts_mark(g.ts, false, lin_block());
@@ -902,11 +939,11 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void k_gemm_eo_dma(GemmArgs g) {
     xcd_tile(bxt, byt, bz);
     bxt += g.olo[0] / BM;
     byt += g.olo[1] / BN;
-    eo_tile_dma<BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, bxt, byt, bz);
+    eo_tile_dma<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, bxt, byt, bz);
     ts_mark(g.ts, true, lin_block());
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
+template <class P, int BM, int BN, int BK, int WM, int WN, int EPI, bool TOUT, int SIDE, int WS>
 __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo_dma(const GemmArgs *gs, const int *off,
                                                                    int n, unsigned long long *ts) {
     ts_mark(ts, false, blockIdx.x);
@@ -920,7 +957,7 @@ __global__ __launch_bounds__(WM *WN * 64 * WS) void kg_gemm_eo_dma(const GemmArg
     const int x0 = g.ohi[0] > 0 ? g.olo[0] / BM : 0, y0 = g.ohi[1] > 0 ? g.olo[1] / BN : 0;
     const int gx = ((g.ohi[0] > 0 ? min(g.ohi[0], rows) : rows) + BM - 1) / BM - x0;
     const int gy = ((g.ohi[1] > 0 ? min(g.ohi[1], cols) : cols) + BN - 1) / BN - y0;
-    eo_tile_dma<BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, x0 + local % gx, y0 + (local / gx) % gy,
+    eo_tile_dma<P, BM, BN, BK, WM, WN, EPI, TOUT, SIDE, WS>(g, x0 + local % gx, y0 + (local / gx) % gy,
                                                         local / (gx * gy));
     ts_mark(ts, true, blockIdx.x);
 }
@@ -1003,15 +1040,15 @@ struct VariantEO {
 // odd/even parts on separate waves (WS = 2)
 #define EOVS(P, BM, BN, BK, WM, WN, NB) EOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
 
-// LDS-direct FP64 variants (eo_tile_dma); WS = 2 puts the two parts on separate waves
-#define EODMA(BM, BN, BK, WM, WN, WS, SUF)                                                         \
+// LDS-direct variants (eo_tile_dma); WS = 2 puts the two parts on separate waves
+#define EODMA(P, BM, BN, BK, WM, WN, WS, SUF)                                                      \
     {BM, BN, WM * WN * 64 * WS,                                                                     \
-     {{k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},       \
-      {k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                                  \
-       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                                  \
-       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                                \
-       k_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                                  \
-     "eo-dma PF64 " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
+     {{k_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},    \
+      {k_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                               \
+       k_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                               \
+       k_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                             \
+       k_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                               \
+     "eo-dma " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
 
 static const VariantEO EF64[] = {
     EOV(PF64, 64, 64, 16, 2, 2, 2), EOV(PF64, 32, 64, 16, 1, 2, 2), EOV(PF64, 64, 32, 16, 2, 1, 2),
@@ -1034,12 +1071,12 @@ static const VariantEO EF64[] = {
     EOVS(PF64, 32, 32, 32, 2, 2, 4), EOVS(PF64, 32, 32, 32, 2, 2, 5), EOVS(PF64, 32, 32, 16, 2, 2, 4),
     EOVS(PF64, 32, 32, 16, 2, 2, 5), EOVS(PF64, 32, 64, 16, 2, 2, 4), EOVS(PF64, 64, 32, 16, 2, 2, 4),
     // operand tiles staged by LDS-direct loads
-    EODMA(32, 32, 16, 2, 2, 1, ""), EODMA(64, 32, 16, 2, 2, 1, ""), EODMA(32, 64, 16, 2, 2, 1, ""),
-    EODMA(32, 32, 32, 2, 2, 1, ""), EODMA(32, 32, 16, 2, 2, 2, " oe-waves"),
-    EODMA(32, 32, 32, 2, 2, 2, " oe-waves"), EODMA(32, 16, 16, 2, 1, 2, " oe-waves"),
-    EODMA(16, 32, 16, 1, 2, 2, " oe-waves"), EODMA(32, 16, 32, 2, 1, 2, " oe-waves"),
-    EODMA(16, 32, 32, 1, 2, 2, " oe-waves"), EODMA(64, 32, 16, 4, 2, 2, " oe-waves"),
-    EODMA(32, 64, 16, 2, 4, 2, " oe-waves"),
+    EODMA(PF64, 32, 32, 16, 2, 2, 1, ""), EODMA(PF64, 64, 32, 16, 2, 2, 1, ""), EODMA(PF64, 32, 64, 16, 2, 2, 1, ""),
+    EODMA(PF64, 32, 32, 32, 2, 2, 1, ""), EODMA(PF64, 32, 32, 16, 2, 2, 2, " oe-waves"),
+    EODMA(PF64, 32, 32, 32, 2, 2, 2, " oe-waves"), EODMA(PF64, 32, 16, 16, 2, 1, 2, " oe-waves"),
+    EODMA(PF64, 16, 32, 16, 1, 2, 2, " oe-waves"), EODMA(PF64, 32, 16, 32, 2, 1, 2, " oe-waves"),
+    EODMA(PF64, 16, 32, 32, 1, 2, 2, " oe-waves"), EODMA(PF64, 64, 32, 16, 4, 2, 2, " oe-waves"),
+    EODMA(PF64, 32, 64, 16, 2, 4, 2, " oe-waves"),
 };
 static const VariantEO EF32[] = {
     EOV(PF32, 64, 64, 16, 2, 2, 2), EOV(PF32, 32, 64, 32, 1, 2, 2), EOV(PF32, 64, 32, 32, 2, 1, 2),
@@ -1047,6 +1084,11 @@ static const VariantEO EF32[] = {
     EOV(PF32, 64, 32, 16, 2, 2, 1), EOV(PF32, 64, 64, 32, 2, 2, 1),
     EOV(PF32, 32, 64, 32, 2, 2, 3), EOV(PF32, 64, 64, 32, 2, 2, 3),
     EOVS(PF32, 32, 64, 16, 2, 2, 1), EOVS(PF32, 64, 32, 16, 2, 2, 1),
+    EODMA(PF32, 32, 32, 16, 2, 2, 1, ""), EODMA(PF32, 64, 32, 16, 2, 2, 1, ""),
+    EODMA(PF32, 32, 64, 16, 2, 2, 1, ""), EODMA(PF32, 64, 64, 16, 2, 2, 1, ""),
+    EODMA(PF32, 32, 32, 16, 2, 2, 2, " oe-waves"), EODMA(PF32, 32, 16, 32, 2, 1, 2, " oe-waves"),
+    EODMA(PF32, 16, 32, 32, 1, 2, 2, " oe-waves"), EODMA(PF32, 64, 32, 16, 4, 2, 2, " oe-waves"),
+    EODMA(PF32, 32, 64, 16, 2, 4, 2, " oe-waves"),
 };
 static const VariantEO EBF3[] = {
     EOV(PBF3, 64, 64, 32, 2, 2, 2), EOV(PBF3, 128, 64, 32, 2, 2, 2), EOV(PBF3, 64, 128, 32, 2, 2, 2),
@@ -1078,10 +1120,16 @@ void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t
     hipLaunchKernelGGL(k_sine_low, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, f, b, n);
 }
 
-bool dma_fits(const GemmArgs &g, int side) {
+bool dma_fits(const GemmArgs &g, int side, int prec) {
     const long ldx = side == 0 ? g.ldb : g.lda, rows = side == 0 ? g.N : g.M;
-    const long data = (ldx * (rows - 1) + g.K) * 8, sine = (long)g.h * g.h * 16;
-    return g.Soe && data < (1L << 31) && sine < (1L << 31);
+    const long data = (ldx * (rows - 1) + g.K) * 8, sine = (long)g.h * g.h * (prec == 0 ? 16 : 8);
+    return (prec == 0 ? g.Soe != nullptr : prec == 1 ? g.Soef != nullptr : false) &&
+           data < (1L << 31) && sine < (1L << 31);
+}
+
+void launch_sine_interleave(const float *So, const float *Se, float *Soe, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_sine_interleave_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, So,
+                       Se, Soe, n);
 }
 
 void launch_sine_interleave(const double *So, const double *Se, double *Soe, long n, hipStream_t s) {
@@ -1096,7 +1144,7 @@ void launch_gemm_eo(const GemmArgs &g, int prec, int variant, int side, int epi,
     }
     int n;
     const VariantEO *t = table_eo(prec, &n);
-    if (variant < 0 || variant >= n || (t[variant].dma && !dma_fits(g, side))) variant = 0;
+    if (variant < 0 || variant >= n || (t[variant].dma && !dma_fits(g, side, prec))) variant = 0;
     const VariantEO &V = t[variant];
     const int gm = side == 0 ? g.h : g.M, gn = side == 0 ? g.N : g.h;
     const int ex = g.ohi[0] > 0 ? std::min(g.ohi[0], gm) : gm;
@@ -1130,14 +1178,14 @@ struct VariantGEO {
 #define GEOV(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 1, "")
 #define GEOVS(P, BM, BN, BK, WM, WN, NB) GEOVW(P, BM, BN, BK, WM, WN, NB, 2, " oe-waves")
 
-#define GEODMA(BM, BN, BK, WM, WN, WS, SUF)                                                        \
+#define GEODMA(P, BM, BN, BK, WM, WN, WS, SUF)                                                     \
     {BM, BN, WM * WN * 64 * WS,                                                                     \
-     {{kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},      \
-      {kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                                 \
-       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                                 \
-       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                               \
-       kg_gemm_eo_dma<BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                                 \
-     "grouped eo-dma PF64 " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
+     {{kg_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_STORE, true, 0, WS>, nullptr, nullptr, nullptr},   \
+      {kg_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_STORE, false, 1, WS>,                              \
+       kg_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_SCALE, false, 1, WS>,                              \
+       kg_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_SUBFROM, false, 1, WS>,                            \
+       kg_gemm_eo_dma<P, BM, BN, BK, WM, WN, EPI_RHS, false, 1, WS>}},                              \
+     "grouped eo-dma " #P " " #BM "x" #BN "x" #BK " " #WM "x" #WN SUF, 1}
 
 static const VariantGEO GF64[] = {
     GEOV(PF64, 64, 64, 16, 2, 2, 1), GEOV(PF64, 32, 64, 16, 2, 2, 1), GEOV(PF64, 64, 32, 16, 2, 2, 1),
@@ -1146,16 +1194,20 @@ static const VariantGEO GF64[] = {
     GEOV(PF64, 32, 32, 64, 2, 2, 1), GEOV(PF64, 32, 64, 64, 2, 2, 1),
     GEOVS(PF64, 32, 32, 16, 2, 2, 1), GEOVS(PF64, 32, 64, 16, 2, 2, 1),
     GEOVS(PF64, 32, 32, 32, 2, 2, 4), GEOVS(PF64, 32, 64, 16, 2, 2, 4),
-    GEODMA(32, 32, 16, 2, 2, 1, ""), GEODMA(64, 32, 16, 2, 2, 1, ""), GEODMA(32, 64, 16, 2, 2, 1, ""),
-    GEODMA(32, 32, 16, 2, 2, 2, " oe-waves"), GEODMA(64, 32, 16, 4, 2, 2, " oe-waves"),
-    GEODMA(32, 64, 16, 2, 4, 2, " oe-waves"), GEODMA(32, 16, 32, 2, 1, 2, " oe-waves"),
-    GEODMA(16, 32, 32, 1, 2, 2, " oe-waves"),
+    GEODMA(PF64, 32, 32, 16, 2, 2, 1, ""), GEODMA(PF64, 64, 32, 16, 2, 2, 1, ""), GEODMA(PF64, 32, 64, 16, 2, 2, 1, ""),
+    GEODMA(PF64, 32, 32, 16, 2, 2, 2, " oe-waves"), GEODMA(PF64, 64, 32, 16, 4, 2, 2, " oe-waves"),
+    GEODMA(PF64, 32, 64, 16, 2, 4, 2, " oe-waves"), GEODMA(PF64, 32, 16, 32, 2, 1, 2, " oe-waves"),
+    GEODMA(PF64, 16, 32, 32, 1, 2, 2, " oe-waves"),
 };
 static const VariantGEO GF32[] = {
     GEOV(PF32, 64, 64, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 1), GEOV(PF32, 32, 64, 16, 2, 2, 1),
     GEOV(PF32, 64, 32, 16, 2, 2, 1), GEOV(PF32, 64, 64, 32, 2, 2, 3),
     GEOVS(PF32, 32, 32, 16, 2, 2, 1), GEOVS(PF32, 32, 64, 16, 2, 2, 1), GEOVS(PF32, 64, 64, 16, 2, 2, 1),
     GEOVS(PF32, 32, 32, 32, 2, 2, 4), GEOVS(PF32, 32, 64, 32, 2, 2, 3),
+    GEODMA(PF32, 32, 32, 16, 2, 2, 1, ""), GEODMA(PF32, 64, 32, 16, 2, 2, 1, ""),
+    GEODMA(PF32, 32, 64, 16, 2, 2, 1, ""), GEODMA(PF32, 64, 64, 16, 2, 2, 1, ""),
+    GEODMA(PF32, 32, 32, 16, 2, 2, 2, " oe-waves"), GEODMA(PF32, 64, 32, 16, 4, 2, 2, " oe-waves"),
+    GEODMA(PF32, 32, 64, 16, 2, 4, 2, " oe-waves"),
 };
 static const VariantGEO GBF3[] = {
     GEOV(PBF3, 64, 64, 32, 2, 2, 1), GEOV(PBF3, 128, 64, 32, 2, 2, 1), GEOV(PBF3, 64, 128, 32, 2, 2, 1),

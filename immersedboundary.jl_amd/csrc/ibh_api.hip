@@ -381,6 +381,7 @@ static void set_sine(GemmArgs &g, const ibh_ctx *c, bool y) {
     g.So = y ? c->Sop : c->Som;
     g.Se = y ? c->Sep : c->Sem;
     g.Soe = y ? c->Spoe : c->Smoe;
+    g.Soef = y ? c->Spoef : c->Smoef;
     g.Sof = y ? c->Sopf : c->Somf;
     g.Sef = y ? c->Sepf : c->Semf;
     g.Sob = y ? c->Sopb : c->Somb;
@@ -1738,7 +1739,9 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         CK(dalloc(c, &c->Somb, hx * hx));
         CK(dalloc(c, &c->Semb, hx * hx));
         CK(dalloc(c, &c->Smoe, 2 * hx * hx));
+        CK(dalloc(c, &c->Smoef, 2 * hx * hx));
         c->Spoe = c->Smoe;
+        c->Spoef = c->Smoef;
         c->Sop = c->Som;
         c->Sep = c->Sem;
         c->Sopf = c->Somf;
@@ -1753,6 +1756,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             CK(dalloc(c, &c->Sopb, hy * hy));
             CK(dalloc(c, &c->Sepb, hy * hy));
             CK(dalloc(c, &c->Spoe, 2 * hy * hy));
+            CK(dalloc(c, &c->Spoef, 2 * hy * hy));
         }
         if (const char *e = std::getenv("IBH_DST_SPLIT")) c->use_eo = std::atoi(e) != 0;
     }
@@ -1833,6 +1837,8 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             launch_sine_low(c->Sop, c->Sopf, c->Sopb, hy * hy, c->s);
             launch_sine_low(c->Sep, c->Sepf, c->Sepb, hy * hy, c->s);
         }
+        launch_sine_interleave(c->Somf, c->Semf, c->Smoef, hx * hx, c->s);
+        if (c->Spoef != c->Smoef) launch_sine_interleave(c->Sopf, c->Sepf, c->Spoef, hy * hy, c->s);
     }
     launch_scale_tables(d, c->sD, c->sA, c->sAinv, c->L, c->h, c->dt, c->Re, c->s);
     std::vector<double> fac(L + 1, 0.0);

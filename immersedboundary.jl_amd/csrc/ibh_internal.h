@@ -57,6 +57,7 @@ struct GemmArgs {
     // the FP64 parts interleaved, Soe[2(k + h·a)] = So[a][k], Soe[2(k + h·a) + 1] = Se[a][k]
     // (one 16-byte element per pair: the tiles staged by LDS-direct loads read both at once)
     const double *Soe;
+    const float *Soef;  // the FP32 parts interleaved likewise
     int h;
     int batch;
     // even/odd split: only pair indices [klo, khi) of the data's contiguous dimension are
@@ -164,9 +165,10 @@ void launch_sine_split(double *So, double *Se, int n, hipStream_t s);
 void launch_sine_low(const double *S, float *f, unsigned *b, long n, hipStream_t s);
 // Soe[2t] = So[t], Soe[2t + 1] = Se[t] for t < n (the interleaved parts of GemmArgs::Soe)
 void launch_sine_interleave(const double *So, const double *Se, double *Soe, long n, hipStream_t s);
+void launch_sine_interleave(const float *So, const float *Se, float *Soe, long n, hipStream_t s);
 // a split contraction in precision prec (0 FP64, 1 FP32, 2 bf16x3) has its sine parts
 inline bool sine_ready(const GemmArgs &g, int prec) {
-    return prec == 0 ? g.So && g.Se && g.Soe : prec == 1 ? g.Sof && g.Sef : g.Sob && g.Seb;
+    return prec == 0 ? g.So && g.Se && g.Soe : prec == 1 ? g.Sof && g.Sef && g.Soef : g.Sob && g.Seb;
 }
 void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, int L, double h,
                          double dt, double Re, hipStream_t s);
@@ -401,7 +403,7 @@ int gemm_eo_grp_variant_count(int prec);
 bool gemm_eo_grp_variant_dma(int prec, int v);
 // the LDS-direct split tiles address the sine parts and one batch entry's data operand by
 // 32-bit byte offsets; false if either exceeds them (or the interleaved parts are missing)
-bool dma_fits(const GemmArgs &g, int side);
+bool dma_fits(const GemmArgs &g, int side, int prec);
 const char *gemm_eo_grp_variant_name(int prec, int v);
 long grouped_eo_tiles(const GemmArgs &g, int prec, int variant, int side);
 

@@ -140,7 +140,7 @@ def test_batch_graph_replay_matches_eager(monkeypatch):
             assert rel(a, b) < 1e-12
 
 
-@pytest.mark.parametrize("precision,nvar", [("fp64", 23), ("fp32", 10), ("bf16x3", 10)])
+@pytest.mark.parametrize("precision,nvar", [("fp64", 23), ("fp32", 17), ("bf16x3", 10)])
 def test_batch_grouped_variants_identical(monkeypatch, precision, nvar):
     # every grouped tile variant of a precision (forced for all stages) sums k in the same
     # order, so the tuner's choice cannot change a batch's numbers

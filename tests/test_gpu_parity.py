@@ -147,7 +147,7 @@ def test_b_product_four_levels_reproducible(ibhip, monkeypatch):
 
     ref = run()
     assert rel(ref, want) < 1e-11
-    for v in (0, 9, 20, 30, 39):
+    for v in (0, 9, 20, 30, 39, 40, 44, 48, 51):
         assert np.array_equal(run(v), ref)
 
 
@@ -500,11 +500,14 @@ def test_split_contraction_matches_full(ibhip, monkeypatch):
         assert rel(a, b) < 1e-13
 
 
-def test_split_tile_variants_identical(ibhip, monkeypatch):
-    prob = cylinder_case(n=200, nlev=2)
+@pytest.mark.parametrize("precision,nvar", [("fp64", 52), ("fp32", 21), ("bf16x3", 9)])
+def test_split_tile_variants_identical(ibhip, monkeypatch, precision, nvar):
+    # every split variant of a precision (register- or LDS-direct-staged tiles, both parts per
+    # wave or one) sums the same k in the same order: bitwise-equal Laplacian inverses
+    prob = cylinder_case(n=200, nlev=2, precision=precision)
     x = np.random.default_rng(13).standard_normal(199 * 199)
     outs = []
-    for v in range(52):  # every FP64 split variant (indices past the table fall back to 0)
+    for v in range(nvar):  # every split variant of the precision
         monkeypatch.setenv("IBH_GEMM_VARIANT", str(v))
         outs.append(ibhip.DeviceCase(prob).lapinv(2, 0, x))
     for o in outs[1:]:
