@@ -834,20 +834,27 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
 
     // fragment of K step kk: the sine pair (So, Se) of row i, the data pair (x even, x odd) in
     // the operand type (FP32 rounds the data here, as eo_tile's tile store does)
+    // (WS = 2: only this wave's part, slot half `part`)
     auto ldf = [&](const char *stage, int kk, ST *sf, E (*df)[2]) {
         const int k = 4 * kk + lq;
 #pragma unroll
         for (int i = 0; i < FP; ++i) {
             const int row = wp * Q::TP + i * 16 + l16;
-            sf[i] = *(const ST *)(stage + (row * Q::SPR + ((k / PPS) ^ (row & Q::SWM))) * 16 +
-                                  (k % PPS) * SB);
+            const char *a = stage + (row * Q::SPR + ((k / PPS) ^ (row & Q::SWM))) * 16 + (k % PPS) * SB;
+            if (WS == 2) sf[i].x = *(const E *)(a + part * (SB / 2));
+            else sf[i] = *(const ST *)a;
         }
 #pragma unroll
         for (int j = 0; j < FD; ++j) {
             const int row = wd * Q::TD + j * 16 + l16;
-            const d2 x = *(const d2 *)(stage + Q::SBYTES + row * Q::PITCH + k * 16);
-            df[j][0] = (E)x.x;
-            df[j][1] = (E)x.y;
+            const char *a = stage + Q::SBYTES + row * Q::PITCH + k * 16;
+            if (WS == 2) {
+                df[j][0] = (E) * (const double *)(a + part * 8);
+            } else {
+                const d2 x = *(const d2 *)a;
+                df[j][0] = (E)x.x;
+                df[j][1] = (E)x.y;
+            }
         }
     };
     auto mm = [&](const ST *sf, const E (*df)[2]) {
@@ -856,7 +863,7 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
 #pragma unroll
             for (int j = 0; j < FD; ++j) {
                 if (WS == 2) {
-                    accO[i][j] = mfma16(part ? sf[i].y : sf[i].x, df[j][part], accO[i][j]);
+                    accO[i][j] = mfma16(sf[i].x, df[j][0], accO[i][j]);
                 } else {
                     accO[i][j] = mfma16(sf[i].x, df[j][0], accO[i][j]);
                     accE[i][j] = mfma16(sf[i].y, df[j][1], accE[i][j]);
