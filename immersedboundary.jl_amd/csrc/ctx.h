@@ -16,9 +16,12 @@ namespace ibh {
 struct TuneKey {
     int prec, kind, M, N, K, batch;
     bool tout;
+    // split contractions: launched output extents (grid x, y) and summed pairs, which the
+    // body-box narrowing makes differ between launches of one M, N, K
+    int ex, ey, kp;
     bool operator<(const TuneKey &o) const {
-        return std::tie(prec, kind, M, N, K, batch, tout) <
-               std::tie(o.prec, o.kind, o.M, o.N, o.K, o.batch, o.tout);
+        return std::tie(prec, kind, M, N, K, batch, tout, ex, ey, kp) <
+               std::tie(o.prec, o.kind, o.M, o.N, o.K, o.batch, o.tout, o.ex, o.ey, o.kp);
     }
 };
 

@@ -196,7 +196,14 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     // build tuned instead of timing every variant inside a step; all variants give the same bits
     const int bb = g0.batch <= 2 ? g0.batch : g0.batch <= 4 ? 4 : g0.batch <= 8 ? 8
                  : g0.batch <= 16 ? 16 : g0.batch <= 32 ? 32 : 64;
-    const TuneKey key{prec, kind, g0.M, g0.N, g0.K, bb, tout};
+    int ex = 0, ey = 0, kp = 0;
+    if (kind != 0) {
+        const int gm = kind == 1 ? g0.h : g0.M, gn = kind == 1 ? g0.N : g0.h;
+        ex = g0.ohi[0] > 0 ? std::min(g0.ohi[0], gm) - g0.olo[0] : gm;
+        ey = g0.ohi[1] > 0 ? std::min(g0.ohi[1], gn) - g0.olo[1] : gn;
+        kp = g0.khi > 0 ? g0.khi - g0.klo : g0.h;
+    }
+    const TuneKey key{prec, kind, g0.M, g0.N, g0.K, bb, tout, ex, ey, kp};
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) return it->second;
     if (c->tune_fixed >= 0) return c->tuned[key] = c->tune_fixed;
@@ -241,10 +248,15 @@ int tune(ibh_ctx *c, const GemmArgs &g0, int prec, int kind, bool tout) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     const char *vname = kind == 0 ? gemm_variant_name(prec, best) : gemm_eo_variant_name(prec, best);
+    const bool narrowed = kind != 0 && (g0.ohi[0] > 0 || g0.ohi[1] > 0 || g0.khi > 0);
     c->tune_log.push_back(std::string(vname) + (kind == 2 ? " right" : kind == 1 ? " left" : "") +
                           " M=" + std::to_string(g0.M) + " N=" + std::to_string(g0.N) + " K=" +
                           std::to_string(g0.K) + " batch=" + std::to_string(g0.batch) +
-                          (tout ? " T" : "") + " " + std::to_string(best_ms / 3 * 1e3) + "us");
+                          (tout ? " T" : "") +
+                          (narrowed ? " out " + std::to_string(ex) + "x" + std::to_string(ey) +
+                                          " pairs " + std::to_string(kp)
+                                    : std::string()) +
+                          " " + std::to_string(best_ms / 3 * 1e3) + "us");
     return c->tuned[key] = best;
 }
 
