@@ -488,14 +488,43 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 // iteration u = r + (‖r‖²/‖r_prev‖²) u, c = A u, α = ‖r‖²/(u·c), x += α u, r -= α c.  The stop
 // test (it ≥ maxiter or ‖r‖ ≤ tol) runs before every iteration.  One block does the vector
 // work; the device flag `done` makes the surrounding B-products skip once converged.
-// z = P r for PCG (dense n×n preconditioner), skipped once converged
-// PCG direction (IterativeSolvers 0.9.2 PCGIterable): ρ_prev = ρ; ρ = z·r; u = z + (ρ/ρ_prev) u
+// PCG direction (IterativeSolvers 0.9.2 PCGIterable): z = P r; ρ_prev = ρ; ρ = z·r;
+// u = z + (ρ/ρ_prev) u, skipped once converged.  P is stored transposed, so z_i is one
+// workgroup's dot product of a contiguous row with r (four independent partial sums per thread
+// keep loads in flight), which also leaves z_i·r_i; one workgroup then sums those, forms β and
+// updates u.  Every sum runs in a fixed order.
+constexpr int PCG_BS = 256;
+
+int pcg_parts(int n) { return n; }
+
+__global__ __launch_bounds__(PCG_BS) void k_pcg_gemv(CGDev cg) {
+    __shared__ double sh[PCG_BS / 64 + 1];
+    if (*cg.done) return;
+    const int n = cg.n, i = blockIdx.x;
+    const double *__restrict__ row = cg.P + (long)n * i;
+    const double *__restrict__ r = cg.r;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = threadIdx.x;
+    for (; k + 3 * PCG_BS < n; k += 4 * PCG_BS) {
+        a0 += row[k] * r[k];
+        a1 += row[k + PCG_BS] * r[k + PCG_BS];
+        a2 += row[k + 2 * PCG_BS] * r[k + 2 * PCG_BS];
+        a3 += row[k + 3 * PCG_BS] * r[k + 3 * PCG_BS];
+    }
+    for (; k < n; k += PCG_BS) a0 += row[k] * r[k];
+    const double z = block_sum<PCG_BS>((a0 + a1) + (a2 + a3), sh);
+    if (threadIdx.x == 0) {
+        cg.z[i] = z;
+        cg.part[i] = z * r[i];
+    }
+}
+
 __global__ void k_pcg_dir(CGDev cg) {
     __shared__ double sh[17];
     if (*cg.done) return;
     const int n = cg.n;
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.z[i] * cg.r[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.part[i];
     s = block_sum<1024>(s, sh);
     const double beta = s / cg.sc[3];
     for (int i = threadIdx.x; i < n; i += blockDim.x) cg.u[i] = cg.z[i] + beta * cg.u[i];
@@ -504,8 +533,23 @@ __global__ void k_pcg_dir(CGDev cg) {
 }
 
 void launch_pcg_dir(const CGDev &cg, hipStream_t s) {
-    launch_gemv(cg.n, cg.P, cg.r, cg.z, cg.ws, cg.done, s);  // z = P r
+    if (cg.n == 0) return;
+    hipLaunchKernelGGL(k_pcg_gemv, dim3(cg.n), dim3(PCG_BS), 0, s, cg);
     hipLaunchKernelGGL(k_pcg_dir, dim3(1), dim3(1024), 0, s, cg);
+}
+
+__global__ void k_transpose_inplace(int n, double *A) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long i = t / n, j = t % n;
+    if (i >= n || j <= i) return;
+    const double a = A[i + n * j], b = A[j + n * i];
+    A[i + n * j] = b;
+    A[j + n * i] = a;
+}
+
+void launch_transpose_inplace(int n, double *A, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_transpose_inplace, dim3(nblk((long)n * n, 256)), dim3(256), 0, s, n, A);
 }
 
 __global__ void k_cg_init(CGDev cg) {

@@ -1207,6 +1207,7 @@ int refresh_pc(ibh_ctx *c) {
     if (const char *pi = std::getenv("IBH_PC_INV")) spd = std::strcmp(pi, "spd") == 0;
     rc = spd ? invert_spd(c, c->Bwork, c->Pc) : invert(c, c->Bwork, c->Pc);
     if (rc) return rc;
+    launch_transpose_inplace(nf, c->Pc, c->s);  // the PCG step reads P by rows
     HIPCK(hipStreamSynchronize(c->s));
     const double t2 = now_s();
     ++c->pc_rebuilds;
@@ -1826,6 +1827,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.u, nf));
     CK(dalloc(c, &c->cg.z, nf));
     CK(dalloc(c, &c->cg.ws, gemv_ws_size(nf)));
+    CK(dalloc(c, &c->cg.part, std::max(1, pcg_parts(nf))));
     CK(dalloc(c, &c->cg.sc, 4));
     c->cg.done = c->status;
     c->cg.it = c->status + 1;

@@ -257,8 +257,9 @@ struct CGDev {
     double *x, *b, *r, *c, *u;  // 2nb each
     double *z;                  // preconditioned residual (PCG)
     double *ws;                 // GEMV workspace (gemv_ws_size(n))
-    const double *P;            // dense preconditioner n×n (PCG) or null
+    const double *P;            // dense preconditioner n×n (PCG), stored transposed, or null
     double *sc;                 // [residual, prev_residual, tol, ρ]
+    double *part;               // z_i·r_i of the PCG step (pcg_parts(n))
     int *it, *done;
     int n, maxiter;
     double reltol;
@@ -266,6 +267,9 @@ struct CGDev {
 void launch_cg_init(const CGDev &cg, hipStream_t s);        // after c = B·x: r = b - c, ...
 void launch_cg_dir(const CGDev &cg, hipStream_t s);         // u = r + β u   (CG)
 void launch_pcg_dir(const CGDev &cg, hipStream_t s);        // z = P r; u = z + β u (PCG)
+int pcg_parts(int n);                                        // doubles of CGDev::part
+// in-place transpose of an n×n matrix (the PCG preconditioner, read by rows)
+void launch_transpose_inplace(int n, double *A, hipStream_t s);
 void launch_cg_update(const CGDev &cg, hipStream_t s);      // after c = B·u
 // mirror fresh columns' transposes into the stale rows of an assembled n×n matrix
 void launch_mirror_rows(int n, double *Bm, const int *cols, int ncols, const int *is_col,
