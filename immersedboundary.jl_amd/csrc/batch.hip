@@ -29,7 +29,7 @@ namespace {
 
 enum Op {
     OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
-    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMVP, OP_GEMVS, OP_STRESS, OP_RHS, OP_RESID,
+    OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMV, OP_STRESS, OP_RHS, OP_RESID,
     OP_ETZS, OP_ROTI, OP_EDGE, OP_SPEC, OP_LNP, OP_LNC, OP_LNO, OP_AXPY
 };
 
@@ -392,16 +392,15 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
     {
         std::vector<EA> e;
         std::vector<GemvA> g;
-        std::vector<int> be, bp, bs;
+        std::vector<int> be, bp;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             if (c->nb == 0) continue;
             const int nf = 2 * c->nb;
             e.push_back({c->d, c->reg, c->qs, c->Ftmp, b->uv, c->h});
             be.push_back(nblk(nf, GRP_BS / 64));  // one wave per output
-            g.push_back({nf, c->Binv, c->Ftmp, c->cg.ws, c->Fb});
-            bp.push_back(nblk(nf, GRP_BS) * ((nf + GV_CHUNK - 1) / GV_CHUNK));
-            bs.push_back(nblk(nf, GRP_BS / 64));  // one wave per row
+            g.push_back({nf, c->Binv, c->Ftmp, c->Fb});
+            bp.push_back(nf);  // one workgroup per row of Binv (stored transposed)
         }
         add(v, OP_E, e, be);
         if (!e.empty()) {
@@ -412,8 +411,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
                 S.vblocks.push_back(vb);
             }
         }
-        add(v, OP_GEMVP, g, bp);
-        add(v, OP_GEMVS, g, bs);
+        add(v, OP_GEMV, g, bp);
     }
     // ---- update_stress! and CircProjecter (timestepping.jl:156-171): Γ = Γs;
     //      Γ₁ -= Ainv₁ C' Eᵀ Fb
@@ -574,8 +572,7 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
     case OP_LNC: launch_grp_lines_colsum(g, s); break;
     case OP_LNO: launch_grp_lines_out(g, s); break;
     case OP_AXPY: launch_grp_axpy_sub(g, s); break;
-    case OP_GEMVP: launch_grp_gemv_part(g, s); break;
-    case OP_GEMVS: launch_grp_gemv_sum(g, s); break;
+    case OP_GEMV: launch_grp_gemv_t(g, s); break;
     case OP_STRESS: launch_grp_stress(g, s); break;
     case OP_RHS: launch_grp_rhs(g, s); break;
     case OP_RESID: launch_grp_resid(g, s); break;

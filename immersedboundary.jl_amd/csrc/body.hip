@@ -362,61 +362,6 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
                        (const double *)scale, skip);
 }
 
-// y = A·x for a dense n×n column-major matrix (the Binv GEMV of cnab/ops.jl:97 and the PCG
-// preconditioner).  Phase 1: block (row block, column chunk of GV_CH) writes partial sums,
-// one thread per row so every column read is coalesced; phase 2: one wave per row sums that
-// row's chunks (stored contiguously, [row][chunk]) lane-strided and then in a fixed tree order
-// (deterministic).  Short chunks give phase 1 enough waves to keep the matrix stream in flight.
-// ws holds ceil(n/GV_CH)·n doubles.
-static constexpr int GV_CH = 16;
-
-__device__ inline void gemv_part_at(int n, const double *__restrict__ A,
-                                    const double *__restrict__ x, double *__restrict__ ws, int i,
-                                    int chunk) {
-    const int j0 = chunk * GV_CH, j1 = min(n, j0 + GV_CH);
-    const int nch = (n + GV_CH - 1) / GV_CH;
-    double s = 0.0;
-    for (int j = j0; j < j1; ++j) s += A[i + (long)n * j] * x[j];
-    ws[(long)i * nch + chunk] = s;
-}
-
-// lane `lane` of the wave that owns row i; lane 0 writes y[i]
-__device__ inline void gemv_sum_at(int n, const double *__restrict__ ws, double *__restrict__ y,
-                                   int i, int lane) {
-    const int nch = (n + GV_CH - 1) / GV_CH;
-    const double *w = ws + (long)i * nch;
-    double s = 0.0;
-    for (int c = lane; c < nch; c += 64) s += w[c];
-    s = wave_sum(s);
-    if (lane == 0) y[i] = s;
-}
-
-__global__ void k_gemv_part(int n, const double *__restrict__ A, const double *__restrict__ x,
-                            double *__restrict__ ws, const int *skip) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    gemv_part_at(n, A, x, ws, i, blockIdx.y);
-}
-
-__global__ void k_gemv_sum(int n, const double *__restrict__ ws, double *__restrict__ y,
-                           const int *skip) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (i >= n) return;
-    gemv_sum_at(n, ws, y, i, threadIdx.x & 63);
-}
-
-long gemv_ws_size(int n) { return (long)((n + GV_CH - 1) / GV_CH) * n; }
-
-void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
-                 hipStream_t s) {
-    if (n == 0) return;
-    const int nch = (n + GV_CH - 1) / GV_CH;
-    hipLaunchKernelGGL(k_gemv_part, dim3(nblk(n, 256), nch), dim3(256), 0, s, n, A, x, ws, skip);
-    hipLaunchKernelGGL(k_gemv_sum, dim3(nblk(n, 4)), dim3(256), 0, s, n, ws, y, skip);
-}
-
 __global__ void k_axpy_sub(int n, double *y, const double *v, double a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] -= v[i] * a;
@@ -443,6 +388,42 @@ __device__ double block_sum(double v, double *sh) {
     t = sh[BS / 64];
     __syncthreads();
     return t;
+}
+
+// y = A·x for a dense n×n matrix stored transposed (AT column-major, i.e. A by rows): the Binv
+// GEMV of cnab/ops.jl:97 and the PCG preconditioner's.  One workgroup per output reads its row
+// contiguously; four independent partial sums per thread keep the loads in flight, and a
+// fixed-order block sum finishes the row (deterministic).
+constexpr int GEMV_BS = 256;
+
+__device__ inline double gemv_row(int n, const double *__restrict__ AT,
+                                  const double *__restrict__ x, int i, double *sh) {
+    const double *__restrict__ row = AT + (long)n * i;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = threadIdx.x;
+    for (; k + 3 * GEMV_BS < n; k += 4 * GEMV_BS) {
+        a0 += row[k] * x[k];
+        a1 += row[k + GEMV_BS] * x[k + GEMV_BS];
+        a2 += row[k + 2 * GEMV_BS] * x[k + 2 * GEMV_BS];
+        a3 += row[k + 3 * GEMV_BS] * x[k + 3 * GEMV_BS];
+    }
+    for (; k < n; k += GEMV_BS) a0 += row[k] * x[k];
+    return block_sum<GEMV_BS>((a0 + a1) + (a2 + a3), sh);
+}
+
+__global__ __launch_bounds__(GEMV_BS) void k_gemv_t(int n, const double *__restrict__ AT,
+                                                     const double *__restrict__ x,
+                                                     double *__restrict__ y, const int *skip) {
+    __shared__ double sh[GEMV_BS / 64 + 1];
+    if (skip && *skip) return;
+    const double v = gemv_row(n, AT, x, blockIdx.x, sh);
+    if (threadIdx.x == 0) y[blockIdx.x] = v;
+}
+
+void launch_gemv_t(int n, const double *AT, const double *x, double *y, const int *skip,
+                   hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gemv_t, dim3(n), dim3(GEMV_BS), 0, s, n, AT, x, y, skip);
 }
 
 // update_stress!, streamfcn/states.jl:53-84: per body fb = F·h/dt, CD = 2Σfb_x, CL = 2Σfb_y,
@@ -489,33 +470,20 @@ void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const 
 // test (it ≥ maxiter or ‖r‖ ≤ tol) runs before every iteration.  One block does the vector
 // work; the device flag `done` makes the surrounding B-products skip once converged.
 // PCG direction (IterativeSolvers 0.9.2 PCGIterable): z = P r; ρ_prev = ρ; ρ = z·r;
-// u = z + (ρ/ρ_prev) u, skipped once converged.  P is stored transposed, so z_i is one
-// workgroup's dot product of a contiguous row with r (four independent partial sums per thread
-// keep loads in flight), which also leaves z_i·r_i; one workgroup then sums those, forms β and
+// u = z + (ρ/ρ_prev) u, skipped once converged.  P is stored transposed: z_i comes from
+// gemv_row, whose workgroup also leaves z_i·r_i; one workgroup then sums those, forms β and
 // updates u.  Every sum runs in a fixed order.
-constexpr int PCG_BS = 256;
 
 int pcg_parts(int n) { return n; }
 
-__global__ __launch_bounds__(PCG_BS) void k_pcg_gemv(CGDev cg) {
-    __shared__ double sh[PCG_BS / 64 + 1];
+__global__ __launch_bounds__(GEMV_BS) void k_pcg_gemv(CGDev cg) {
+    __shared__ double sh[GEMV_BS / 64 + 1];
     if (*cg.done) return;
-    const int n = cg.n, i = blockIdx.x;
-    const double *__restrict__ row = cg.P + (long)n * i;
-    const double *__restrict__ r = cg.r;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int k = threadIdx.x;
-    for (; k + 3 * PCG_BS < n; k += 4 * PCG_BS) {
-        a0 += row[k] * r[k];
-        a1 += row[k + PCG_BS] * r[k + PCG_BS];
-        a2 += row[k + 2 * PCG_BS] * r[k + 2 * PCG_BS];
-        a3 += row[k + 3 * PCG_BS] * r[k + 3 * PCG_BS];
-    }
-    for (; k < n; k += PCG_BS) a0 += row[k] * r[k];
-    const double z = block_sum<PCG_BS>((a0 + a1) + (a2 + a3), sh);
+    const int i = blockIdx.x;
+    const double z = gemv_row(cg.n, cg.P, cg.r, i, sh);
     if (threadIdx.x == 0) {
         cg.z[i] = z;
-        cg.part[i] = z * r[i];
+        cg.part[i] = z * cg.r[i];
     }
 }
 
@@ -534,7 +502,7 @@ __global__ void k_pcg_dir(CGDev cg) {
 
 void launch_pcg_dir(const CGDev &cg, hipStream_t s) {
     if (cg.n == 0) return;
-    hipLaunchKernelGGL(k_pcg_gemv, dim3(cg.n), dim3(PCG_BS), 0, s, cg);
+    hipLaunchKernelGGL(k_pcg_gemv, dim3(cg.n), dim3(GEMV_BS), 0, s, cg);
     hipLaunchKernelGGL(k_pcg_dir, dim3(1), dim3(1024), 0, s, cg);
 }
 
@@ -736,7 +704,7 @@ void launch_gj_step(int n, double *A, double *Inv, int k, int *piv_row, int *err
 // ------------------------------------------------------------------------ grouped launches
 // E, Eᵀ, the dense-inverse GEMV and the stress sums over a list of cases (one problem per
 // case; see Grp).  Same per-element bodies as the single-case kernels.
-static_assert(GRP_NS == NS && GV_CHUNK == GV_CH, "grouped constants");
+static_assert(GRP_NS == NS, "grouped constants");
 
 template <int W>
 __global__ __launch_bounds__(64 * W) void kg_E(const EA *args, const int *off, int n) {
@@ -839,21 +807,14 @@ __global__ __launch_bounds__(GRP_BS) void kg_ET_fix(const ETA *args, const int *
     if (t < A.d.nq) et_fix_at(A.q, *A.scale, t);
 }
 
-__global__ __launch_bounds__(GRP_BS) void kg_gemv_part(const GemvA *args, const int *off, int n) {
+__global__ __launch_bounds__(GRP_BS) void kg_gemv_t(const GemvA *args, const int *off, int n) {
+    static_assert(GRP_BS == GEMV_BS, "one row per workgroup");
+    __shared__ double sh[GEMV_BS / 64 + 1];
     const int p = grp_find(off, n, blockIdx.x);
     const GemvA A = args[p];
-    const int lb = blockIdx.x - off[p], nbx = (int)nblk(A.n, GRP_BS);
-    const int i = (lb % nbx) * GRP_BS + threadIdx.x;
-    if (i >= A.n) return;
-    gemv_part_at(A.n, A.A, A.x, A.ws, i, lb / nbx);
-}
-
-__global__ __launch_bounds__(GRP_BS) void kg_gemv_sum(const GemvA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
-    const GemvA A = args[p];
-    const int i = (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6);
-    if (i >= A.n) return;
-    gemv_sum_at(A.n, A.ws, A.y, i, threadIdx.x & 63);
+    const int i = blockIdx.x - off[p];
+    const double v = gemv_row(A.n, A.A, A.x, i, sh);
+    if (threadIdx.x == 0) A.y[i] = v;
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_stress(const StressA *args, const int *off, int n) {
@@ -873,8 +834,7 @@ GRP_LAUNCH(launch_grp_ET_zero, kg_ET_zero, ETA)
 GRP_LAUNCH(launch_grp_ET_scale, kg_ET_scale, ETA)
 GRP_LAUNCH(launch_grp_ET_zero_scale, kg_ET_zero_scale, ETA)
 GRP_LAUNCH(launch_grp_ET_fix, kg_ET_fix, ETA)
-GRP_LAUNCH(launch_grp_gemv_part, kg_gemv_part, GemvA)
-GRP_LAUNCH(launch_grp_gemv_sum, kg_gemv_sum, GemvA)
+GRP_LAUNCH(launch_grp_gemv_t, kg_gemv_t, GemvA)
 GRP_LAUNCH(launch_grp_stress, kg_stress, StressA)
 
 }  // namespace ibh

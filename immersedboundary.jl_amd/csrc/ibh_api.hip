@@ -1161,6 +1161,7 @@ int precompute_binv(ibh_ctx *c) {
     if (rc) return rc;
     rc = invert(c, Bmat, c->Binv);  // Binv = inv(Bmat), cnab/ops.jl:96
     if (rc) return rc;
+    launch_transpose_inplace(nf, c->Binv, c->s);  // the GEMV reads Binv by rows
     dfree(c, Bmat);
     return release_B_work(c);
 }
@@ -1388,7 +1389,7 @@ void seg_solve_begin(ibh_ctx *c) {
     const int nf = 2 * c->nb;
     if (nf == 0) return;
     if (!c->use_cg) {
-        launch_gemv(nf, c->Binv, c->Ftmp, c->Fb, c->cg.ws, nullptr, c->s);
+        launch_gemv_t(nf, c->Binv, c->Ftmp, c->Fb, nullptr, c->s);
         return;
     }
     CGDev &cg = c->cg;
@@ -1826,7 +1827,6 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.c, nf));
     CK(dalloc(c, &c->cg.u, nf));
     CK(dalloc(c, &c->cg.z, nf));
-    CK(dalloc(c, &c->cg.ws, gemv_ws_size(nf)));
     CK(dalloc(c, &c->cg.part, std::max(1, pcg_parts(nf))));
     CK(dalloc(c, &c->cg.sc, 4));
     c->cg.done = c->status;

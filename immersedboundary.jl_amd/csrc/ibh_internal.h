@@ -241,9 +241,9 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
 void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
                      long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
                      const double *V = nullptr, double *w = nullptr, int lrk = 0);
-void launch_gemv(int n, const double *A, const double *x, double *y, double *ws, const int *skip,
-                 hipStream_t s);
-long gemv_ws_size(int n);  // doubles of workspace launch_gemv needs
+// y = A x with A stored transposed (AT column-major: A's rows contiguous), one workgroup per row
+void launch_gemv_t(int n, const double *AT, const double *x, double *y, const int *skip,
+                   hipStream_t s);
 void launch_axpy_sub(int n, double *y, const double *v, double a, hipStream_t s);  // y -= a v
 
 void launch_stress(int nbodies, const int *boff, const int *npts, int nb, const double *Fb,
@@ -256,7 +256,6 @@ constexpr int CG_CONVERGED = 1, CG_MAXITER = 2, CG_NONFINITE = 3;
 struct CGDev {
     double *x, *b, *r, *c, *u;  // 2nb each
     double *z;                  // preconditioned residual (PCG)
-    double *ws;                 // GEMV workspace (gemv_ws_size(n))
     const double *P;            // dense preconditioner n×n (PCG), stored transposed, or null
     double *sc;                 // [residual, prev_residual, tol, ρ]
     double *part;               // z_i·r_i of the PCG step (pcg_parts(n))
@@ -332,7 +331,7 @@ struct ResidA { Dom d; const double *x, *rhs; double a, ar; double *r; int nbatc
 struct FillA { double *p; long n; double v; };
 struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
 struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; };
-struct GemvA { int n; const double *A; const double *x; double *ws; double *y; };
+struct GemvA { int n; const double *A; const double *x; double *y; };  // A transposed
 struct StressA {
     int nbodies; const int *boff, *npts; int nb; const double *Fb, *ds; double h, dt;
     double *CD, *CL, *fbs;
@@ -366,7 +365,6 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 static constexpr int GRP_NS = 13;  // delta support per dimension
-static constexpr int GV_CHUNK = 16; // GEMV column chunk
 
 void launch_grp_curl(const Grp &g, hipStream_t s);
 void launch_grp_rot(const Grp &g, hipStream_t s);
@@ -393,8 +391,7 @@ void launch_grp_ET(const Grp &g, int sched, hipStream_t s);  // ETA; blocks: grp
 long grp_E_blocks(int nb, int sched);
 long grp_ET_blocks(int nb, int sched);
 void launch_grp_ET_fix(const Grp &g, hipStream_t s);      // ETA; blocks: ceil(nq/256)
-void launch_grp_gemv_part(const Grp &g, hipStream_t s);   // blocks: ceil(n/256)·ceil(n/64)
-void launch_grp_gemv_sum(const Grp &g, hipStream_t s);    // blocks: ceil(n/256)
+void launch_grp_gemv_t(const Grp &g, hipStream_t s);      // blocks: n (one per row)
 void launch_grp_stress(const Grp &g, hipStream_t s);      // blocks: nbodies
 
 // grouped even/odd split contraction: every problem's GemmArgs in device memory; problem i
