@@ -315,13 +315,15 @@ __global__ void k_et_fix(double *__restrict__ q, long n, long sq, const double *
 // zeroing of the fixed-point image (the first ceil(n/256) blocks), the column's scale (the
 // next one) and, with lrk > 0, the low-rank dots w[b, j] = V[:, j]·fb[b] (one block per j;
 // V n2 × lrk column-major)
-__global__ __launch_bounds__(256) void k_et_zero_scale(double *__restrict__ q, long n, long sq,
-                                                       const double *__restrict__ fb, long sfb,
-                                                       int n2, double *scale, const int *skip,
+__global__ __launch_bounds__(256) void k_et_zero_scale(Dom d, GBox box, double *__restrict__ q,
+                                                       long sq, const double *__restrict__ fb,
+                                                       long sfb, int n2, double *scale,
+                                                       const int *skip,
                                                        const double *__restrict__ V,
                                                        double *__restrict__ w, int lrk) {
     if (skip && *skip) return;
     const int b = blockIdx.y;
+    const long n = gbox_faces(d, box);
     const long nz = nblk(n, 256);
     if (blockIdx.x == nz) {
         et_scale_body(fb + b * sfb, n2, scale + b);
@@ -340,14 +342,14 @@ __global__ __launch_bounds__(256) void k_et_zero_scale(double *__restrict__ q, l
         return;
     }
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t < n) q[b * sq + t] = 0.0;
+    if (t < n) q[b * sq + gbox_face(d, box, t)] = 0.0;
 }
 
 void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
                      long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
-                     const double *V, double *w, int lrk) {
-    hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(d.nq, 256) + 1 + lrk, nbatch), dim3(256), 0, s,
-                       img, d.nq, sq, fb, sfb, 2 * r.nb, scale, skip, V, w, lrk);
+                     const double *V, double *w, int lrk, GBox box) {
+    hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(gbox_faces(d, box), 256) + 1 + lrk, nbatch),
+                       dim3(256), 0, s, d, box, img, sq, fb, sfb, 2 * r.nb, scale, skip, V, w, lrk);
     if (r.nb == 0) return;
     launch_ET_scatter(d, r, fb, sfb, img, sq, nbatch, skip, scale, s);
 }

@@ -124,20 +124,21 @@ __device__ inline double rot_image_at(const Dom &d, const long long *__restrict_
     return (ux0 - ux1) + (vy1 - vy0);
 }
 
-__global__ void k_rot_image(Dom d, const long long *__restrict__ img, long sq,
+__global__ void k_rot_image(Dom d, GBox box, const long long *__restrict__ img, long sq,
                             const double *__restrict__ scale, double *__restrict__ g, long sg,
                             const int *skip) {
     SKIP_GUARD
-    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= d.ng) return;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= gbox_vertices(d, box)) return;
+    const long t = gbox_vertex(d, box, k);
     const int b = blockIdx.y;
     g[b * sg + t] = rot_image_at(d, img + b * sq, 1.0 / scale[b], t);
 }
 
 void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
-                      long sg, int nbatch, const int *skip, hipStream_t s) {
-    hipLaunchKernelGGL(k_rot_image, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d,
-                       reinterpret_cast<const long long *>(img), sq, scale, g, sg, skip);
+                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box) {
+    hipLaunchKernelGGL(k_rot_image, dim3(nblk(gbox_vertices(d, box), 256), nbatch), dim3(256), 0,
+                       s, d, box, reinterpret_cast<const long long *>(img), sq, scale, g, sg, skip);
 }
 
 // coarsify!(Γc, Γ), fluid-ops.jl:184-202: fine (i, j) = (2a + 2, 2b + 2) → coarse

@@ -679,14 +679,39 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
 // (n/2)·I per dimension), and its inverse transform on the body box: four narrowed
 // contractions where the full product makes about fourteen and forty small launches.
 constexpr int LR_MAXNC = 128;
+
+// The body box as a GBox: Eᵀ's image is cleared and rot formed only there (the whole grid
+// when the box is off).  The buffers they write (qB, g1B of the B-products, qtmp, g1 of the
+// projection) then hold zeros outside it: clear_box_work restores that when the box changes
+// and after the low-rank build, which uses qB and g1B as full-grid scratch.
+GBox body_gbox(const ibh_ctx *c) {
+    GBox b;
+    if (c->box_on) {
+        b.x0 = c->bxa;
+        b.x1 = c->bxb;
+        b.y0 = c->bya;
+        b.y1 = c->byb;
+    }
+    return b;
+}
+
+void clear_box_work(ibh_ctx *c) {
+    const Dom &d = c->d;
+    if (c->qB) launch_fill(c->qB, (long)c->ncB * d.nq, 0.0, c->s);
+    if (c->g1B) launch_fill(c->g1B, (long)c->ncB * d.ng, 0.0, c->s);
+    if (c->qtmp) launch_fill(c->qtmp, d.nq, 0.0, c->s);
+    if (c->g1) launch_fill(c->g1, d.ng, 0.0, c->s);
+}
+
 void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
                 const int *skip) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
+    const GBox box = body_gbox(c);
     launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, c->lrEV,
-                    c->lrW, c->lr_k);  // w = EVᵀ z in the same launch
-    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
+                    c->lrW, c->lr_k, box);  // w = EVᵀ z in the same launch
+    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box);
     LapInvPlan P;
     lapinv_plan(c, c->g1B, ng, c->psiB, L * ng, c->sAD, 0, nc, EPI_STORE, nullptr, skip, &P);
     P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
@@ -714,8 +739,10 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
         B_times_lr(c, z, sz, x, sx, nc, skip);
         return;
     }
-    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
-    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s);
+    const GBox box = body_gbox(c);
+    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, nullptr, nullptr,
+                    0, box);
+    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box);
     // Γ's coarse levels (zero here) need no clearing: their buffer is zeroed when allocated and
     // only ever receives coarsify!'s fixed centre blocks, which every product overwrites
     // Ainv₁ with its scaled spectrum kept (specB) for the level-1 Poisson solve
@@ -1110,6 +1137,7 @@ int lr_build(ibh_ctx *c) {
         for (int j = 0; j < k; j += LR_NC)
             lr_Tt(c, c->lrQ + (long)j * nq, Z + (long)j * ng, std::min(LR_NC, k - j), A);
         launch_curl_bc(d, Z, ng, c->zerobc, 0, c->lrV, nq, k, nullptr, c->s);
+        clear_box_work(c);  // lr_T and lr_Tt used qB and g1B over the whole grid
         lr_points(c);
         c->lr_valid = true;
         for (int j = 0; j < 3; ++j)
@@ -1259,6 +1287,7 @@ int update_box(ibh_ctx *c, const double *r) {
         c->byb = std::min(p - 1, yb + BOX_REACH + BOX_SLACK);
         if (c->bxa > c->bxb || c->bya > c->byb) c->box_on = false;
     }
+    if (c->box_on) clear_box_work(c);
     return IBH_OK;
 }
 
@@ -1420,8 +1449,10 @@ void seg_post(ibh_ctx *c) {
     post_swap(c);
     const bool fuse = c->use_eo && c->refine == 0;  // Γ₁'s spectrum known (seg_trial)
     if (nf) {
-        launch_ET_image(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s);
-        launch_rot_image(d, c->qtmp, d.nq, c->etscale, c->g1, ng, 1, nullptr, c->s);
+        const GBox box = body_gbox(c);
+        launch_ET_image(d, c->reg, c->Fb, nf, c->qtmp, d.nq, 1, nullptr, c->etscale, c->s,
+                        nullptr, nullptr, 0, box);
+        launch_rot_image(d, c->qtmp, d.nq, c->etscale, c->g1, ng, 1, nullptr, c->s, box);
         if (fuse) {
             // Γ₁ = S(F_trial − F_proj)S: keep the projection's spectrum and subtract it
             LapInvPlan P;

@@ -27,6 +27,45 @@ inline Dom make_dom(int nx, int ny) {
     return d;
 }
 
+// A rectangle of the interior vertices (0-based [x0, x1] × [y0, y1]) and the flux faces a
+// kernel touches around it (1-based [x0, x1 + 3] × [y0, y1 + 3], clipped to each block): the
+// body box of the B-products.  x0 < 0 means the whole grid.
+struct GBox { int x0 = -1, x1 = 0, y0 = 0, y1 = 0; };
+__host__ __device__ inline long gbox_vertices(const Dom &d, const GBox &b) {
+    return b.x0 < 0 ? d.ng : (long)(b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1);
+}
+__host__ __device__ inline long gbox_vertex(const Dom &d, const GBox &b, long k) {
+    if (b.x0 < 0) return k;
+    const int w = b.x1 - b.x0 + 1;
+    return (b.x0 + k % w) + (long)d.m * (b.y0 + k / w);
+}
+// faces of the box: the u block's [max(1, x0), min(nx+1, x1+3)] × [max(1, y0), min(ny, y1+3)]
+// first, then the v block's [max(1, x0), min(nx, x1+3)] × [max(1, y0), min(ny+1, y1+3)]
+__host__ __device__ inline void gbox_face_ranges(const Dom &d, const GBox &b, int *r) {
+    const int i0 = b.x0 > 1 ? b.x0 : 1, j0 = b.y0 > 1 ? b.y0 : 1;
+    r[0] = i0; r[1] = b.x1 + 3 < d.nx + 1 ? b.x1 + 3 : d.nx + 1;   // u: i
+    r[2] = j0; r[3] = b.y1 + 3 < d.ny ? b.y1 + 3 : d.ny;           // u: j
+    r[4] = i0; r[5] = b.x1 + 3 < d.nx ? b.x1 + 3 : d.nx;           // v: i
+    r[6] = j0; r[7] = b.y1 + 3 < d.ny + 1 ? b.y1 + 3 : d.ny + 1;   // v: j
+}
+__host__ __device__ inline long gbox_faces(const Dom &d, const GBox &b) {
+    if (b.x0 < 0) return d.nq;
+    int r[8];
+    gbox_face_ranges(d, b, r);
+    return (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) + (long)(r[5] - r[4] + 1) * (r[7] - r[6] + 1);
+}
+__host__ __device__ inline long gbox_face(const Dom &d, const GBox &b, long k) {
+    if (b.x0 < 0) return k;
+    int r[8];
+    gbox_face_ranges(d, b, r);
+    const int wu = r[1] - r[0] + 1;
+    const long nu = (long)wu * (r[3] - r[2] + 1);
+    if (k < nu) return (r[0] - 1 + k % wu) + (long)(d.nx + 1) * (r[2] - 1 + k / wu);
+    k -= nu;
+    const int wv = r[5] - r[4] + 1;
+    return d.nu + (r[4] - 1 + k % wv) + (long)d.nx * (r[6] - 1 + k / wv);
+}
+
 // ---------------------------------------------------------------------------- sine GEMM
 // C[b] = op(A[b]·B[b]) with A(m, k) at A[k + lda·m] and B(k, n) at B[k + ldb·n] (both
 // k-fastest); batch b = blockIdx.z.
@@ -99,8 +138,9 @@ void launch_curl_box(const Dom &d, const double *psi, long spsi, const double *p
 void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int nbatch,
                 const int *skip, hipStream_t s);
 // rot of the q whose fixed-point image (scale per batch entry) launch_ET_image left in img
+// (with a box: only Γ inside it; the caller keeps Γ zero outside)
 void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
-                      long sg, int nbatch, const int *skip, hipStream_t s);
+                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box = GBox());
 void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
                      const int *skip, hipStream_t s);
 void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
@@ -238,9 +278,12 @@ void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double
 // launch_rot_image reads that image directly, converting as launch_ET's last pass would.
 // With lrk > 0 the first launch also forms w[b] = Vᵀ fb[b] (V 2nb × lrk), the low-rank
 // coarse-level part's input
+// With a box, only its faces are cleared: every face Eᵀ writes lies in it (the box holds the
+// points' supports), and the caller keeps the image zero outside it.
 void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, double *img,
                      long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
-                     const double *V = nullptr, double *w = nullptr, int lrk = 0);
+                     const double *V = nullptr, double *w = nullptr, int lrk = 0,
+                     GBox box = GBox());
 // y = A x with A stored transposed (AT column-major: A's rows contiguous), one workgroup per row
 void launch_gemv_t(int n, const double *AT, const double *x, double *y, const int *skip,
                    hipStream_t s);
