@@ -159,7 +159,9 @@ struct ibh_ctx {
     int dev = 0;            // the device current at ibh_create
     int parity = 0;
     long steps_done = 0;
-    hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter = nullptr,
+    // step segments as HIP graphs: the trial state (two pointer parities), chunks of 1, 2, 4,
+    // 8 and 16 CG iterations, the projection and Vort2Flux (two parities)
+    hipGraphExec_t gTrial[2] = {nullptr, nullptr}, gIter[5] = {nullptr, nullptr, nullptr, nullptr, nullptr},
                    gPost[2] = {nullptr, nullptr};
     ibh::Prof prof;
     std::string err;

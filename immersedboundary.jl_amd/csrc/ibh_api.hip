@@ -1526,7 +1526,8 @@ int run_seg(ibh_ctx *c, hipGraphExec_t *slot, bool use_graph, F body, void (*aft
 }
 
 void drop_graphs(ibh_ctx *c) {
-    for (hipGraphExec_t *g : {&c->gTrial[0], &c->gTrial[1], &c->gIter, &c->gPost[0], &c->gPost[1]})
+    for (hipGraphExec_t *g : {&c->gTrial[0], &c->gTrial[1], &c->gIter[0], &c->gIter[1],
+                              &c->gIter[2], &c->gIter[3], &c->gIter[4], &c->gPost[0], &c->gPost[1]})
         if (*g) {
             hipGraphExecDestroy(*g);
             *g = nullptr;
@@ -1555,7 +1556,14 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     // and not on steps that rebuild the preconditioner
     const bool g = c->use_graphs && !c->prof.on && !refresh && c->steps_done >= 1;
     const int par = c->parity;
-    const int k1 = c->use_pcg ? 6 : 16, k2 = c->use_pcg ? 2 : 16;
+    // CG iterations: k1 in the trial segment; for PCG then as many as the last step needed
+    // (the count drifts slowly), queued as chunks of 8, 4, 2 and 1 before the first status
+    // read, and chunks of k2 until converged (each status read is a host round trip)
+    const int k1 = c->use_pcg ? 2 : 16, k2 = c->use_pcg ? 2 : 16;
+    auto chunk = [&](int k) {  // k ∈ {1, 2, 4, 8, 16}
+        const int slot = k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : k == 8 ? 3 : 4;
+        return run_seg(c, &c->gIter[slot], g, [&] { seg_cg_iters(c, k); }, nullptr);
+    };
     int rc;
     if (refresh) {  // eager: the preconditioner needs this step's regularization
         seg_trial(c);
@@ -1587,9 +1595,17 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     }
     int st[3] = {0, 0, 0};
     if (c->use_cg) {
+        if (c->use_pcg && !refresh) {
+            int ahead = std::max(0, (c->cg_last > 0 ? c->cg_last : 8) - k1);
+            for (int k = 8; k >= 1; k /= 2)
+                while (ahead >= k) {
+                    if ((rc = chunk(k))) return rc;
+                    ahead -= k;
+                }
+        }
         if ((rc = read_status(c, st))) return rc;
         while (!st[0]) {
-            if ((rc = run_seg(c, &c->gIter, g, [&] { seg_cg_iters(c, k2); }, nullptr))) return rc;
+            if ((rc = chunk(k2))) return rc;
             if ((rc = read_status(c, st))) return rc;
         }
         c->cg_last = st[1];
