@@ -261,80 +261,110 @@ void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long s
 
 // Nonlinear term at one level: avg_flux! + direct_product! + rot! + 1/hc² (cnab/ops.jl:151-168,
 // fluid-ops.jl:277-401).  q0 is the constant base flux (ux·hc, uy·hc) (fluid-ops.jl:403-417).
-struct NLCtx {
+// The face products are written once against an accessor of q, Γ and Γbc: NLGlobal reads
+// device memory, NLTile a workgroup's LDS copy; both give the same operands to the same
+// arithmetic.
+struct NLGlobal {
     Dom d;
     const double *q, *g, *gbc;
     double U, V, gs;  // q0x, q0y values; scale of gbc
+    __device__ double qx(int i, int j) const { return QX(q, i, j); }
+    __device__ double qy(int i, int j) const { return QY(q, i, j); }
+    __device__ double gi(int i, int j) const { return GI(g, i, j); }
 };
 
-__device__ inline double Qx_avg(const NLCtx &c, int i, int j) {  // j in 2..ny
-    const Dom &d = c.d;
-    return (QX(c.q, i, j) + QX(c.q, i, j - 1) + c.U + c.U) / 2;
+// one tile's operands in LDS: QX over i ∈ [i0-1, i0+TX], j ∈ [j0-1, j0+TY-1]; QY over
+// i ∈ [i0-1, i0+TX-1], j ∈ [j0-1, j0+TY]; Γ over i ∈ [i0-2, i0+TX-1], j ∈ [j0-2, j0+TY-1]
+// (1-based, zero outside the grid: the products never read those)
+struct NLTileMem {
+    double qx[NL_TY + 1][NL_TX + 2];
+    double qy[NL_TY + 2][NL_TX + 1];
+    double g[NL_TY + 2][NL_TX + 2];
+};
+struct NLTile {
+    Dom d;
+    const NLTileMem *m;
+    int i0, j0;
+    const double *gbc;
+    double U, V, gs;
+    __device__ double qx(int i, int j) const { return m->qx[j - j0 + 1][i - i0 + 1]; }
+    __device__ double qy(int i, int j) const { return m->qy[j - j0 + 1][i - i0 + 1]; }
+    __device__ double gi(int i, int j) const { return m->g[j - j0 + 2][i - i0 + 2]; }
+};
+
+template <class C> __device__ inline double Qx_avg(const C &c, int i, int j) {  // j in 2..ny
+    return (c.qx(i, j) + c.qx(i, j - 1) + c.U + c.U) / 2;
 }
-__device__ inline double Qy_avg(const NLCtx &c, int i, int j) {  // i in 2..nx
-    const Dom &d = c.d;
-    return (QY(c.q, i, j) + QY(c.q, i - 1, j) + c.V + c.V) / 2;
+template <class C> __device__ inline double Qy_avg(const C &c, int i, int j) {  // i in 2..nx
+    return (c.qy(i, j) + c.qy(i - 1, j) + c.V + c.V) / 2;
 }
-__device__ inline double gbcv(const NLCtx &c, int k) { return c.gbc[k - 1] * c.gs; }
+template <class C> __device__ inline double gbcv(const C &c, int k) { return c.gbc[k - 1] * c.gs; }
 
 // fqx[i, j], i in 1..nx+1, j in 1..ny (zero outside i = 2..nx)
-__device__ inline double fqx(const NLCtx &c, int i, int j) {
+template <class C> __device__ inline double fqx(const C &c, int i, int j) {
     const Dom &d = c.d;
     if (i < 2 || i > d.nx) return 0.0;
-    if (j == 1) return (Qy_avg(c, i, j + 1) * GI(c.g, i - 1, j) + Qy_avg(c, i, j) * gbcv(c, d.Bo + i)) / 2;
+    if (j == 1) return (Qy_avg(c, i, j + 1) * c.gi(i - 1, j) + Qy_avg(c, i, j) * gbcv(c, d.Bo + i)) / 2;
     if (j == d.ny)
-        return (Qy_avg(c, i, j) * GI(c.g, i - 1, j - 1) + Qy_avg(c, i, j + 1) * gbcv(c, d.To + i)) / 2;
-    return (Qy_avg(c, i, j + 1) * GI(c.g, i - 1, j) + Qy_avg(c, i, j) * GI(c.g, i - 1, j - 1)) / 2;
+        return (Qy_avg(c, i, j) * c.gi(i - 1, j - 1) + Qy_avg(c, i, j + 1) * gbcv(c, d.To + i)) / 2;
+    return (Qy_avg(c, i, j + 1) * c.gi(i - 1, j) + Qy_avg(c, i, j) * c.gi(i - 1, j - 1)) / 2;
 }
 // fqy[i, j], i in 1..nx, j in 1..ny+1 (zero outside j = 2..ny)
-__device__ inline double fqy(const NLCtx &c, int i, int j) {
+template <class C> __device__ inline double fqy(const C &c, int i, int j) {
     const Dom &d = c.d;
     if (j < 2 || j > d.ny) return 0.0;
     if (i == 1)
-        return -(Qx_avg(c, i + 1, j) * GI(c.g, i, j - 1) + Qx_avg(c, i, j) * gbcv(c, d.Lo + j)) / 2;
+        return -(Qx_avg(c, i + 1, j) * c.gi(i, j - 1) + Qx_avg(c, i, j) * gbcv(c, d.Lo + j)) / 2;
     if (i == d.nx)
-        return -(Qx_avg(c, i, j) * GI(c.g, i - 1, j - 1) + Qx_avg(c, i + 1, j) * gbcv(c, d.Ro + j)) / 2;
-    return -(Qx_avg(c, i + 1, j) * GI(c.g, i, j - 1) + Qx_avg(c, i, j) * GI(c.g, i - 1, j - 1)) / 2;
+        return -(Qx_avg(c, i, j) * c.gi(i - 1, j - 1) + Qx_avg(c, i + 1, j) * gbcv(c, d.Ro + j)) / 2;
+    return -(Qx_avg(c, i + 1, j) * c.gi(i, j - 1) + Qx_avg(c, i, j) * c.gi(i - 1, j - 1)) / 2;
 }
 
 __device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q,
                                       const double *__restrict__ g,
                                       const double *__restrict__ gbc, double gbc_scale,
                                       double U, double V, double hc, long t) {
-    NLCtx c;
-    c.d = d;
-    c.q = q;
-    c.g = g;
-    c.gbc = gbc;
-    c.U = U;
-    c.V = V;
-    c.gs = gbc_scale;
+    const NLGlobal c{d, q, g, gbc, U, V, gbc_scale};
     const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
     double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
     return v * (1 / (hc * hc));
 }
 
-// The same values over an NL_TX×NL_TY tile of outputs per 256-thread workgroup: each thread
-// forms the face products it owns, fqx(i, j) and fqy(i, j), and the tile's first row and column
-// also the ones just below and left of the tile, so every product is formed once instead of
-// once per adjacent output; the sum is nonlinear_at's, term for term.
+// The same values over an NL_TX×NL_TY tile of outputs per 256-thread workgroup.  The tile's
+// operands (with their one- or two-wide halo) are first copied to LDS in coalesced rows, so each
+// value is loaded once per workgroup instead of about twelve times per output through the
+// vector caches; then each thread forms the face products it owns, fqx(i, j) and fqy(i, j),
+// and the tile's first row and column also the ones just below and left of the tile, so every
+// product is formed once; the sum is nonlinear_at's, term for term.
 __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q,
                                       const double *__restrict__ g,
                                       const double *__restrict__ gbc, double gbc_scale, double U,
                                       double V, double hc, long tile, double *__restrict__ out) {
+    __shared__ NLTileMem mem;
     __shared__ double fx[NL_TY + 1][NL_TX];
     __shared__ double fy[NL_TY][NL_TX + 1];
-    NLCtx c;
-    c.d = d;
-    c.q = q;
-    c.g = g;
-    c.gbc = gbc;
-    c.U = U;
-    c.V = V;
-    c.gs = gbc_scale;
     const int tx = threadIdx.x % NL_TX, ty = threadIdx.x / NL_TX;
     const int ntx = (d.m + NL_TX - 1) / NL_TX;
-    const int i = 2 + (int)(tile % ntx) * NL_TX + tx, j = 2 + (int)(tile / ntx) * NL_TY + ty;
+    const int i0 = 2 + (int)(tile % ntx) * NL_TX, j0 = 2 + (int)(tile / ntx) * NL_TY;
+    // stage: QX, QY, Γ with bounds (1-based valid ranges of each block)
+    for (int e = threadIdx.x; e < (NL_TY + 1) * (NL_TX + 2); e += NL_TX * NL_TY) {
+        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
+        const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+        mem.qx[r][cidx] = (i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny) ? QX(q, i, j) : 0.0;
+    }
+    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 1); e += NL_TX * NL_TY) {
+        const int r = e / (NL_TX + 1), cidx = e % (NL_TX + 1);
+        const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+        mem.qy[r][cidx] = (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1) ? QY(q, i, j) : 0.0;
+    }
+    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 2); e += NL_TX * NL_TY) {
+        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
+        const int i = i0 - 2 + cidx, j = j0 - 2 + r;
+        mem.g[r][cidx] = (i >= 1 && i <= d.m && j >= 1 && j <= d.p) ? GI(g, i, j) : 0.0;
+    }
+    __syncthreads();
+    const NLTile c{d, &mem, i0, j0, gbc, U, V, gbc_scale};
+    const int i = i0 + tx, j = j0 + ty;
     const bool in = i <= d.m + 1 && j <= d.p + 1;
     if (in) {
         fx[ty + 1][tx] = fqx(c, i, j);
