@@ -30,7 +30,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "immersedboundary.jl_amd"))
 
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 3}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 3, "mixed": 78.6e12}
+# the contraction launches a profile reads: the precision's own (FP64 for the mixed form)
+KIND = {"fp64": 0, "fp32": 1, "bf16x3": 2, "mixed": 0}
 
 
 def log(*a):
@@ -183,40 +185,26 @@ def batch_cpu_baseline(probs, every=8):
     }
 
 
-def batch_lift_drag_error(ibhip, make, nsteps):
-    """CD/CL of every case of the batch in the requested precision against the same batch in
-    FP64 (untimed), as in lift_drag_error: max over cases and steps 2..n."""
+def history_error(ref, run):
+    """Drag and lift errors of one CD/CL history against another (arrays steps × cases), as
+    in lift_drag_error: max over cases and steps 2..n."""
     import numpy as np
 
-    runs = []
-    for prec in ("fp64", None):
-        probs = make(precision="fp64") if prec else make()
-        states = [ibhip.initstate(p) for p in probs]
-        for p, st in zip(probs, states):
-            ibhip.initsolver(p, st)
-        b = ibhip.Batch(states)
-        cds, cls = [], []
-        for k in range(nsteps):
-            b.advance()
-            b.sync()
-            cds.append(np.array([st.case.CD()[0] for st in states]))
-            cls.append(np.array([st.case.CL()[0] for st in states]))
-        b.close()
-        for st in states:
-            st.case.close()
-        runs.append((np.array(cds), np.array(cls)))
-    (cd0, cl0), (cd1, cl1) = runs
+    (cd0, cl0), (cd1, cl1) = ref, run
     scale = np.abs(cd0[1:]).max(axis=0)
-    return {"steps": nsteps,
+    return {"steps": int(cd0.shape[0]),
             "CD_rel": float((np.abs(cd1[1:] - cd0[1:]).max(axis=0) / scale).max()),
             "CL_rel_to_CD": float((np.abs(cl1[1:] - cl0[1:]).max(axis=0) / scale).max()),
             "step1_CD_rel": float((np.abs(cd1[0] - cd0[0]) / np.abs(cd0[0])).max())}
 
 
-def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accuracy=True):
+def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, acc_steps=20):
     """The batched form: the ellipse sweep (64 independent cases of 128²–512² cells, 2–5
     levels) advanced together, one grouped launch per substep, the step replayed from a HIP
-    graph.  A step is one advance! of every case; cell-steps/s counts all cases."""
+    graph.  A step is one advance! of every case; cell-steps/s counts all cases.  The first
+    acc_steps steps (untimed, at least the warmup) record every case's CD and CL."""
+    import numpy as np
+
     from ibhip.workloads import ellipse_batch
 
     make = lambda **kw: ellipse_batch(**{"precision": precision, **kw})  # noqa: E731
@@ -233,10 +221,13 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
     batch.sync()
     t_setup = time.perf_counter() - t0
     log(f"bench: setup {t_setup:.2f} s")
-    warm = max(warmup, 3)  # step 1 tunes, steps 2 and 3 capture the two parity graphs
+    warm = max(warmup, 3, acc_steps)  # step 1 tunes, steps 2 and 3 capture the two graphs
+    cds, cls = [], []
     for _ in range(warm):
         batch.advance()
-    batch.sync()
+        batch.sync()
+        cds.append(np.array([st.case.CD()[0] for st in states]))
+        cls.append(np.array([st.case.CL()[0] for st in states]))
     marker = torch.zeros(1, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -255,7 +246,7 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
         batch.advance()
     batch.sync()
     prof_elapsed = time.perf_counter() - t1
-    kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[precision]
+    kind = KIND[precision]
     ms, launches, flops = batch.profile_read(kind)
     ev_ms, ev_launches, _ = batch.profile_read(kind, events=True)
     records = batch.profile_launches(kind)
@@ -265,17 +256,13 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
     batch.close()
     for st in states:
         st.case.close()
-    acc = None
-    if accuracy and precision != "fp64":
-        acc = batch_lift_drag_error(ibhip, make, 20)
-        log(f"bench: lift/drag error vs FP64 {acc}")
     avg_launch_s = (ms / 1e3) / max(launches, 1)
     achieved = (flops / max(launches, 1)) / avg_launch_s if launches else 0.0
     return {
         "probs": probs, "cells": cells, "npts": npts, "warmup": warm, "elapsed": elapsed,
-        "value": cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+        "value": cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / max(steps, 1),
         "setup_s": t_setup, "launches_per_step": launches_per_step, "tune": tune,
-        "accuracy": acc,
+        "history": (np.array(cds), np.array(cls)),
         "roofline": {
             "bound": "mfma",
             "achieved": achieved / 1e12,
@@ -298,11 +285,17 @@ def measure_batch(torch, ibhip, precision, steps, warmup, profile_steps, accurac
 
 
 DTYPE = {"fp64": "f64", "fp32": "f32 transforms / f64 forces",
-         "bf16x3": "bf16x3 transforms / f64 forces"}
+         "bf16x3": "bf16x3 transforms / f64 forces",
+         "mixed": "f64, f32 transforms on the final Vort2Flux pass's coarse levels"}
 
 
 def run_batch(args, torch, ibhip):
     m = measure_batch(torch, ibhip, args.precision, args.steps, args.warmup, args.profile_steps)
+    m["accuracy"] = None
+    if args.precision != "fp64":
+        ref = measure_batch(torch, ibhip, "fp64", 0, 0, 0)
+        m["accuracy"] = history_error(ref["history"], m["history"])
+        log(f"bench: lift/drag error vs FP64 {m['accuracy']}")
     out = {
         "metric": "cell-steps/s: FP64 vs mixed-precision batched, at ≤1e-6 rel lift/drag err",
         "value": m["value"],
@@ -335,27 +328,29 @@ def run_batch(args, torch, ibhip):
 
 
 def batched_summary(torch, ibhip, args, plain_value):
-    """The metric's comparison: the mixed-precision (FP32 transforms, FP64 forces) batched
-    step and the FP64 batched step on the 64-case sweep, beside the plain FP64 single-case
-    step measured above."""
-    m32 = measure_batch(torch, ibhip, "fp32", args.batch_steps, 3, 2)
-    m64 = measure_batch(torch, ibhip, "fp64", args.batch_steps, 3, 2, accuracy=False)
-    return {
-        "workload": (f"ellipse_batch: {len(m32['probs'])} cases, 128²–512², 2–5 levels, "
-                     f"{m32['cells']} cells per step"),
+    """The metric's comparison on the 64-case sweep, beside the plain FP64 single-case step
+    measured above: the FP64 batched step, the mixed batched step (FP64 where the transform
+    error reaches the forces, FP32 on the final pass's coarse levels) and the FP32 batched
+    step (FP32 transforms refined against FP64 residuals); CD/CL errors against the FP64
+    batch over the first 20 steps."""
+    m = {p: measure_batch(torch, ibhip, p, args.batch_steps, 3, 2)
+         for p in ("fp64", "mixed", "fp32")}
+    out = {
+        "workload": (f"ellipse_batch: {len(m['fp64']['probs'])} cases, 128²–512², 2–5 levels, "
+                     f"{m['fp64']['cells']} cells per step"),
         "steps": args.batch_steps,
-        "fp32_cell_steps_per_s": m32["value"],
-        "fp64_cell_steps_per_s": m64["value"],
-        "fp32_ms_per_step": m32["ms_per_step"],
-        "fp64_ms_per_step": m64["ms_per_step"],
-        "fp32_vs_fp64_batched": m32["value"] / m64["value"],
-        "fp32_batched_vs_plain_fp64_step": m32["value"] / plain_value,
-        "fp32_lift_drag_error_vs_fp64": m32["accuracy"],
-        "fp32_contraction_roofline": {k: m32["roofline"][k] for k in
-                                      ("achieved", "peak", "frac", "avg_launch_us")},
-        "fp64_contraction_roofline": {k: m64["roofline"][k] for k in
-                                      ("achieved", "peak", "frac", "avg_launch_us")},
     }
+    for p in ("fp64", "mixed", "fp32"):
+        out[f"{p}_cell_steps_per_s"] = m[p]["value"]
+        out[f"{p}_ms_per_step"] = m[p]["ms_per_step"]
+        out[f"{p}_batched_vs_plain_fp64_step"] = m[p]["value"] / plain_value
+        if p != "fp64":
+            out[f"{p}_vs_fp64_batched"] = m[p]["value"] / m["fp64"]["value"]
+            out[f"{p}_lift_drag_error_vs_fp64"] = history_error(m["fp64"]["history"],
+                                                                 m[p]["history"])
+        out[f"{p}_contraction_roofline"] = {k: m[p]["roofline"][k] for k in
+                                            ("achieved", "peak", "frac", "avg_launch_us")}
+    return out
 
 
 def main():
@@ -366,7 +361,7 @@ def main():
                          "window as often as the run needs, so its cost is amortized)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cylinder_plate")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=5,
                     help="steps of the profiled (eager) pass that times the contraction kernel")
@@ -449,7 +444,7 @@ def main():
         k += 1
     case.sync()
     prof_elapsed = time.perf_counter() - t1
-    kind = {"fp64": 0, "fp32": 1, "bf16x3": 2}[args.precision]
+    kind = KIND[args.precision]
     ms, launches, flops = case.profile_read(kind)
     ev_ms, ev_launches, _ = case.profile_read(kind, events=True)
     records = case.profile_launches(kind)
@@ -532,8 +527,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 transforms / f64 forces",
-                  "bf16x3": "bf16x3 transforms / f64 forces"}[args.precision],
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {fl.nx}x{fl.ny} cells, {fl.nlevels} levels, "
                                f"{prob.bodies.npoints} body points, Re={fl.Re:g}, dt={dt:g}",

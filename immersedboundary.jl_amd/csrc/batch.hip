@@ -61,6 +61,8 @@ struct Stage {
 struct ibh_batch {
     std::vector<ibh_ctx *> cs;
     int prec = 0, Lmax = 1;
+    // IBH_MIXED cases: the coarse levels of the final Vort2Flux pass run in FP32
+    bool mix = false;
     hipStream_t s = nullptr;
     std::vector<Stage> st[2];
     void *arena = nullptr;
@@ -161,7 +163,8 @@ struct SolveItem {
     const double *spec;  // SPEC: the kept spectrum
 };
 
-void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem> &items) {
+void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem> &items,
+                int prec) {
     std::vector<EdgeA> ed;
     std::vector<SpecA> sp;
     std::vector<LinesA> ln;
@@ -196,13 +199,13 @@ void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem
     }
     add(v, OP_EDGE, ed, edb);
     add(v, OP_SPEC, sp, spb);
-    add_gemm(v, left1, 1, EPI_STORE, b->prec);
-    add_gemm(v, right1, 2, EPI_SCALE, b->prec);
+    add_gemm(v, left1, 1, EPI_STORE, prec);
+    add_gemm(v, right1, 2, EPI_SCALE, prec);
     add(v, OP_LNP, ln, lb[0]);
     add(v, OP_LNC, ln, lb[1]);
     add(v, OP_LNO, ln, lb[2]);
-    add_gemm(v, left2, 1, EPI_STORE, b->prec);
-    add_gemm(v, right2, 2, EPI_STORE, b->prec);
+    add_gemm(v, left2, 1, EPI_STORE, prec);
+    add_gemm(v, right2, 2, EPI_STORE, prec);
 }
 
 // cases whose trial transforms are unrefined keep the trial spectrum of every level (specT)
@@ -210,8 +213,7 @@ void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem
 bool keeps_spectrum(const ibh_ctx *c) { return c->use_eo && c->refine == 0; }
 
 // FP64-residual refinement of the solves on nlev consecutive levels per listed case
-// (x -= Solve(M x − rhs), the correction in the batch precision), as many sweeps as each
-// case's `refine`.
+// (x -= Solve(M x − rhs), the correction in the batch precision), `sweeps` sweeps per item.
 struct RefineItem {
     int i;
     double *x;
@@ -220,18 +222,19 @@ struct RefineItem {
     const double *scale;  // eigenvalue tables of the solves (level stride sscale)
     long sscale;
     int nlev;
+    int sweeps;
 };
 
 void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineItem> &items) {
     int maxref = 0;
-    for (const RefineItem &it : items) maxref = std::max(maxref, b->cs[it.i]->refine);
+    for (const RefineItem &it : items) maxref = std::max(maxref, it.sweeps);
     for (int k = 0; k < maxref; ++k) {
         std::vector<ResidA> ra;
         std::vector<int> bl;
         std::vector<LapInvPlan> plans;
         for (const RefineItem &it : items) {
             ibh_ctx *c = b->cs[it.i];
-            if (c->refine <= k) continue;
+            if (it.sweeps <= k) continue;
             const Dom &d = c->d;
             ra.push_back({d, it.x, it.rhs, it.a, it.a < 0 ? 1.0 : 0.25, c->rres, it.nlev});
             bl.push_back((int)grp_blocks(d.ng) * it.nlev);
@@ -253,8 +256,13 @@ void add_refine(ibh_batch *b, std::vector<Stage> &v, const std::vector<RefineIte
 // without `curl_all` their coarser levels on get_bc!'s lines only (SV_LINES).
 void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double *> &G,
                    const std::vector<double *> &Psi, const std::vector<double *> &Q,
-                   bool curl_all, bool refine = false) {
+                   bool curl_all, bool final_pass) {
     const int n = (int)b->cs.size();
+    // refinement of the trial pass, or of the final pass (its own sweeps and levels)
+    auto sweeps = [&](const ibh_ctx *c) { return final_pass ? c->refine_final : c->refine; };
+    auto rlevels = [&](const ibh_ctx *c) {
+        return final_pass ? c->refine_final_levels : c->refine_levels;
+    };
     auto mode = [&](const ibh_ctx *c, int lev) {
         if (!keeps_spectrum(c)) return (int)SV_FULL;
         if (lev == 0) return (int)SV_SPEC;
@@ -308,8 +316,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
                 rhs = c->gtmp;
                 sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
             }
-            if (refine && lev < c->refine_levels)
-                rf.push_back({i, Psi[i] + lev * d.ng, rhs, -1.0, c->sD, 0, 1});
+            if (sweeps(c) && lev < rlevels(c))
+                rf.push_back({i, Psi[i] + lev * d.ng, rhs, -1.0, c->sD, 0, 1, sweeps(c)});
             if (curl_all || lev == 0) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
@@ -318,7 +326,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
         }
         add(v, OP_GETBC, gb, b1);
         add(v, OP_COPYBC, cp, b2);
-        add_solves(b, v, sv);
+        add_solves(b, v, sv, lev >= 1 && final_pass && b->mix ? IBH_FP32 : b->prec);
         add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
     }
@@ -380,14 +388,14 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
             }
             pI.push_back(P2);
             rf.push_back({i, gs[i], c->rhsb, c->dt / (2 * c->Re * c->h * c->h), c->sAinv, d.ng,
-                          std::min(c->refine_levels, c->L)});
+                          std::min(c->refine_levels, c->L), c->refine});
         }
         add(v, OP_NONLIN, a, bln);
         add(v, OP_RHS, ra, bl);
         add_lapinv(v, pI, b->prec);
         add_refine(b, v, rf);
     }
-    add_vort2flux(b, v, gs, psi, qs, false, true);
+    add_vort2flux(b, v, gs, psi, qs, false, false);
     // ---- BoundaryForcer (timestepping.jl:106-120): Fb = Binv·E(qs + q0)
     {
         std::vector<EA> e;
@@ -748,7 +756,8 @@ int ibh_batch_create(int32_t n, ibh_ctx *const *cases, ibh_batch **out) {
         if (c->use_cg || c->anymove)
             return bad("case " + std::to_string(i) + ": batches take fixed bodies with the dense force solver");
         if (!c->use_eo) return bad("case " + std::to_string(i) + ": batches need the split transforms");
-        if (c->precision != cases[0]->precision) return bad("cases differ in precision");
+        if (c->precision != cases[0]->precision || c->mixed != cases[0]->mixed)
+            return bad("cases differ in precision");
         for (int j = 0; j < i; ++j)
             if (cases[j] == c) return bad("a case appears twice");
         b->cs.push_back(c);
@@ -759,6 +768,7 @@ int ibh_batch_create(int32_t n, ibh_ctx *const *cases, ibh_batch **out) {
         b->nl10.push_back(c->nl[1]);
     }
     b->prec = cases[0]->precision;
+    b->mix = cases[0]->mixed;
     if (const char *ge = std::getenv("IBH_GRAPH")) b->use_graphs = std::atoi(ge) != 0;
     if (const char *gv = std::getenv("IBH_GRP_VARIANT")) b->tune_fixed = std::atoi(gv);
     if (const char *rs = std::getenv("IBH_REG_SCHED")) parse_reg_sched(rs, &b->e_fixed, &b->et_fixed);

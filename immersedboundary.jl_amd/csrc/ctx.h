@@ -92,6 +92,11 @@ struct ibh_ctx {
     double *rhsb = nullptr;  // trial right-hand side, L levels
     int refine = 0;          // FP64-residual refinement sweeps of the low-precision solves
     int refine_levels = 1;   // ... on the finest refine_levels levels
+    // the same for the step's final Vort2Flux pass, whose ψ and q feed only the next step's
+    // nonlinear term (the forces see the trial pass)
+    int refine_final = 0, refine_final_levels = 1;
+    // IBH_MIXED: precision is FP64 and the final pass's coarse levels run in FP32
+    bool mixed = false;
     double *rres = nullptr;  // refinement residuals, L levels
     // scaled spectra kept for the level-1 Poisson solves (trial Ainv over L levels, projection)
     // and the step's work for ψ on get_bc!'s lines and for boundary terms in the sine basis

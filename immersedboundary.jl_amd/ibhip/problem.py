@@ -204,7 +204,7 @@ def circle(center, r, h, frame=None) -> RigidBody:
 
 
 # --------------------------------------------------------------------------- scheme / problem
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "mixed": 3}
 FORCE_SOLVERS = {"auto": 0, "dense": 1, "cg": 2, "pcg": 3}
 
 
@@ -213,7 +213,9 @@ class SchemeCNAB:
     """SchemeCNAB(dt) — src/schemes.jl:19-28, 2-step Adams–Bashforth weights [1.5, -0.5].
 
     The extra keywords select the device implementation and have no reference counterpart:
-    `precision` of the bulk sine transforms ("fp64", "fp32", "bf16x3") and the surface-force
+    `precision` of the bulk sine transforms ("fp64", "fp32", "bf16x3"; "mixed": FP64 for the
+    transforms whose error reaches the forces, FP32 for the coarse levels of the step's final
+    Vort2Flux pass, whose ψ and q feed only the next nonlinear term) and the surface-force
     solver ("auto": the precomputed dense inverse for bodies fixed in the grid, PCG
     otherwise; "dense"; "cg": the reference's unpreconditioned cg!; "pcg": cg! with the dense
     inverse of B as preconditioner, rebuilt for moving bodies every `pc_refresh` steps, or
@@ -221,8 +223,9 @@ class SchemeCNAB:
     cg_maxiter / cg_reltol are the reference's cg! settings (cnab/ops.jl:110); every CG
     variant stops on the same rule ‖b − B·x‖ ≤ reltol·‖b − B·x₀‖.  `refine_sweeps` (FP32 /
     bf16x3 only): FP64-residual refinement sweeps of every low-precision solve (trial Ainv_ℓ,
-    both Vort2Flux Poisson solves) on the finest `refine_levels` levels (0: every level);
-    refine_sweeps = 0 picks 1 for FP32 and 2 for bf16x3, a negative value turns refinement off.
+    both Vort2Flux Poisson solves) on the finest `refine_levels` levels (0: every level; the
+    final Vort2Flux pass refines level 1 only in FP32); refine_sweeps = 0 picks 1 for FP32 and
+    2 for bf16x3, a negative value turns refinement off.
     """
 
     dt: float

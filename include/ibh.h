@@ -39,7 +39,11 @@ enum ibh_status {
 enum ibh_precision {
     IBH_FP64 = 0,   /* every sine transform in FP64 (the drop-in path)                     */
     IBH_FP32 = 1,   /* bulk sine transforms FP32 on MFMA, forces/CG/residuals FP64          */
-    IBH_BF16X3 = 2  /* bulk sine transforms as 3 bf16 MFMA products, FP32 accumulation     */
+    IBH_BF16X3 = 2, /* bulk sine transforms as 3 bf16 MFMA products, FP32 accumulation     */
+    IBH_MIXED = 3   /* FP64 for the transforms whose error reaches the forces (trial Ainv,  */
+                    /* the trial Vort2Flux pass, level 1 of the final pass), FP32 for the   */
+                    /* coarse levels of the final Vort2Flux pass (their ψ and q feed only   */
+                    /* the next step's nonlinear term); no refinement                       */
 };
 
 enum ibh_force_solver {
@@ -93,9 +97,12 @@ typedef struct ibh_case_desc {
                                the transform precision on the levels refine_levels selects
                                (trial Ainv_ℓ and both Vort2Flux Poisson solves), each an FP64
                                5-point residual plus a low-precision correction;
-                               0 → 1 for FP32 and 2 for BF16X3, < 0 → none.  FP64: unused. */
+                               0 → 1 for FP32 and 2 for BF16X3, < 0 → none.  FP64, MIXED:
+                               unused. */
     int32_t refine_levels;  /* FP32/BF16X3: refine the solves of the finest refine_levels
-                               levels; ≤ 0 → every level.  FP64: unused.                   */
+                               levels; ≤ 0 → every level.  The final Vort2Flux pass refines
+                               level 1 only (FP32) or every level (BF16X3).  FP64, MIXED:
+                               unused.                                                      */
 } ibh_case_desc;
 
 typedef struct ibh_ctx ibh_ctx;

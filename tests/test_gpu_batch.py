@@ -51,7 +51,7 @@ def fields(case):
     return f
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3", "mixed"])
 def test_batch_matches_single_cases(precision):
     import ibhip as m
 

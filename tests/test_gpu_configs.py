@@ -79,11 +79,11 @@ def report(label, st, orc, forces):
     return fe
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "mixed"])
 def test_mixed_precision_cylinder_pitching_plate(ibhip, precision):
     # the cylinder + pitching plate geometry on 128² cells, 3 levels; the plate moves, so the
     # forces come from PCG (FP64 B-products) and the bulk transforms are low precision with
-    # FP64-residual refinement.  Both CG iterations stop at 1e-14 so that their histories
+    # FP64-residual refinement (mixed: FP32 on the final pass's coarse levels, unrefined).  Both CG iterations stop at 1e-14 so that their histories
     # differ by less than the precision effect being measured.
     prob = cylinder_plate_case(n=128, nlev=3, precision=precision, cg_reltol=1e-14)
     st, orc, forces = run_with_oracle(ibhip, prob, 5)
@@ -93,11 +93,12 @@ def test_mixed_precision_cylinder_pitching_plate(ibhip, precision):
     assert max(max(v) for v in fe.values()) < 1e-5
 
 
-def test_fp32_ellipse_batch_matches_oracle(ibhip):
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_low_precision_ellipse_batch_matches_oracle(ibhip, precision):
     import ibhip as m
     from ibhip.workloads import ellipse_batch
 
-    probs = ellipse_batch(ncases=3, seed=4, precision="fp32", sizes=(64, 96, 128),
+    probs = ellipse_batch(ncases=3, seed=4, precision=precision, sizes=(64, 96, 128),
                           levels=(2, 3, 4))
     nsteps = 5
     states = [m.initstate(p) for p in probs]
@@ -120,7 +121,8 @@ def test_fp32_ellipse_batch_matches_oracle(ibhip):
             cdr.append(orc.CD())
             clr.append(orc.CL())
         forces = force_errors(cd[i], cl[i], cdr, clr)
-        report(f"batch case {i} ({p.fluid.nx}², L{p.fluid.nlevels}, fp32)", states[i], orc, forces)
+        report(f"batch case {i} ({p.fluid.nx}², L{p.fluid.nlevels}, {precision})", states[i], orc,
+               forces)
         assert max(forces) < 1e-6
 
 
