@@ -130,8 +130,20 @@ __device__ inline int et_entry(int t, int n2, bool pm) {
 // lane l sums the entries l, l + 64, l + 128 (column-major, s fastest), then a fixed-order wave
 // sum; the total is returned in lane 0.  (The reference's sumprod adds the 169 terms in one
 // column-major sweep; the two orders differ by roundoff.)
-__device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__restrict__ q,
-                                double u0, double v0, int t, int lane) {
+// The flux values E reads: stored (QStored) or curl!'s values formed from ψ on the fly (QCurl).
+struct QStored {
+    const double *q;
+    __device__ double operator()(long t) const { return q[t]; }
+};
+struct QCurl {
+    Dom d;
+    const double *psi, *pbc;
+    __device__ double operator()(long t) const { return curl_at(d, psi, pbc, t); }
+};
+
+template <class QA>
+__device__ inline double E_wave(const Dom &d, const RegDev &r, const QA &qa, double u0, double v0,
+                                int t, int lane) {
     const int comp = t / r.nb, k = t % r.nb;
     const int bx = r.bidx[k], by = r.bidx[r.nb + k];
     if (!support_inside(d, bx, by)) {  // uniform over the wave; the host reports the flag
@@ -140,14 +152,14 @@ __device__ inline double E_wave(const Dom &d, const RegDev &r, const double *__r
     }
     const double *wx = r.w + ((long)k * 4 + 2 * comp) * NS;
     const double *wy = wx + NS;
-    const double *base = comp == 0 ? q - 1 : q + d.nu - 1;
+    const long off = comp == 0 ? -1 : d.nu - 1;  // flux index of (i, j) = off + ld·(j−1) + i
     const long ld = comp == 0 ? d.nx + 1 : d.nx;
     const double q0 = comp == 0 ? u0 : v0;
     double acc = 0.0;
     for (int e = lane; e < NS * NS; e += 64) {
         const int tt = e / NS, s = e % NS;
         const int j = by + tt - SUPP;
-        acc += (base[ld * (j - 1) + bx + s - SUPP] + q0) * (wx[s] * wy[tt]);
+        acc += (qa(off + ld * (j - 1) + bx + s - SUPP) + q0) * (wx[s] * wy[tt]);
     }
     return wave_sum(acc);
 }
@@ -162,7 +174,8 @@ __global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__r
                                               const double *uv, double uvscale,
                                               const double *__restrict__ U,
                                               const double *__restrict__ w, int lrk, int t0,
-                                              int tn) {
+                                              int tn, const double *__restrict__ psi, long spsi,
+                                              const double *__restrict__ pbc, long spbc) {
     if (skip && *skip) return;
     if (uv) {  // q0 = freestream·h from the device parameter block
         u0 = uv[0] * uvscale;
@@ -172,7 +185,8 @@ __global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__r
     if (tl >= tn) return;
     const int t = t0 + tl;
     const int b = blockIdx.y;
-    double v = E_wave(d, r, q + b * sq, u0, v0, t, lane);
+    double v = psi ? E_wave(d, r, QCurl{d, psi + b * spsi, pbc + b * spbc}, u0, v0, t, lane)
+                   : E_wave(d, r, QStored{q + b * sq}, u0, v0, t, lane);
     if (lrk > 0) {
         const long n2 = 2L * r.nb;
         double s = 0.0;
@@ -185,13 +199,14 @@ __global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__r
 
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s, const double *uv,
-              double uvscale, const double *U, const double *w, int lrk, int t0, int tn) {
+              double uvscale, const double *U, const double *w, int lrk, int t0, int tn,
+              const double *psi, long spsi, const double *pbc, long spbc) {
     if (r.nb == 0) return;
     if (tn < 0) tn = 2 * r.nb - t0;
     if (tn <= 0) return;
 #define LAUNCH_E(W)                                                                           \
     hipLaunchKernelGGL(k_E<W>, dim3(nblk(tn, W), nbatch), dim3(64 * W), 0, s, d, r, q, sq, u0, \
-                       v0, fb, sfb, skip, uv, uvscale, U, w, lrk, t0, tn)
+                       v0, fb, sfb, skip, uv, uvscale, U, w, lrk, t0, tn, psi, spsi, pbc, spbc)
     switch (E_WAVES[(r.esched >= 0 && r.esched < E_NSCHED) ? r.esched : 0]) {
     case 1: LAUNCH_E(1); break;
     case 2: LAUNCH_E(2); break;
@@ -715,7 +730,8 @@ __global__ __launch_bounds__(64 * W) void kg_E(const EA *args, const int *off, i
     const int t = (blockIdx.x - off[p]) * W + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (t >= 2 * A.r.nb) return;
-    const double v = E_wave(A.d, A.r, A.q, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t, lane);
+    const double v = E_wave(A.d, A.r, QStored{A.q}, A.uv[0] * A.uvscale, A.uv[1] * A.uvscale, t,
+                            lane);
     if (lane == 0) A.fb[t] = v;
 }
 

@@ -20,29 +20,7 @@ __host__ __device__ static inline unsigned nblk(long n, int bs) {
     return (unsigned)((n + bs - 1) / bs);
 }
 
-// curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182 — each q entry written once.
-__device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
-                                 const double *__restrict__ pbc, long t) {
-    const int nx = d.nx, ny = d.ny;
-    double v;
-    if (t < d.nu) {
-        const int i = (int)(t % (nx + 1)) + 1, j = (int)(t / (nx + 1)) + 1;
-        if (i == 1) v = BC(pbc, (d.Lo + 1) + j) - BC(pbc, d.Lo + j);
-        else if (i == nx + 1) v = BC(pbc, (d.Ro + 1) + j) - BC(pbc, d.Ro + j);
-        else if (j == 1) v = GI(psi, i - 1, j) - BC(pbc, d.Bo + i);
-        else if (j == ny) v = BC(pbc, i + d.To) - GI(psi, i - 1, j - 1);
-        else v = GI(psi, i - 1, j) - GI(psi, i - 1, j - 1);
-    } else {
-        const long u = t - d.nu;
-        const int i = (int)(u % nx) + 1, j = (int)(u / nx) + 1;
-        if (j == 1) v = BC(pbc, d.Bo + i) - BC(pbc, (d.Bo + 1) + i);
-        else if (j == ny + 1) v = BC(pbc, d.To + i) - BC(pbc, (d.To + 1) + i);
-        else if (i == 1) v = BC(pbc, d.Lo + j) - GI(psi, i, j - 1);
-        else if (i == nx) v = GI(psi, i - 1, j - 1) - BC(pbc, d.Ro + j);
-        else v = GI(psi, i - 1, j - 1) - GI(psi, i, j - 1);
-    }
-    return v;
-}
+// curl_at (curl!, fluid-ops.jl:138-182) is in ibh_internal.h: E evaluates it too.
 
 __global__ void k_curl_bc(Dom d, const double *__restrict__ psi, long spsi,
                           const double *__restrict__ pbc, long sbc, double *__restrict__ q,

@@ -731,10 +731,9 @@ void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc
     P.g[3].ohi[0] = c->bxb + 1;
     pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
     for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
-    launch_curl_box(d, c->psiB, L * ng, c->zerobc, 0, c->qLB, d.nq, nc, skip, c->s, c->bxa + 1,
-                    c->bxb + 2, c->bya + 1, c->byb + 2);
-    launch_E(d, c->reg, c->qLB, d.nq, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, c->lrEQ,
-             c->lrW, c->lr_k);  // + EQ·w in E's wave sums
+    // E of curl ψ₁ (zero boundary values), the flux formed inside E's wave sums, + EQ·w
+    launch_E(d, c->reg, nullptr, 0, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, c->lrEQ,
+             c->lrW, c->lr_k, 0, -1, c->psiB, L * ng, c->zerobc, 0);
 }
 
 void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,

@@ -27,6 +27,36 @@ inline Dom make_dom(int nx, int ny) {
     return d;
 }
 
+// curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182: the value of flux entry t (u faces first,
+// then v faces; 1-based (i, j) within each block), each q entry written once by its caller.
+// (Inside a HIP source only: the curl kernels and E, which can evaluate q on the fly.)
+#ifdef __HIPCC__
+__device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
+                                 const double *__restrict__ pbc, long t) {
+    const int nx = d.nx, ny = d.ny;
+    auto G = [&](int i, int j) { return psi[(i - 1) + (long)d.m * (j - 1)]; };
+    auto B = [&](int k) { return pbc[k - 1]; };
+    double v;
+    if (t < d.nu) {
+        const int i = (int)(t % (nx + 1)) + 1, j = (int)(t / (nx + 1)) + 1;
+        if (i == 1) v = B((d.Lo + 1) + j) - B(d.Lo + j);
+        else if (i == nx + 1) v = B((d.Ro + 1) + j) - B(d.Ro + j);
+        else if (j == 1) v = G(i - 1, j) - B(d.Bo + i);
+        else if (j == ny) v = B(i + d.To) - G(i - 1, j - 1);
+        else v = G(i - 1, j) - G(i - 1, j - 1);
+    } else {
+        const long u = t - d.nu;
+        const int i = (int)(u % nx) + 1, j = (int)(u / nx) + 1;
+        if (j == 1) v = B(d.Bo + i) - B((d.Bo + 1) + i);
+        else if (j == ny + 1) v = B(d.To + i) - B((d.To + 1) + i);
+        else if (i == 1) v = B(d.Lo + j) - G(i, j - 1);
+        else if (i == nx) v = G(i - 1, j - 1) - B(d.Ro + j);
+        else v = G(i - 1, j - 1) - G(i, j - 1);
+    }
+    return v;
+}
+#endif
+
 // A rectangle of the interior vertices (0-based [x0, x1] × [y0, y1]) and the flux faces a
 // kernel touches around it (1-based [x0, x1 + 3] × [y0, y1 + 3], clipped to each block): the
 // body box of the B-products.  x0 < 0 means the whole grid.
@@ -267,10 +297,14 @@ void launch_reg_update(const Dom &d, const RegDev &r, const double *pts, double 
 // With lrk > 0 also fb[b] += U·w[b] (U 2nb × lrk, w lrk per batch entry): the low-rank
 // coarse-level part of a B-product, summed inside E's wave reduction.  Only the entries
 // [t0, t0 + tn) of each fb[b] are formed (tn < 0: through the end).
+// With psi set, q is not read: each flux value E needs is curl_at(ψ, ψbc) evaluated on the fly
+// (ψ, ψbc: level-1 column stride spsi, spbc), the same values curl! would have stored.
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s,
               const double *uv = nullptr, double uvscale = 1.0, const double *U = nullptr,
-              const double *w = nullptr, int lrk = 0, int t0 = 0, int tn = -1);
+              const double *w = nullptr, int lrk = 0, int t0 = 0, int tn = -1,
+              const double *psi = nullptr, long spsi = 0, const double *pbc = nullptr,
+              long spbc = 0);
 // q = Eᵀ fb (zero, scale, fixed-point scatter, conversion); scale: nbatch doubles of work
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, double *scale, hipStream_t s);
