@@ -52,7 +52,7 @@ __global__ void k_curl_box(Dom d, const double *__restrict__ psi, long spsi,
     if (t >= 2 * nb) return;
     const bool uq = t < nb;
     const long e = uq ? t : t - nb;
-    const int i = i0 + (int)(e % wi), j = j0 + (int)(e / wi);
+    const int i = i0 + imod(e, wi), j = j0 + idiv(e, wi);
     if (uq ? (i > d.nx + 1 || j > d.ny) : (i > d.nx || j > d.ny + 1)) return;
     const long idx = uq ? (i - 1) + (long)(d.nx + 1) * (j - 1) : d.nu + (i - 1) + (long)d.nx * (j - 1);
     q[b * sq + idx] = curl_at(d, psi + b * spsi, pbc + b * sbc, idx);
@@ -73,7 +73,7 @@ void launch_curl_box(const Dom &d, const double *psi, long spsi, const double *p
 
 // rot! (C'), fluid-ops.jl:117-129
 __device__ inline double rot_at(const Dom &d, const double *__restrict__ q, long t) {
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;  // Γ[i-1, j-1]
+    const int i = imod(t, d.m) + 2, j = idiv(t, d.m) + 2;  // Γ[i-1, j-1]
     return (QX(q, i, j - 1) - QX(q, i, j)) + (QY(q, i, j) - QY(q, i - 1, j));
 }
 
@@ -96,7 +96,7 @@ void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int 
 // conversion pass of Eᵀ does ((double)v · (1/scale), inv = 1/scale), then the same differences
 __device__ inline double rot_image_at(const Dom &d, const long long *__restrict__ q, double inv,
                                       long t) {
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    const int i = imod(t, d.m) + 2, j = idiv(t, d.m) + 2;
     const double ux0 = (double)QX(q, i, j - 1) * inv, ux1 = (double)QX(q, i, j) * inv;
     const double vy1 = (double)QY(q, i, j) * inv, vy0 = (double)QY(q, i - 1, j) * inv;
     return (ux0 - ux1) + (vy1 - vy0);
@@ -124,7 +124,7 @@ void launch_rot_image(const Dom &d, const double *img, long sq, const double *sc
 __device__ inline void coarsify_at(const Dom &d, double *__restrict__ gc,
                                    const double *__restrict__ g, long t) {
     const int na = d.nx / 2 - 1;
-    const int a = (int)(t % na), b = (int)(t / na);
+    const int a = imod(t, na), b = idiv(t, na);
     const int i = 2 + 2 * a, j = 2 + 2 * b;
     const int ic = d.nx / 4 + 1 + a, jc = d.ny / 4 + 1 + b;
     GI(gc, ic, jc) = GI(g, i, j) +
@@ -204,7 +204,7 @@ void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc
 // out = in + fac·edges(rbc)  (apply_bc!, fluid-ops.jl:249-275, fused with the copy into Γtmp)
 __device__ inline double copy_apply_bc_at(const Dom &d, const double *__restrict__ in,
                                           const double *__restrict__ rbc, double fac, long t) {
-    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
     double v = in ? in[t] : 0.0;
     // same order as the reference: bottom, top, left, right
     if (j == 1) v += fac * BC(rbc, (d.Bo + 1) + i);
@@ -303,7 +303,7 @@ __device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q
                                       const double *__restrict__ gbc, double gbc_scale,
                                       double U, double V, double hc, long t) {
     const NLGlobal c{d, q, g, gbc, U, V, gbc_scale};
-    const int i = (int)(t % d.m) + 2, j = (int)(t / d.m) + 2;
+    const int i = imod(t, d.m) + 2, j = idiv(t, d.m) + 2;
     double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
     return v * (1 / (hc * hc));
 }
@@ -323,7 +323,7 @@ __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q
     __shared__ double fy[NL_TY][NL_TX + 1];
     const int tx = threadIdx.x % NL_TX, ty = threadIdx.x / NL_TX;
     const int ntx = (d.m + NL_TX - 1) / NL_TX;
-    const int i0 = 2 + (int)(tile % ntx) * NL_TX, j0 = 2 + (int)(tile / ntx) * NL_TY;
+    const int i0 = 2 + imod(tile, ntx) * NL_TX, j0 = 2 + idiv(tile, ntx) * NL_TY;
     // stage: QX, QY, Γ with bounds (1-based valid ranges of each block)
     for (int e = threadIdx.x; e < (NL_TY + 1) * (NL_TX + 2); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
@@ -368,11 +368,13 @@ __global__ __launch_bounds__(256) void k_nonlinear(Dom d, const double *__restri
         ux = uv[0];
         uy = uv[1];
     }
-    const int b = blockIdx.y;
+    // tiles of all levels numbered XCD-contiguously (neighbouring tiles share halo rows)
+    const int id = xcd_block();
+    const int b = id / (int)gridDim.x;
     const int lev = lev0 + b;
     const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
     nonlinear_tile(d, q + b * sq, g + b * sg, gbc + b * sbc, gbc_scale,
-                   q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, blockIdx.x,
+                   q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, id % (int)gridDim.x,
                    out + b * sout);
 }
 
@@ -420,7 +422,7 @@ __device__ inline double rhs_at(const Dom &d, const double *__restrict__ g,
                                 const double *__restrict__ N1, const double *__restrict__ N2,
                                 const double *__restrict__ bc, double a, double f1, double f2,
                                 long t) {
-    const int i = (int)(t % d.m), j = (int)(t / d.m);  // 0-based
+    const int i = imod(t, d.m), j = idiv(t, d.m);  // 0-based
     const double c = g[t];
     const double nsum = (i > 0 ? g[t - 1] : 0.0) + (i < d.m - 1 ? g[t + 1] : 0.0) +
                         (j > 0 ? g[t - d.m] : 0.0) + (j < d.p - 1 ? g[t + d.m] : 0.0);
@@ -460,7 +462,7 @@ void launch_rhs(const Dom &d, const double *g, const double *N1, const double *N
 // M = −L (Poisson, a < 0) or M = I + a(−L) (the implicit CN operator, a ≥ 0).
 __device__ inline double resid_at(const Dom &d, const double *__restrict__ x,
                                   const double *__restrict__ rhs, double a, long t) {
-    const int i = (int)(t % d.m), j = (int)(t / d.m);
+    const int i = imod(t, d.m), j = idiv(t, d.m);
     const double c = x[t];
     const double nsum = (i > 0 ? x[t - 1] : 0.0) + (i < d.m - 1 ? x[t + 1] : 0.0) +
                         (j > 0 ? x[t - d.m] : 0.0) + (j < d.p - 1 ? x[t + d.m] : 0.0);
@@ -562,7 +564,7 @@ __global__ void k_edge_add(Dom d, const double *__restrict__ Sm, const double *_
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    const int a = (int)(t % d.m), c = (int)(t / d.m);
+    const int a = imod(t, d.m), c = idiv(t, d.m);
     F[b * sF + t] += D[t] * edge_term(d, Sm, Sp, W + b * sW, a, c);
 }
 
@@ -760,7 +762,7 @@ __global__ void k_spectral_scale(Dom d, const double *__restrict__ A, long sA,
     const int b = blockIdx.y;
     double v = kappa * A[b * sA + t] * D[t];
     if (EW) {
-        const int a = (int)(t % d.m), c = (int)(t / d.m);
+        const int a = imod(t, d.m), c = idiv(t, d.m);
         v += D[t] * edge_term(d, Sm, Sp, EW + b * 2L * (d.m + d.p), a, c);
     }
     F[b * sF + t] = v;
@@ -842,7 +844,7 @@ __global__ void k_scale_tables(Dom d, double *sD, double *sA, double *sAinv, int
                                double dt, double Re, double scale) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
-    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
     const double lam = -2 * (cos(M_PI * i / d.nx) + cos(M_PI * j / d.ny) - 2);
     sD[t] = scale / lam;
     for (int l = 0; l < L; ++l) {
@@ -865,10 +867,12 @@ void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, in
 // ------------------------------------------------------------------------ grouped launches
 // The same per-element bodies over a list of problems of different sizes: the workgroup finds
 // its problem, then its batch entry b and element t inside that problem.
+// grouped kernels take their workgroup id XCD-contiguously (xcd_block)
+#define XB xcd_block()
 #define GRP_PROLOGUE(TYPE, ELEMS)                                    \
-    const int p_ = grp_find(off, n, blockIdx.x);                     \
+    const int p_ = grp_find(off, n, XB);                     \
     const TYPE A = args[p_];                                         \
-    const int lb_ = blockIdx.x - off[p_];                            \
+    const int lb_ = XB - off[p_];                            \
     const int nbx_ = (int)nblk((ELEMS), GRP_BS);                     \
     const int b = lb_ / nbx_;                                        \
     const long t = (long)(lb_ % nbx_) * GRP_BS + threadIdx.x;        \
@@ -879,9 +883,9 @@ void launch_scale_tables(const Dom &d, double *sD, double *sA, double *sAinv, in
 // formed before any is stored, so the loads of all of them are in flight together.  Defines A,
 // d, b and the thread's first element t0; workgroups past the problem's batch entries leave.
 #define GRP_PROLOGUE_N(TYPE, ELEMS)                                  \
-    const int p_ = grp_find(off, n, blockIdx.x);                     \
+    const int p_ = grp_find(off, n, XB);                     \
     const TYPE A = args[p_];                                         \
-    const int lb_ = blockIdx.x - off[p_];                            \
+    const int lb_ = XB - off[p_];                            \
     const int nbx_ = (int)grp_blocks(ELEMS);                         \
     const int b = lb_ / nbx_;                                        \
     const Dom &d = A.d;                                              \
@@ -942,11 +946,11 @@ __global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, 
 // nl_tiles(d) workgroups per batch entry (level)
 __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, const int *off, int n) {
     static_assert(GRP_BS == NL_TX * NL_TY, "one thread per tile output");
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const NonlinA A = args[p];
-    const long nt = nl_tiles(A.d);
-    const long lb = blockIdx.x - off[p];
-    const int b = (int)(lb / nt);
+    const int nt = (int)nl_tiles(A.d);
+    const int lb = XB - off[p];
+    const int b = lb / nt;
     const double hc = A.h * (double)(1L << (A.lev0 + b));
     const double ux = A.uv[0], uy = A.uv[1];
     nonlinear_tile(A.d, A.q + b * A.sq, A.g + b * A.sg, A.gbc + b * A.sbc, A.gbc_scale, ux * hc,
@@ -970,17 +974,17 @@ __global__ __launch_bounds__(GRP_BS) void kg_resid(const ResidA *args, const int
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_bc_prefix(const BcPrefA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const BcPrefA A = args[p];
-    const int t = (blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const int t = (XB - off[p]) * GRP_BS + threadIdx.x;
     if (t >= A.nbc) return;
     bc_prefix_at(A.nbc, A.bcraw, A.acc, A.fac, A.L, t);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_fill(const FillA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const FillA A = args[p];
-    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const long t = (long)(XB - off[p]) * GRP_BS + threadIdx.x;
     if (t < A.n) A.p[t] = A.v;
 }
 
@@ -1006,9 +1010,9 @@ static_assert(GRP_BS == 256, "the edge and lines passes run 256-thread workgroup
 
 // m + p workgroups per problem
 __global__ __launch_bounds__(GRP_BS) void kg_edge_gemv(const EdgeA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const EdgeA A = args[p];
-    edge_gemv_row(A.d, A.Sm, A.Sp, A.bc, A.W, blockIdx.x - off[p]);
+    edge_gemv_row(A.d, A.Sm, A.Sp, A.bc, A.W, XB - off[p]);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, const int *off,
@@ -1017,7 +1021,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, c
     (void)b;  // one batch entry per problem
     if (t >= d.ng) return;
     double v = A.kappa * A.A[t] * A.D[t];
-    if (A.EW) v += A.D[t] * edge_term(d, A.Sm, A.Sp, A.EW, (int)(t % d.m), (int)(t / d.m));
+    if (A.EW) v += A.D[t] * edge_term(d, A.Sm, A.Sp, A.EW, imod(t, d.m), idiv(t, d.m));
     A.F[t] = v;
 }
 
@@ -1028,9 +1032,9 @@ long grp_lines_blocks(const Dom &d, int pass) {
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const LinesA A = args[p];
-    const int blk = blockIdx.x - off[p], nca = (int)nblk(A.d.m, 256);
+    const int blk = XB - off[p], nca = (int)nblk(A.d.m, 256);
     if (blk < nca * LN_CH)
         lines_colpart(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0, (blk % nca) * 256 + threadIdx.x,
                       blk / nca, 0);
@@ -1040,23 +1044,24 @@ __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, cons
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, const int *off,
                                                           int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const LinesA A = args[p];
-    lines_colsum(A.d, A.W, (blockIdx.x - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6));
+    lines_colsum(A.d, A.W, (XB - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6));
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_out(const LinesA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const LinesA A = args[p];
-    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, blockIdx.x - off[p]);
+    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, XB - off[p]);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_axpy_sub(const AxpyA *args, const int *off, int n) {
-    const int p = grp_find(off, n, blockIdx.x);
+    const int p = grp_find(off, n, XB);
     const AxpyA A = args[p];
-    const long t = (long)(blockIdx.x - off[p]) * GRP_BS + threadIdx.x;
+    const long t = (long)(XB - off[p]) * GRP_BS + threadIdx.x;
     if (t < A.n) A.y[t] -= A.x[t] * A.a;
 }
+#undef XB
 
 GRP_LAUNCH(launch_grp_edge_gemv, kg_edge_gemv, EdgeA)
 GRP_LAUNCH(launch_grp_spectral_scale, kg_spectral_scale, SpecA)
@@ -1117,7 +1122,7 @@ __global__ void k_get_bc_adj(Dom d, const double *__restrict__ bc, long sbc,
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const double *B = bc + blockIdx.y * sbc;
-    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
     const int nx = d.nx, ny = d.ny;
     double v = 0.0;
     if (j == ny / 4) v += gbc_adj_line(B, d.Bo, i - nx / 4, nx / 2);
@@ -1154,7 +1159,7 @@ __global__ void k_coarsify_adj(Dom d, const double *__restrict__ y, long sy,
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const double *Y = y + blockIdx.y * sy;
-    const int i = (int)(t % d.m) + 1, j = (int)(t / d.m) + 1;
+    const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
     int ai[2], bj[2];
     double wi[2], wj[2];
     const int ni = coarsify_adj_taps(i, d.nx / 2 - 1, ai, wi);
@@ -1180,8 +1185,8 @@ __global__ void k_box_diff(Dom d, const double *__restrict__ a, const double *__
     if (t >= d.nq) return;
     int i, j;
     if (t < d.nu) {
-        i = (int)(t % (d.nx + 1)) + 1;
-        j = (int)(t / (d.nx + 1)) + 1;
+        i = imod(t, (d.nx + 1)) + 1;
+        j = idiv(t, (d.nx + 1)) + 1;
     } else {
         i = (int)((t - d.nu) % d.nx) + 1;
         j = (int)((t - d.nu) / d.nx) + 1;
@@ -1203,7 +1208,7 @@ __global__ void k_fill_random_box(double *p, long stride, int m, int p_, unsigne
                                   int xa, int xb, int ya, int yb) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= (long)m * p_) return;
-    const int i = (int)(t % m), j = (int)(t / m);
+    const int i = imod(t, m), j = idiv(t, m);
     double v = 0.0;
     if (i >= xa && i <= xb && j >= ya && j <= yb) {
         unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (1ull + t + (unsigned long long)blockIdx.y * (unsigned long long)m * p_);

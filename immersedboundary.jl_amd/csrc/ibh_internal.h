@@ -27,6 +27,24 @@ inline Dom make_dom(int nx, int ny) {
     return d;
 }
 
+// Index arithmetic of the grid kernels: an element index within one level (always < 2^31)
+// split by a run-time extent in 32-bit unsigned arithmetic, which the compiler expands to a
+// few instructions (a 64-bit division is a long sequence).
+#ifdef __HIPCC__
+__device__ __forceinline__ int idiv(long t, long n) { return (int)((unsigned)t / (unsigned)n); }
+__device__ __forceinline__ int imod(long t, long n) { return (int)((unsigned)t % (unsigned)n); }
+// The linear workgroup id (over a 1-D or 2-D grid) renumbered so that each XCD runs one
+// contiguous range of ids: the hardware deals workgroup i to XCD i mod 8, so without this the
+// neighbouring tiles of a stencil land on different XCDs and fetch their shared halo rows into
+// different L2s.  A bijection of [0, total).
+__device__ __forceinline__ int xcd_block() {
+    const int total = gridDim.x * gridDim.y, q = total >> 3, r = total & 7;
+    const int lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const int x = lin & 7, slot = lin >> 3;
+    return x * q + min(x, r) + slot;
+}
+#endif
+
 // curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182: the value of flux entry t (u faces first,
 // then v faces; 1-based (i, j) within each block), each q entry written once by its caller.
 // (Inside a HIP source only: the curl kernels and E, which can evaluate q on the fly.)
@@ -38,7 +56,7 @@ __device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
     auto B = [&](int k) { return pbc[k - 1]; };
     double v;
     if (t < d.nu) {
-        const int i = (int)(t % (nx + 1)) + 1, j = (int)(t / (nx + 1)) + 1;
+        const int i = imod(t, nx + 1) + 1, j = idiv(t, nx + 1) + 1;
         if (i == 1) v = B((d.Lo + 1) + j) - B(d.Lo + j);
         else if (i == nx + 1) v = B((d.Ro + 1) + j) - B(d.Ro + j);
         else if (j == 1) v = G(i - 1, j) - B(d.Bo + i);
@@ -46,7 +64,7 @@ __device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
         else v = G(i - 1, j) - G(i - 1, j - 1);
     } else {
         const long u = t - d.nu;
-        const int i = (int)(u % nx) + 1, j = (int)(u / nx) + 1;
+        const int i = imod(u, nx) + 1, j = idiv(u, nx) + 1;
         if (j == 1) v = B(d.Bo + i) - B((d.Bo + 1) + i);
         else if (j == ny + 1) v = B(d.To + i) - B((d.To + 1) + i);
         else if (i == 1) v = B(d.Lo + j) - G(i, j - 1);
