@@ -63,6 +63,7 @@ struct ibh_batch {
     int prec = 0, Lmax = 1;
     // IBH_MIXED cases: the coarse levels of the final Vort2Flux pass run in FP32
     bool mix = false;
+    std::vector<NonlinA> nl_patch;  // nonlinear-term arguments reading a case's stored q
     hipStream_t s = nullptr;
     std::vector<Stage> st[2];
     void *arena = nullptr;
@@ -318,7 +319,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             }
             if (sweeps(c) && lev < rlevels(c))
                 rf.push_back({i, Psi[i] + lev * d.ng, rhs, -1.0, c->sD, 0, 1, sweeps(c)});
-            if (curl_all || lev == 0) {
+            // the final pass stores no flux: the next nonlinear term forms it from ψ and ψbc
+            if (!final_pass && (curl_all || lev == 0)) {
                 cu.push_back({d, Psi[i] + lev * d.ng, 0, bcl, 0, Q[i] + (curl_all ? lev * d.nq : 0),
                               0, 1});
                 cb.push_back((int)grp_blocks(d.nq));
@@ -372,8 +374,10 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
             const Dom &d = c->d;
+            // the previous step's q formed from its ψ and ψbc (batch_step patches in the stored
+            // q for a case whose q was set)
             a.push_back({d, c->q, d.nq, gam[i], d.ng, c->bcraw, d.nbc, 0.25, n0[i], d.ng, c->h, 0,
-                         b->uv, c->L});
+                         b->uv, c->L, c->psi, d.ng, c->psibc, d.nbc});
             bl.push_back((int)grp_blocks(d.ng) * c->L);
             bln.push_back((int)(nl_tiles(d) * c->L));
             // A_ℓΓ as its exact stencil fused with the right-hand side, then Ainv_ℓ
@@ -693,6 +697,23 @@ int batch_step(ibh_batch *b, double ux, double uy) {
     launch_fill(b->uv, 1, ux, b->s);
     launch_fill(b->uv + 1, 1, uy, b->s);
     std::vector<Stage> &v = b->st[P];
+    // a case whose q was set through ibh_set_field has its nonlinear term read the stored q
+    // for this step: its argument entry loses ψ (restored after the step)
+    Stage *nl = nullptr;
+    for (Stage &S : v)
+        if (S.op == OP_NONLIN) nl = &S;
+    std::vector<int> patched;
+    char *nl_dev = nl ? const_cast<char *>(static_cast<const char *>(nl->d_args)) : nullptr;
+    b->nl_patch.resize(n);
+    for (int i = 0; i < n && nl; ++i)
+        if (b->cs[i]->q_given) {
+            NonlinA &a = b->nl_patch[i];
+            std::memcpy(&a, nl->args.data() + i * sizeof(NonlinA), sizeof a);
+            a.psi = nullptr;
+            BCK(hipMemcpyAsync(nl_dev + i * sizeof(NonlinA), &a, sizeof a, hipMemcpyHostToDevice,
+                               b->s));
+            patched.push_back(i);
+        }
     const bool eager = !b->use_graphs || b->prof.on || !b->tuned;
     int rc;
     if (eager) {
@@ -719,8 +740,13 @@ int batch_step(ibh_batch *b, double ux, double uy) {
         }
         BCK(hipGraphLaunch(b->graph[P], b->s));
     }
+    for (int i : patched)
+        BCK(hipMemcpyAsync(nl_dev + i * sizeof(NonlinA), nl->args.data() + i * sizeof(NonlinA),
+                           sizeof(NonlinA), hipMemcpyHostToDevice, b->s));
     // the cases' host-side state follows what ran: cycle!(nonlin), Γ = Γs, t, step count
     for (ibh_ctx *c : b->cs) {
+        c->q_given = false;
+        c->q_stale = true;
         std::swap(c->nl[0], c->nl[1]);
         std::swap(c->gam, c->gs);
         c->parity ^= 1;

@@ -97,6 +97,12 @@ struct ibh_ctx {
     int refine_final = 0, refine_final_levels = 1;
     // IBH_MIXED: precision is FP64 and the final pass's coarse levels run in FP32
     bool mixed = false;
+    // The state's flux q is curl!(ψ, ψbc) of the final Vort2Flux pass.  The step does not store
+    // it: the next nonlinear term forms it from ψ and ψbc while staging, and ibh_get_field
+    // forms it on demand (q_stale: q's buffer does not hold it yet).  q_given: q (or ψ) was set
+    // through ibh_set_field, so the next nonlinear term reads the stored q, as the reference's
+    // step reads its state's q.
+    bool q_stale = false, q_given = false;
     double *rres = nullptr;  // refinement residuals, L levels
     // scaled spectra kept for the level-1 Poisson solves (trial Ainv over L levels, projection)
     // and the step's work for ψ on get_bc!'s lines and for boundary terms in the sine basis

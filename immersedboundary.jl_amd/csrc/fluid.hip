@@ -314,10 +314,14 @@ __device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q
 // vector caches; then each thread forms the face products it owns, fqx(i, j) and fqy(i, j),
 // and the tile's first row and column also the ones just below and left of the tile, so every
 // product is formed once; the sum is nonlinear_at's, term for term.
+// With psi set, the staged flux values are curl!(ψ, ψbc)'s (curl_u, curl_v: the values the curl
+// kernels would have stored in q), and q is not read.
 __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q,
                                       const double *__restrict__ g,
                                       const double *__restrict__ gbc, double gbc_scale, double U,
-                                      double V, double hc, long tile, double *__restrict__ out) {
+                                      double V, double hc, long tile, double *__restrict__ out,
+                                      const double *__restrict__ psi = nullptr,
+                                      const double *__restrict__ pbc = nullptr) {
     __shared__ NLTileMem mem;
     __shared__ double fx[NL_TY + 1][NL_TX];
     __shared__ double fy[NL_TY][NL_TX + 1];
@@ -328,12 +332,16 @@ __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q
     for (int e = threadIdx.x; e < (NL_TY + 1) * (NL_TX + 2); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
         const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-        mem.qx[r][cidx] = (i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny) ? QX(q, i, j) : 0.0;
+        mem.qx[r][cidx] = (i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny)
+                              ? (psi ? curl_u(d, psi, pbc, i, j) : QX(q, i, j))
+                              : 0.0;
     }
     for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 1); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 1), cidx = e % (NL_TX + 1);
         const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-        mem.qy[r][cidx] = (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1) ? QY(q, i, j) : 0.0;
+        mem.qy[r][cidx] = (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1)
+                              ? (psi ? curl_v(d, psi, pbc, i, j) : QY(q, i, j))
+                              : 0.0;
     }
     for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 2); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
@@ -363,7 +371,9 @@ __global__ __launch_bounds__(256) void k_nonlinear(Dom d, const double *__restri
                                                    double gbc_scale, double *__restrict__ out,
                                                    long sout, double ux, double uy, double h,
                                                    int lev0, int q0_given,
-                                                   const double *__restrict__ uv) {
+                                                   const double *__restrict__ uv,
+                                                   const double *__restrict__ psi, long spsi,
+                                                   const double *__restrict__ pbc, long spbc) {
     if (uv) {  // freestream from the device parameter block (graph-replayable steps)
         ux = uv[0];
         uy = uv[1];
@@ -375,16 +385,18 @@ __global__ __launch_bounds__(256) void k_nonlinear(Dom d, const double *__restri
     const double hc = h * (double)(1L << lev);  // gridstep, fluids/fluids.jl:102
     nonlinear_tile(d, q + b * sq, g + b * sg, gbc + b * sbc, gbc_scale,
                    q0_given ? ux : ux * hc, q0_given ? uy : uy * hc, hc, id % (int)gridDim.x,
-                   out + b * sout);
+                   out + b * sout, psi ? psi + b * spsi : nullptr, psi ? pbc + b * spbc : nullptr);
 }
 
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
                       const double *gbc, long sbc, double gbc_scale, double *out, long sout,
                       double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
-                      int q0_given, const double *uv) {
+                      int q0_given, const double *uv, const double *psi, long spsi,
+                      const double *pbc, long spbc) {
     static_assert(NL_TX * NL_TY == 256, "one thread per tile output");
     hipLaunchKernelGGL(k_nonlinear, dim3((unsigned)nl_tiles(d), nbatch), dim3(256), 0, s, d, q,
-                       sq, g, sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given, uv);
+                       sq, g, sg, gbc, sbc, gbc_scale, out, sout, ux, uy, h, lev0, q0_given, uv,
+                       psi, spsi, pbc, spbc);
 }
 
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac[l]·bcraw[l] for ℓ ≤ L-2, accbc[L-1] = 0 (the rhsbc of
@@ -954,7 +966,8 @@ __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, cons
     const double hc = A.h * (double)(1L << (A.lev0 + b));
     const double ux = A.uv[0], uy = A.uv[1];
     nonlinear_tile(A.d, A.q + b * A.sq, A.g + b * A.sg, A.gbc + b * A.sbc, A.gbc_scale, ux * hc,
-                   uy * hc, hc, lb % nt, A.out + b * A.sout);
+                   uy * hc, hc, lb % nt, A.out + b * A.sout, A.psi ? A.psi + b * A.spsi : nullptr,
+                   A.psi ? A.pbc + b * A.spbc : nullptr);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_rhs(const RhsA *args, const int *off, int n) {

@@ -566,6 +566,8 @@ void pair_range(int lo, int hi, int n, int *plo, int *phi) {
     *phi = (lo <= h - 1 && hi >= h - 1) ? h : std::max(f(lo), f(hi)) + 1;
 }
 
+// `skip_q`: ψ on every level as with curl_all, but no flux is stored (the step's final pass,
+// whose q the next nonlinear term forms from ψ and ψbc).
 // `nlev` > 0 solves only the finest nlev levels (the coarsest of them with zero boundary
 // values), as for a domain of nlev levels; the arrays keep the case's L-level layout.
 // Options that change only the order of roundoff:
@@ -580,7 +582,7 @@ void pair_range(int lo, int hi, int n, int *plo, int *phi) {
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
                const int *skip, int prec = -1, int refine = 0, int nlev = -1,
                bool sparse = false, const double *spec0 = nullptr, bool lines_ok = false,
-               int rlev = -1, int prec_coarse = -1) {
+               int rlev = -1, int prec_coarse = -1, bool skip_q = false) {
     const Dom &d = c->d;
     if (rlev < 0) rlev = c->refine_levels;  // refined levels (the finest rlev)
     const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
@@ -650,7 +652,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
                       L > 1 ? prec_coarse : prec0);
     // E reads level 1 only inside the body box in a B-product (whose Poisson solve formed ψ there)
     const bool qbox = !curl_all && sparse && c->box_on && c->use_eo && refine == 0;
-    if (curl_all || L == 1) {
+    if ((curl_all || L == 1) && !skip_q) {
         if (qbox && L == 1)
             launch_curl_box(d, Psi, sG, bcL, sB, Q, sQ, nc, skip, c->s, c->bxa + 1, c->bxb + 2,
                             c->bya + 1, c->byb + 2);
@@ -665,7 +667,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         if (refine && lev < rlev)
             refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine,
                           lev >= 1 ? prec_coarse : prec0);
-        if (curl_all || lev == 0) {
+        if ((curl_all || lev == 0) && !skip_q) {
             if (qbox && lev == 0)
                 launch_curl_box(d, Psi, sG, bcl, sB, Q, sQ, nc, skip, c->s, c->bxa + 1,
                                 c->bxb + 2, c->bya + 1, c->byb + 2);
@@ -1383,8 +1385,9 @@ void seg_trial(ibh_ctx *c) {
         launch_bc_prefix(d, c->bcraw, c->accbc, c->fac, L, c->s);
     }
     // bc of the coarsest level is zero; bcraw[L-1] stays zero
+    // q of the previous step formed from its ψ and ψbc (or the stored q the caller set)
     launch_nonlinear(d, c->q, d.nq, c->gam, ng, c->bcraw, d.nbc, 0.25, c->nl[0], ng, 0.0, 0.0, c->h,
-                     0, L, c->s, 0, c->uv);
+                     0, L, c->s, 0, c->uv, c->q_given ? nullptr : c->psi, ng, c->psibc, d.nbc);
     RhsExtras rx{c->nl[0], c->nl[1], c->accbc, c->dt * c->beta[0], c->dt * c->beta[1]};
     // A_ℓΓ by its exact stencil, fused with the right-hand side terms; then Ainv_ℓ
     launch_rhs(d, c->gam, rx.N1, rx.N2, rx.bcacc, c->rhsb, L, c->h, c->dt, c->Re, rx.f1, rx.f2,
@@ -1485,7 +1488,7 @@ void seg_post(ibh_ctx *c) {
     // flux feeds the next step's nonlinear term next to the bodies)
     vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr, -1, c->refine_final, -1,
               false, fuse ? c->specT : nullptr, false, c->refine_final_levels,
-              c->mixed ? IBH_FP32 : -1);
+              c->mixed ? IBH_FP32 : -1, true);
 }
 
 // status word: [CG done, CG iterations, error flags]
@@ -1561,7 +1564,7 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
     const bool refresh = c->use_pcg && (!c->pc_valid || c->steps_since_pc >= c->pc_refresh);
     // graphs from the second step on (the first one runs the autotuner), not while profiling
     // and not on steps that rebuild the preconditioner
-    const bool g = c->use_graphs && !c->prof.on && !refresh && c->steps_done >= 1;
+    const bool g = c->use_graphs && !c->prof.on && !refresh && c->steps_done >= 1 && !c->q_given;
     const int par = c->parity;
     // CG iterations: k1 in the trial segment; for PCG then as many as the last step needed
     // (the count drifts slowly), queued as chunks of 8, 4, 2 and 1 before the first status
@@ -1637,6 +1640,8 @@ int step(ibh_ctx *c, double t, double ux, double uy) {
         }
     }
     if ((rc = run_seg(c, &c->gPost[par], g, [&] { seg_post(c); }, post_swap))) return rc;
+    c->q_given = false;
+    c->q_stale = true;
     c->parity ^= 1;
     ++c->steps_done;
     c->t = t;
@@ -1966,6 +1971,8 @@ int ibh_init_state(ibh_ctx *c, double t0, double ux, double uy) {
     HIPCK(hipMemsetAsync(c->q, 0, sizeof(double) * L * c->d.nq, c->s));
     HIPCK(hipMemsetAsync(c->gam, 0, sizeof(double) * L * c->d.ng, c->s));
     HIPCK(hipMemsetAsync(c->psi, 0, sizeof(double) * L * c->d.ng, c->s));
+    HIPCK(hipMemsetAsync(c->psibc, 0, sizeof(double) * L * c->d.nbc, c->s));
+    c->q_stale = c->q_given = false;
     HIPCK(hipMemsetAsync(c->nl[0], 0, sizeof(double) * L * c->d.ng, c->s));
     HIPCK(hipMemsetAsync(c->nl[1], 0, sizeof(double) * L * c->d.ng, c->s));
     if (c->nb) {
@@ -1996,6 +2003,16 @@ int ibh_setup(ibh_ctx *c) {
     c->setup_done = true;
     HIPCK(hipStreamSynchronize(c->s));
     return IBH_OK;
+}
+
+// q = curl!(ψ, ψbc) on every level into the state's flux buffer, if the last step left it to be
+// formed on demand
+static hipError_t materialize_q(ibh_ctx *c) {
+    if (!c->q_stale) return hipSuccess;
+    const Dom &d = c->d;
+    launch_curl_bc(d, c->psi, d.ng, c->psibc, d.nbc, c->q, d.nq, c->L, nullptr, c->s);
+    c->q_stale = false;
+    return hipGetLastError();
 }
 
 int ibh_set_body_points(ibh_ctx *c, const double *r, const double *v) {
@@ -2039,6 +2056,7 @@ int ibh_get_field(ibh_ctx *c, int32_t which, double *out) {
     if (!c || !out) return IBH_EINVAL;
     const long n = field_len(c, which);
     if (n < 0) return fail(c, IBH_EINVAL, "unknown field");
+    if (which == IBH_F_Q) HIPCK(materialize_q(c));
     HIPCK(hipStreamSynchronize(c->s));
     // a device error flag still set (e.g. from ibh_advance_n or an operator call) is returned
     // after the copy, so the caller gets both the values and the error
@@ -2063,6 +2081,12 @@ int ibh_set_field(ibh_ctx *c, int32_t which, const double *in) {
     if (n < 0 || which == IBH_F_Q0 || which == IBH_F_CD || which == IBH_F_CL)
         return fail(c, IBH_EINVAL, "field cannot be set");
     if (n == 0) return IBH_OK;
+    // the next step reads q as stored: a new ψ must not change the q it sees
+    if (which == IBH_F_PSI) HIPCK(materialize_q(c));
+    if (which == IBH_F_Q || which == IBH_F_PSI) {
+        c->q_given = true;
+        c->q_stale = false;
+    }
     if (which == IBH_F_POINTS_R) {
         int rc = check_points(c, in);
         if (rc) return rc;

@@ -45,33 +45,38 @@ __device__ __forceinline__ int xcd_block() {
 }
 #endif
 
-// curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182: the value of flux entry t (u faces first,
-// then v faces; 1-based (i, j) within each block), each q entry written once by its caller.
-// (Inside a HIP source only: the curl kernels and E, which can evaluate q on the fly.)
+// curl!(q, ψ, ψbc, domain), fluid-ops.jl:138-182: the u face (i, j) and the v face (i, j)
+// (1-based within each block), and the value of flux entry t (u faces first, then v faces);
+// each q entry is written once by its caller.  (Inside a HIP source only: the curl kernels, E
+// and the nonlinear term evaluate it.)
 #ifdef __HIPCC__
+__device__ inline double curl_u(const Dom &d, const double *__restrict__ psi,
+                                const double *__restrict__ pbc, int i, int j) {
+    const int nx = d.nx, ny = d.ny;
+    auto G = [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; };
+    auto B = [&](int k) { return pbc[k - 1]; };
+    if (i == 1) return B((d.Lo + 1) + j) - B(d.Lo + j);
+    if (i == nx + 1) return B((d.Ro + 1) + j) - B(d.Ro + j);
+    if (j == 1) return G(i - 1, j) - B(d.Bo + i);
+    if (j == ny) return B(i + d.To) - G(i - 1, j - 1);
+    return G(i - 1, j) - G(i - 1, j - 1);
+}
+__device__ inline double curl_v(const Dom &d, const double *__restrict__ psi,
+                                const double *__restrict__ pbc, int i, int j) {
+    const int nx = d.nx, ny = d.ny;
+    auto G = [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; };
+    auto B = [&](int k) { return pbc[k - 1]; };
+    if (j == 1) return B(d.Bo + i) - B((d.Bo + 1) + i);
+    if (j == ny + 1) return B(d.To + i) - B((d.To + 1) + i);
+    if (i == 1) return B(d.Lo + j) - G(i, j - 1);
+    if (i == nx) return G(i - 1, j - 1) - B(d.Ro + j);
+    return G(i - 1, j - 1) - G(i, j - 1);
+}
 __device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
                                  const double *__restrict__ pbc, long t) {
-    const int nx = d.nx, ny = d.ny;
-    auto G = [&](int i, int j) { return psi[(i - 1) + (long)d.m * (j - 1)]; };
-    auto B = [&](int k) { return pbc[k - 1]; };
-    double v;
-    if (t < d.nu) {
-        const int i = imod(t, nx + 1) + 1, j = idiv(t, nx + 1) + 1;
-        if (i == 1) v = B((d.Lo + 1) + j) - B(d.Lo + j);
-        else if (i == nx + 1) v = B((d.Ro + 1) + j) - B(d.Ro + j);
-        else if (j == 1) v = G(i - 1, j) - B(d.Bo + i);
-        else if (j == ny) v = B(i + d.To) - G(i - 1, j - 1);
-        else v = G(i - 1, j) - G(i - 1, j - 1);
-    } else {
-        const long u = t - d.nu;
-        const int i = imod(u, nx) + 1, j = idiv(u, nx) + 1;
-        if (j == 1) v = B(d.Bo + i) - B((d.Bo + 1) + i);
-        else if (j == ny + 1) v = B(d.To + i) - B((d.To + 1) + i);
-        else if (i == 1) v = B(d.Lo + j) - G(i, j - 1);
-        else if (i == nx) v = G(i - 1, j - 1) - B(d.Ro + j);
-        else v = G(i - 1, j - 1) - G(i, j - 1);
-    }
-    return v;
+    if (t < d.nu) return curl_u(d, psi, pbc, imod(t, d.nx + 1) + 1, idiv(t, d.nx + 1) + 1);
+    const long u = t - d.nu;
+    return curl_v(d, psi, pbc, imod(u, d.nx) + 1, idiv(u, d.nx) + 1);
 }
 #endif
 
@@ -203,10 +208,13 @@ void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out,
 // nonlinear term for levels lev0..lev0+nbatch-1 (level stride sq / sg); q0 is the constant
 // (ux·hc, uy·hc) per level (or (ux, uy) as given when q0_given); gbc per level with stride
 // sbc, multiplied by gbc_scale.
+// With psi set the flux of level b is curl!(ψ_b, ψbc_b) (ψ, ψbc level strides spsi, spbc),
+// formed while the tile is staged, and q is not read.
 void launch_nonlinear(const Dom &d, const double *q, long sq, const double *g, long sg,
                       const double *gbc, long sbc, double gbc_scale, double *out, long sout,
                       double ux, double uy, double h, int lev0, int nbatch, hipStream_t s,
-                      int q0_given = 0, const double *uv = nullptr);
+                      int q0_given = 0, const double *uv = nullptr, const double *psi = nullptr,
+                      long spsi = 0, const double *pbc = nullptr, long spbc = 0);
 // accbc[ℓ] = Σ_{l=ℓ}^{L-2} fac_l·bcraw[l]  for ℓ = 0..L-2, accbc[L-1] = 0
 void launch_bc_prefix(const Dom &d, const double *bcraw, double *accbc, const double *fac, int L,
                       hipStream_t s);
@@ -414,9 +422,11 @@ constexpr int NL_TX = 64, NL_TY = 4;
 __host__ __device__ inline long nl_tiles(const Dom &d) {
     return (long)((d.m + NL_TX - 1) / NL_TX) * ((d.p + NL_TY - 1) / NL_TY);
 }
+// psi set: the flux is curl!(ψ, ψbc) formed while staging (q unread), else q is read
 struct NonlinA {
     Dom d; const double *q; long sq; const double *g; long sg; const double *gbc; long sbc;
     double gbc_scale; double *out; long sout; double h; int lev0; const double *uv; int nbatch;
+    const double *psi; long spsi; const double *pbc; long spbc;
 };
 struct BcPrefA { int nbc; const double *bcraw; double *acc; const double *fac; int L; };
 struct RhsA {

@@ -183,3 +183,40 @@ def test_batch_rejects_moving_bodies_and_outside_steps():
     with pytest.raises(m.IbhError):
         batch.advance()
     batch.close()
+
+
+def test_batch_reads_a_set_flux():
+    # a case whose q was set through the ABI before it joins a batch: the batch's first
+    # nonlinear term reads that q (its argument entry is patched for the step), later steps
+    # form q from ψ again, as the case stepped alone does
+    import ibhip as m
+
+    probs = mixed_problems("fp64")[:3]
+
+    def prepare(p, scale):
+        st = m.initstate(p)
+        sv = m.initsolver(p, st)
+        for k in range(2):
+            m.advance_(st, sv, k * p.scheme.dt)
+        st.case.set_field(m._lib.F_Q, scale * st.case.q().T.reshape(-1))
+        return st, sv
+
+    alone = []
+    for i, p in enumerate(probs):
+        st, sv = prepare(p, 1.001 if i == 1 else 1.0)
+        for k in range(2, 5):
+            m.advance_(st, sv, k * p.scheme.dt)
+        alone.append(fields(st.case))
+        st.case.close()
+    states = [prepare(p, 1.001 if i == 1 else 1.0)[0] for i, p in enumerate(probs)]
+    batch = m.Batch(states)
+    for _ in range(3):
+        batch.advance()
+    batch.sync()
+    for i, st in enumerate(states):
+        for name, a, b in zip(NAMES, fields(st.case), alone[i]):
+            err = np.abs(a - b).max() / np.abs(fields(st.case)[6]).max() if name == "CL" else rel(a, b)
+            assert err < 1e-12, f"case {i} {name}: {err:.3e}"
+    batch.close()
+    for st in states:
+        st.case.close()

@@ -540,3 +540,36 @@ def test_graph_replay_matches_eager(ibhip, monkeypatch, moving):
     for (g1, q1, f1, c1), (g2, q2, f2, c2) in zip(*runs):
         assert rel(g1, g2) < 1e-12 and rel(q1, q2) < 1e-12
         assert rel(f1, f2) < 1e-10 and rel(c1, c2) < 1e-10
+
+
+def test_set_flux_is_read_by_the_next_step(ibhip):
+    # the step keeps q as curl!(ψ, ψbc) and forms it where it is read; a q (or ψ) set through
+    # the ABI is what the next nonlinear term reads, as the reference's step reads its state's q
+    import ibhip as m
+
+    prob = cylinder_case(n=64, nlev=3)
+    dt = prob.scheme.dt
+
+    def run(edit, nsteps=5):
+        st = m.initstate(prob)
+        sv = m.initsolver(prob, st)
+        for k in range(3):
+            m.advance_(st, sv, k * dt)
+        edit(st.case)
+        for k in range(3, nsteps):
+            m.advance_(st, sv, k * dt)
+        # nonlin(1): after cycle! the second slot holds this step's nonlinear term
+        out = [st.case.gamma(), st.case.q(), st.case.nonlin(1), st.case.CD()]
+        st.case.close()
+        return out
+
+    base = run(lambda c: None)
+    same_q = run(lambda c: c.set_field(m._lib.F_Q, c.q().T.reshape(-1)))
+    same_psi = run(lambda c: c.set_field(m._lib.F_PSI, c.psi().T.reshape(-1)))
+    for a, b, c in zip(base, same_q, same_psi):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    pert = run(lambda c: c.set_field(m._lib.F_Q, 1.001 * c.q().T.reshape(-1)), nsteps=4)
+    ref = run(lambda c: None, nsteps=4)
+    d = rel(pert[2], ref[2])  # the nonlinear term of step 4 saw the scaled flux
+    print(f"nonlinear term with q scaled by 1.001: rel. change {d:.2e}")
+    assert 1e-5 < d < 1e-2
