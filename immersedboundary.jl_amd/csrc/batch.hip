@@ -28,7 +28,7 @@ using namespace ibh;
 namespace {
 
 enum Op {
-    OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_NONLIN, OP_BCPREF, OP_FILL,
+    OP_GEMM, OP_CURL, OP_ROT, OP_COARS, OP_GETBC, OP_COPYBC, OP_COPYGETBC, OP_NONLIN, OP_BCPREF, OP_FILL,
     OP_E, OP_ETZ, OP_ETS, OP_ET, OP_ETF, OP_GEMV, OP_STRESS, OP_RHS, OP_RESID,
     OP_ETZS, OP_ROTI, OP_EDGE, OP_SPEC, OP_LNP, OP_LNC, OP_LNO, OP_AXPY
 };
@@ -179,7 +179,7 @@ void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem
         if (it.mode == SV_SPEC) {
             if (it.bc) {
                 ed.push_back({d, c->Sm, c->Sp, it.bc, c->edgeS});
-                edb.push_back(d.m + d.p);
+                edb.push_back((int)nblk(d.nx / 2 + d.ny / 2, GRP_BS / 64));  // a wave per pair
             }
             sp.push_back({d, it.spec, c->sD, 0.25 * d.nx * d.ny, c->Sm, c->Sp,
                           it.bc ? c->edgeS : nullptr, c->T2});
@@ -293,7 +293,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
     }
     for (int lev = b->Lmax - 1; lev >= 0; --lev) {
         std::vector<GetBcA> gb;
-        std::vector<CopyBcA> cp;
+        std::vector<CopyGetBcA> cg;
         std::vector<int> b1, b2, cb;
         std::vector<SolveItem> sv;
         std::vector<CurlA> cu;
@@ -308,11 +308,12 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             if (lev == c->L - 1) {  // the coarsest level: ψ_L = Δinv Γ_L
                 sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, nullptr, c->specT + lev * d.ng});
             } else {
-                gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
-                b1.push_back(nblk(d.nbc, GRP_BS));
-                if (md != SV_SPEC) {
-                    cp.push_back({d, rhs, 0, c->gtmp, 0, bcl, 0, 1.0, 1});
-                    b2.push_back((int)grp_blocks(d.ng));
+                if (md != SV_SPEC) {  // get_bc! fused with the copy into Γtmp + edges
+                    cg.push_back({d, rhs, c->gtmp, Psi[i] + (lev + 1) * d.ng, bcl});
+                    b2.push_back(nblk(d.ng + d.nbc, GRP_BS));
+                } else {
+                    gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
+                    b1.push_back(nblk(d.nbc, GRP_BS));
                 }
                 rhs = c->gtmp;
                 sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
@@ -327,7 +328,7 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             }
         }
         add(v, OP_GETBC, gb, b1);
-        add(v, OP_COPYBC, cp, b2);
+        add(v, OP_COPYGETBC, cg, b2);
         add_solves(b, v, sv, lev >= 1 && final_pass && b->mix ? IBH_FP32 : b->prec);
         add_refine(b, v, rf);
         add(v, OP_CURL, cu, cb);
@@ -560,6 +561,7 @@ void launch_stage(ibh_batch *b, const Stage &S, int variant, unsigned long long 
     case OP_COARS: launch_grp_coarsify(g, s); break;
     case OP_GETBC: launch_grp_get_bc(g, s); break;
     case OP_COPYBC: launch_grp_copy_apply_bc(g, s); break;
+    case OP_COPYGETBC: launch_grp_copy_getbc(g, s); break;
     case OP_NONLIN: launch_grp_nonlinear(g, s); break;
     case OP_BCPREF: launch_grp_bc_prefix(g, s); break;
     case OP_FILL: launch_grp_fill(g, s); break;

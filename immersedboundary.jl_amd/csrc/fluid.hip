@@ -232,6 +232,42 @@ void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out,
                        sin, out, sout, rbc, sbc, fac, skip);
 }
 
+// get_bc! and the copy with apply_bc! in one pass: rbc = get_bc!(r) (the coarser level's ψ) and
+// out = in + edges(rbc); the edge elements evaluate their boundary value from r themselves, the
+// same value the rbc entry receives.  Threads [0, nΓ) copy, [nΓ, nΓ + nbc) write rbc.
+__device__ inline void copy_getbc_at(const Dom &d, const double *__restrict__ in,
+                                     double *__restrict__ out, const double *__restrict__ r,
+                                     double *__restrict__ rbc, long t) {
+    if (t >= d.ng) {
+        if (t < d.ng + d.nbc) rbc[t - d.ng] = get_bc_at(d, r, (int)(t - d.ng));
+        return;
+    }
+    const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
+    double v = in[t];
+    // as copy_apply_bc_at (fac = 1): bottom, top, left, right
+    if (j == 1) v += get_bc_at(d, r, d.Bo + i);
+    if (j == d.ny - 1) v += get_bc_at(d, r, d.To + i);
+    if (i == 1) v += get_bc_at(d, r, d.Lo + j);
+    if (i == d.nx - 1) v += get_bc_at(d, r, d.Ro + j);
+    out[t] = v;
+}
+
+__global__ void k_copy_getbc(Dom d, const double *__restrict__ in, long sin,
+                             double *__restrict__ out, long sout, const double *__restrict__ r,
+                             long sr, double *__restrict__ rbc, long sbc, const int *skip) {
+    SKIP_GUARD
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    copy_getbc_at(d, in + b * sin, out + b * sout, r + b * sr, rbc + b * sbc, t);
+}
+
+void launch_copy_getbc(const Dom &d, const double *in, long sin, double *out, long sout,
+                       const double *r, long sr, double *rbc, long sbc, int nbatch,
+                       const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_getbc, dim3(nblk(d.ng + d.nbc, 256), nbatch), dim3(256), 0, s, d, in,
+                       sin, out, sout, r, sr, rbc, sbc, skip);
+}
+
 void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long sbc, double fac,
                      int nbatch, const int *skip, hipStream_t s) {
     launch_copy_apply_bc(d, r, sr, r, sr, rbc, sbc, fac, nbatch, skip, s);
@@ -514,41 +550,43 @@ __global__ void k_fill(double *p, long n, double v) {
 // (S symmetric).  k_edge_gemv forms W = [S_m cB | S_m cT | S_p cL | S_p cR] (2m + 2p per
 // column), one workgroup per output entry; k_edge_add adds the rank-4 term, times the
 // eigenvalue scale D, to the scaled spectral array F = (S_m X S_p) ⊙ D.
-// one 256-thread workgroup: output entry x of one batch entry (boundary values r, W row Wb)
-__device__ inline void edge_gemv_row(const Dom &d, const double *__restrict__ Sm,
-                                     const double *__restrict__ Sp, const double *__restrict__ r,
-                                     double *__restrict__ Wb, int x) {
-    __shared__ double sh[2][4];
-    const bool xs = x < d.m;                    // S_m rows first, then S_p rows
-    const int row = xs ? x : x - d.m, n = xs ? d.m : d.p;
-    const double *S = (xs ? Sm : Sp) + (long)row * n;
+// Rows i and n−1−i of S (n = m or p, 0-based; S[i, k] = sin(π(i+1)(k+1)/(n+1))) differ only by
+// the sign of the odd columns: S[n−1−i, k] = (−1)^k S[i, k].  So one read of row i gives both outputs, E + O and E − O, with
+// E and O the sums over even and odd k.  One wave per such pair (the middle row, i = (n−1)/2, is
+// its own mirror: its odd columns are zero); the outputs of pair x of one batch entry (boundary
+// values r, W row Wb), x over the pairs of S_m and then of S_p.
+__device__ inline int edge_pairs(const Dom &d) { return d.nx / 2 + d.ny / 2; }
+__device__ inline void edge_gemv_pair(const Dom &d, const double *__restrict__ Sm,
+                                      const double *__restrict__ Sp, const double *__restrict__ r,
+                                      double *__restrict__ Wb, int x) {
+    const int hx = d.nx / 2;
+    if (x >= edge_pairs(d)) return;
+    const bool xs = x < hx;                     // S_m pairs first, then S_p pairs
+    const int i = xs ? x : x - hx, n = xs ? d.m : d.p;
+    const double *S = (xs ? Sm : Sp) + (long)i * n;
     const int o1 = xs ? d.Bo : d.Lo, o2 = xs ? d.To : d.Ro;  // 0-based: entry k at o + 1 + k
+    const int lane = threadIdx.x & 63;
+    // lane l sums the columns k ≡ l (mod 64): even lanes the even k, odd lanes the odd ones
     double s1 = 0.0, s2 = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) {
+    for (int k = lane; k < n; k += 64) {
         const double sv = S[k];
         s1 += sv * r[o1 + 1 + k];
         s2 += sv * r[o2 + 1 + k];
     }
-    for (int o = 32; o > 0; o >>= 1) {
+    // fixed-order sums over the even lanes (into lane 0) and the odd lanes (into lane 1)
+    for (int o = 32; o > 1; o >>= 1) {
         s1 += __shfl_down(s1, o, 64);
         s2 += __shfl_down(s2, o, 64);
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        sh[0][w] = s1;
-        sh[1][w] = s2;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double t1 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-        const double t2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
-        if (xs) {
-            Wb[row] = t1;
-            Wb[d.m + row] = t2;
-        } else {
-            Wb[2 * d.m + row] = t1;
-            Wb[2 * d.m + d.p + row] = t2;
-        }
+    const double o1s = __shfl_down(s1, 1, 64), o2s = __shfl_down(s2, 1, 64);
+    if (lane != 0) return;
+    const int mi = n - 1 - i;  // the mirror row (sin(π(N − i − 1)(k + 1)/N), N = n + 1)
+    double *W1 = Wb + (xs ? 0 : 2 * d.m), *W2 = Wb + (xs ? d.m : 2 * d.m + d.p);
+    W1[i] = s1 + o1s;
+    W2[i] = s2 + o2s;
+    if (mi != i) {
+        W1[mi] = s1 - o1s;
+        W2[mi] = s2 - o2s;
     }
 }
 
@@ -558,7 +596,8 @@ __global__ __launch_bounds__(256) void k_edge_gemv(Dom d, const double *__restri
                                                    double *__restrict__ W, long sW,
                                                    const int *skip) {
     SKIP_GUARD
-    edge_gemv_row(d, Sm, Sp, bc + blockIdx.y * sbc, W + blockIdx.y * sW, blockIdx.x);
+    edge_gemv_pair(d, Sm, Sp, bc + blockIdx.y * sbc, W + blockIdx.y * sW,
+                   blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
 }
 
 // the rank-4 boundary term at spectral entry (a, c), from one batch entry's W
@@ -582,8 +621,8 @@ __global__ void k_edge_add(Dom d, const double *__restrict__ Sm, const double *_
 
 void launch_edge_gemv(const Dom &d, const double *Sm, const double *Sp, const double *bc,
                       long sbc, double *W, int nbatch, const int *skip, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_gemv, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp, bc, sbc,
-                       W, 2L * (d.m + d.p), skip);
+    hipLaunchKernelGGL(k_edge_gemv, dim3(nblk(d.nx / 2 + d.ny / 2, 4), nbatch), dim3(256), 0, s,
+                       d, Sm, Sp, bc, sbc, W, 2L * (d.m + d.p), skip);
 }
 
 void launch_edge_spectral(const Dom &d, const double *Sm, const double *Sp, const double *bc,
@@ -955,6 +994,13 @@ __global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, 
     GRP_ELEMENTS(d.ng, A.out + b * A.sout, copy_apply_bc_at(d, in, rbc, A.fac, t))
 }
 
+__global__ __launch_bounds__(GRP_BS) void kg_copy_getbc(const CopyGetBcA *args, const int *off,
+                                                        int n) {
+    GRP_PROLOGUE(CopyGetBcA, A.d.ng + A.d.nbc)
+    (void)b;  // one batch entry per problem
+    copy_getbc_at(d, A.in, A.out, A.r, A.rbc, t);
+}
+
 // nl_tiles(d) workgroups per batch entry (level)
 __global__ __launch_bounds__(GRP_BS) void kg_nonlinear(const NonlinA *args, const int *off, int n) {
     static_assert(GRP_BS == NL_TX * NL_TY, "one thread per tile output");
@@ -1013,6 +1059,7 @@ GRP_LAUNCH(launch_grp_rot_image, kg_rot_image, RotIA)
 GRP_LAUNCH(launch_grp_coarsify, kg_coarsify, CoarsA)
 GRP_LAUNCH(launch_grp_get_bc, kg_get_bc, GetBcA)
 GRP_LAUNCH(launch_grp_copy_apply_bc, kg_copy_apply_bc, CopyBcA)
+GRP_LAUNCH(launch_grp_copy_getbc, kg_copy_getbc, CopyGetBcA)
 GRP_LAUNCH(launch_grp_nonlinear, kg_nonlinear, NonlinA)
 GRP_LAUNCH(launch_grp_bc_prefix, kg_bc_prefix, BcPrefA)
 GRP_LAUNCH(launch_grp_rhs, kg_rhs, RhsA)
@@ -1025,7 +1072,7 @@ static_assert(GRP_BS == 256, "the edge and lines passes run 256-thread workgroup
 __global__ __launch_bounds__(GRP_BS) void kg_edge_gemv(const EdgeA *args, const int *off, int n) {
     const int p = grp_find(off, n, XB);
     const EdgeA A = args[p];
-    edge_gemv_row(A.d, A.Sm, A.Sp, A.bc, A.W, XB - off[p]);
+    edge_gemv_pair(A.d, A.Sm, A.Sp, A.bc, A.W, (XB - off[p]) * (GRP_BS / 64) + (threadIdx.x >> 6));
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, const int *off,

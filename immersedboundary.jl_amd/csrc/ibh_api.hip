@@ -596,7 +596,8 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     const bool fuse0 = spec0 && c->use_eo && !(refine && 0 < rlev);
     // B-products work on their column batch's buffers, the step on its own
     double *lineW = sparse ? c->lineW : c->lineS, *edgeW = sparse ? c->edgeW : c->edgeS;
-    auto solve = [&](int lev, const double *bcl) {
+    // pc: the next coarser level's ψ when the copy path forms bcl = get_bc!(pc) itself
+    auto solve = [&](int lev, const double *bcl, const double *pc) {
         double *Gl = G + lev * ng, *Pl = Psi + lev * ng;
         const int prec = lev >= 1 ? prec_coarse : prec0;
         const bool ln = lines && lev >= 1 && !(refine && lev < rlev);
@@ -625,7 +626,11 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
         const double *in = Gl;
         long sin = sG;
         if (bcl) {
-            launch_copy_apply_bc(d, Gl, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
+            if (pc)
+                launch_copy_getbc(d, Gl, sG, c->gtmp, ng, pc, sG, const_cast<double *>(bcl), sB,
+                                  nc, skip, c->s);
+            else
+                launch_copy_apply_bc(d, Gl, sG, c->gtmp, ng, bcl, sB, 1.0, nc, skip, c->s);
             in = c->gtmp;
             sin = ng;
         }
@@ -646,7 +651,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     // a B-product over every level finds its coarsest boundary values zero already: their
     // buffer is zeroed when allocated and get_bc! writes only the finer levels' values
     if (!(sparse && L == Ls)) launch_fill_strided(bcL, d.nbc, sB, nc, 0.0, c->s);
-    solve(L - 1, nullptr);
+    solve(L - 1, nullptr, nullptr);
     if (refine && L - 1 < rlev)
         refine_sweeps(c, Psi + (L - 1) * ng, G + (L - 1) * ng, -1.0, c->sD, 0, 1, refine,
                       L > 1 ? prec_coarse : prec0);
@@ -662,8 +667,10 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
     }
     for (int lev = L - 2; lev >= 0; --lev) {
         double *bcl = bc + (long)lev * d.nbc;
-        launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
-        solve(lev, bcl);
+        // the copy path forms the boundary values in its copy of Γ (launch_copy_getbc)
+        const bool copy_path = !(bsp && lev >= 1) && !(lev == 0 && fuse0);
+        if (!copy_path) launch_get_bc(d, Psi + (lev + 1) * ng, sG, bcl, sB, nc, skip, c->s);
+        solve(lev, bcl, copy_path ? Psi + (lev + 1) * ng : nullptr);
         if (refine && lev < rlev)
             refine_sweeps(c, Psi + lev * ng, c->gtmp, -1.0, c->sD, 0, 1, refine,
                           lev >= 1 ? prec_coarse : prec0);

@@ -202,6 +202,11 @@ void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc
 void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long sbc, double fac,
                      int nbatch, const int *skip, hipStream_t s);
 // out[b] = in[b] + edges(rbc[b])·fac
+// rbc = get_bc!(r) and out = in + edges(rbc) in one pass (the Poisson right-hand side of a level
+// from the next coarser level's ψ r)
+void launch_copy_getbc(const Dom &d, const double *in, long sin, double *out, long sout,
+                       const double *r, long sr, double *rbc, long sbc, int nbatch,
+                       const int *skip, hipStream_t s);
 void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out, long sout,
                           const double *rbc, long sbc, double fac, int nbatch, const int *skip,
                           hipStream_t s);
@@ -417,6 +422,8 @@ struct RotIA { Dom d; const double *img; const double *scale; double *g; int nba
 struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
 struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
 struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
+// get_bc! of r into rbc and out = in + edges(rbc) (one batch entry); blocks ceil((nΓ + nbc)/256)
+struct CopyGetBcA { Dom d; const double *in; double *out; const double *r; double *rbc; };
 // the nonlinear term's output tiles per level (NL_TX × NL_TY outputs each)
 constexpr int NL_TX = 64, NL_TY = 4;
 __host__ __device__ inline long nl_tiles(const Dom &d) {
@@ -477,6 +484,7 @@ void launch_grp_rot_image(const Grp &g, hipStream_t s);    // RotIA; blocks: cei
 void launch_grp_coarsify(const Grp &g, hipStream_t s);
 void launch_grp_get_bc(const Grp &g, hipStream_t s);
 void launch_grp_copy_apply_bc(const Grp &g, hipStream_t s);
+void launch_grp_copy_getbc(const Grp &g, hipStream_t s);
 void launch_grp_nonlinear(const Grp &g, hipStream_t s);
 void launch_grp_bc_prefix(const Grp &g, hipStream_t s);   // blocks: ceil(nbc/256)
 void launch_grp_rhs(const Grp &g, hipStream_t s);         // blocks: ceil(nΓ/256)·levels
