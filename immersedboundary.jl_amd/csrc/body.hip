@@ -175,8 +175,17 @@ __global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__r
                                               const double *__restrict__ U,
                                               const double *__restrict__ w, int lrk, int t0,
                                               int tn, const double *__restrict__ psi, long spsi,
-                                              const double *__restrict__ pbc, long spbc) {
+                                              const double *__restrict__ pbc, long spbc,
+                                              double *__restrict__ zimg, long szimg, GBox zbox) {
     if (skip && *skip) return;
+    const int ne = (tn + W - 1) / W;  // E's workgroups; those after them clear zimg on zbox
+    if ((int)blockIdx.x >= ne) {
+        const long n = gbox_faces(d, zbox);
+        for (long t = (blockIdx.x - ne) * (long)blockDim.x + threadIdx.x; t < n;
+             t += (gridDim.x - ne) * (long)blockDim.x)
+            zimg[blockIdx.y * szimg + gbox_face(d, zbox, t)] = 0.0;
+        return;
+    }
     if (uv) {  // q0 = freestream·h from the device parameter block
         u0 = uv[0] * uvscale;
         v0 = uv[1] * uvscale;
@@ -200,13 +209,17 @@ __global__ __launch_bounds__(64 * W) void k_E(Dom d, RegDev r, const double *__r
 void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0, double v0,
               double *fb, long sfb, int nbatch, const int *skip, hipStream_t s, const double *uv,
               double uvscale, const double *U, const double *w, int lrk, int t0, int tn,
-              const double *psi, long spsi, const double *pbc, long spbc) {
+              const double *psi, long spsi, const double *pbc, long spbc, double *zimg,
+              long szimg, GBox zbox) {
     if (r.nb == 0) return;
     if (tn < 0) tn = 2 * r.nb - t0;
     if (tn <= 0) return;
+    // zimg: extra workgroups clear its faces on zbox (512 of them at most, striding)
+    const long zf = zimg ? gbox_faces(d, zbox) : 0;
 #define LAUNCH_E(W)                                                                           \
-    hipLaunchKernelGGL(k_E<W>, dim3(nblk(tn, W), nbatch), dim3(64 * W), 0, s, d, r, q, sq, u0, \
-                       v0, fb, sfb, skip, uv, uvscale, U, w, lrk, t0, tn, psi, spsi, pbc, spbc)
+    hipLaunchKernelGGL(k_E<W>, dim3(nblk(tn, W) + (unsigned)std::min<long>(512, nblk(zf, 64 * W)), \
+                       nbatch), dim3(64 * W), 0, s, d, r, q, sq, u0, v0, fb, sfb, skip, uv,      \
+                       uvscale, U, w, lrk, t0, tn, psi, spsi, pbc, spbc, zimg, szimg, zbox)
     switch (E_WAVES[(r.esched >= 0 && r.esched < E_NSCHED) ? r.esched : 0]) {
     case 1: LAUNCH_E(1); break;
     case 2: LAUNCH_E(2); break;
@@ -345,15 +358,8 @@ __global__ __launch_bounds__(256) void k_et_zero_scale(Dom d, GBox box, double *
         return;
     }
     if (blockIdx.x > nz) {
-        __shared__ double sh[4];
         const int j = blockIdx.x - (int)nz - 1;
-        const double *Vj = V + (long)n2 * j, *f = fb + b * sfb;
-        double s = 0.0;
-        for (int i = threadIdx.x; i < n2; i += 256) s += Vj[i] * f[i];
-        s = wave_sum(s);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) w[(long)b * lrk + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        lr_dot256(V + (long)n2 * j, fb + b * sfb, n2, w + (long)b * lrk + j);
         return;
     }
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -365,6 +371,13 @@ void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, 
                      const double *V, double *w, int lrk, GBox box) {
     hipLaunchKernelGGL(k_et_zero_scale, dim3(nblk(gbox_faces(d, box), 256) + 1 + lrk, nbatch),
                        dim3(256), 0, s, d, box, img, sq, fb, sfb, 2 * r.nb, scale, skip, V, w, lrk);
+    if (r.nb == 0) return;
+    launch_ET_scatter(d, r, fb, sfb, img, sq, nbatch, skip, scale, s);
+}
+
+void launch_ET_scatter_only(const Dom &d, const RegDev &r, const double *fb, long sfb,
+                            double *img, long sq, int nbatch, const int *skip,
+                            const double *scale, hipStream_t s) {
     if (r.nb == 0) return;
     launch_ET_scatter(d, r, fb, sfb, img, sq, nbatch, skip, scale, s);
 }
@@ -512,9 +525,25 @@ __global__ void k_pcg_dir(CGDev cg) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += cg.part[i];
     s = block_sum<1024>(s, sh);
     const double beta = s / cg.sc[3];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) cg.u[i] = cg.z[i] + beta * cg.u[i];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double u = cg.z[i] + beta * cg.u[i];
+        cg.u[i] = u;
+        m = fmax(m, fabs(u));
+    }
     __syncthreads();
     if (threadIdx.x == 0) cg.sc[3] = s;
+    if (cg.etscale) {  // the Eᵀ scale of u for the B-product that follows (as et_scale_body)
+        __shared__ double mx[16];
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+        if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            m = 0.0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, mx[k]);
+            *cg.etscale = (m > 0.0 && isfinite(m)) ? ldexp(1.0, 50 - ilogb(m)) : 1.0;
+        }
+    }
 }
 
 void launch_pcg_dir(const CGDev &cg, hipStream_t s) {

@@ -104,19 +104,30 @@ __device__ inline double rot_image_at(const Dom &d, const long long *__restrict_
 
 __global__ void k_rot_image(Dom d, GBox box, const long long *__restrict__ img, long sq,
                             const double *__restrict__ scale, double *__restrict__ g, long sg,
-                            const int *skip) {
+                            const int *skip, const double *__restrict__ V,
+                            const double *__restrict__ fb, long sfb, int n2,
+                            double *__restrict__ w, int lrk) {
     SKIP_GUARD
+    const long nv = gbox_vertices(d, box);
+    const int nrb = (int)nblk(nv, 256);
+    if ((int)blockIdx.x >= nrb) {  // the low-rank dots (lrk workgroups)
+        const int j = blockIdx.x - nrb, b = blockIdx.y;
+        lr_dot256(V + (long)n2 * j, fb + b * sfb, n2, w + (long)b * lrk + j);
+        return;
+    }
     const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (k >= gbox_vertices(d, box)) return;
+    if (k >= nv) return;
     const long t = gbox_vertex(d, box, k);
     const int b = blockIdx.y;
     g[b * sg + t] = rot_image_at(d, img + b * sq, 1.0 / scale[b], t);
 }
 
 void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
-                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box) {
-    hipLaunchKernelGGL(k_rot_image, dim3(nblk(gbox_vertices(d, box), 256), nbatch), dim3(256), 0,
-                       s, d, box, reinterpret_cast<const long long *>(img), sq, scale, g, sg, skip);
+                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box,
+                      const double *V, const double *fb, long sfb, int n2, double *w, int lrk) {
+    hipLaunchKernelGGL(k_rot_image, dim3(nblk(gbox_vertices(d, box), 256) + lrk, nbatch),
+                       dim3(256), 0, s, d, box, reinterpret_cast<const long long *>(img), sq, scale,
+                       g, sg, skip, V, fb, sfb, n2, w, lrk);
 }
 
 // coarsify!(Γc, Γ), fluid-ops.jl:184-202: fine (i, j) = (2a + 2, 2b + 2) → coarse

@@ -719,15 +719,24 @@ void clear_box_work(ibh_ctx *c) {
     if (c->g1) launch_fill(c->g1, d.ng, 0.0, c->s);
 }
 
+// `chained` (PCG iterations): the direction kernel has set z's Eᵀ scale and the previous
+// product's E cleared the image, so Eᵀ is the scatter alone and rot's extra workgroups form
+// w = EVᵀz.  Every product leaves the image zero on the box (E's extra workgroups).
 void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
-                const int *skip) {
+                const int *skip, bool chained = false) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
     const GBox box = body_gbox(c);
-    launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, c->lrEV,
-                    c->lrW, c->lr_k, box);  // w = EVᵀ z in the same launch
-    launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box);
+    if (chained) {
+        launch_ET_scatter_only(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
+        launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box, c->lrEV, z,
+                         sz, 2 * c->nb, c->lrW, c->lr_k);
+    } else {
+        launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, c->lrEV,
+                        c->lrW, c->lr_k, box);  // w = EVᵀ z in the same launch
+        launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box);
+    }
     LapInvPlan P;
     lapinv_plan(c, c->g1B, ng, c->psiB, L * ng, c->sAD, 0, nc, EPI_STORE, nullptr, skip, &P);
     P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
@@ -740,18 +749,24 @@ void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc
     P.g[3].ohi[0] = c->bxb + 1;
     pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
     for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
-    // E of curl ψ₁ (zero boundary values), the flux formed inside E's wave sums, + EQ·w
+    // E of curl ψ₁ (zero boundary values), the flux formed inside E's wave sums, + EQ·w; its
+    // extra workgroups clear Eᵀ's image on the box for the next product
     launch_E(d, c->reg, nullptr, 0, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, c->lrEQ,
-             c->lrW, c->lr_k, 0, -1, c->psiB, L * ng, c->zerobc, 0);
+             c->lrW, c->lr_k, 0, -1, c->psiB, L * ng, c->zerobc, 0, c->qB, d.nq, box);
+}
+
+// the product of a column batch of nc goes through B_times_lr
+bool lr_path(const ibh_ctx *c, int prec, int nlev, int nc) {
+    return c->lr_valid && !c->lr_bypass && prec == IBH_FP64 && nlev <= 0 && nc <= LR_MAXNC;
 }
 
 void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, const int *skip,
-             int prec = IBH_FP64, int nlev = -1) {
+             int prec = IBH_FP64, int nlev = -1, bool chained = false) {
     const Dom &d = c->d;
     const int L = c->L, Lv = nlev > 0 ? std::min(nlev, L) : L;
     const long ng = d.ng;
-    if (c->lr_valid && !c->lr_bypass && prec == IBH_FP64 && nlev <= 0 && nc <= LR_MAXNC) {
-        B_times_lr(c, z, sz, x, sx, nc, skip);
+    if (lr_path(c, prec, nlev, nc)) {
+        B_times_lr(c, z, sz, x, sx, nc, skip, chained);
         return;
     }
     const GBox box = body_gbox(c);
@@ -1445,12 +1460,17 @@ void seg_solve_begin(ibh_ctx *c) {
 
 void seg_cg_iters(ibh_ctx *c, int k) {
     CGDev &cg = c->cg;
+    // PCG over low-rank products: the direction kernel sets u's Eᵀ scale, and each product
+    // follows one (the initial residual's or the previous iteration's) that cleared the image
+    const bool chain = c->use_pcg && lr_path(c, IBH_FP64, -1, 1);
+    cg.etscale = chain ? c->etscale : nullptr;
     for (int i = 0; i < k; ++i) {
         if (c->use_pcg) launch_pcg_dir(cg, c->s);
         else launch_cg_dir(cg, c->s);
-        B_times(c, cg.u, 0, cg.c, 0, 1, cg.done);
+        B_times(c, cg.u, 0, cg.c, 0, 1, cg.done, IBH_FP64, -1, chain);
         launch_cg_update(cg, c->s);
     }
+    cg.etscale = nullptr;
 }
 
 void seg_post(ibh_ctx *c) {
@@ -1904,6 +1924,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->cg.z, nf));
     CK(dalloc(c, &c->cg.part, std::max(1, pcg_parts(nf))));
     CK(dalloc(c, &c->cg.sc, 4));
+    c->cg.etscale = nullptr;
     c->cg.done = c->status;
     c->cg.it = c->status + 1;
     c->cg.x = c->Fb;

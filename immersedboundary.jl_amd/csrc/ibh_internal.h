@@ -192,8 +192,12 @@ void launch_rot(const Dom &d, const double *q, long sq, double *g, long sg, int 
                 const int *skip, hipStream_t s);
 // rot of the q whose fixed-point image (scale per batch entry) launch_ET_image left in img
 // (with a box: only Γ inside it; the caller keeps Γ zero outside)
+// With lrk > 0 its extra workgroups also form w[b, j] = V[:, j]·fb[b] (V n2 × lrk
+// column-major), as launch_ET_image would.
 void launch_rot_image(const Dom &d, const double *img, long sq, const double *scale, double *g,
-                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box = GBox());
+                      long sg, int nbatch, const int *skip, hipStream_t s, GBox box = GBox(),
+                      const double *V = nullptr, const double *fb = nullptr, long sfb = 0,
+                      int n2 = 0, double *w = nullptr, int lrk = 0);
 void launch_coarsify(const Dom &d, double *gc, const double *g, long sgc, long sg, int nbatch,
                      const int *skip, hipStream_t s);
 void launch_get_bc(const Dom &d, const double *r, long sr, double *rbc, long sbc, int nbatch,
@@ -335,7 +339,7 @@ void launch_E(const Dom &d, const RegDev &r, const double *q, long sq, double u0
               const double *uv = nullptr, double uvscale = 1.0, const double *U = nullptr,
               const double *w = nullptr, int lrk = 0, int t0 = 0, int tn = -1,
               const double *psi = nullptr, long spsi = 0, const double *pbc = nullptr,
-              long spbc = 0);
+              long spbc = 0, double *zimg = nullptr, long szimg = 0, GBox zbox = GBox());
 // q = Eᵀ fb (zero, scale, fixed-point scatter, conversion); scale: nbatch doubles of work
 void launch_ET(const Dom &d, const RegDev &r, const double *fb, long sfb, double *q, long sq,
                int nbatch, const int *skip, double *scale, hipStream_t s);
@@ -349,6 +353,11 @@ void launch_ET_image(const Dom &d, const RegDev &r, const double *fb, long sfb, 
                      long sq, int nbatch, const int *skip, double *scale, hipStream_t s,
                      const double *V = nullptr, double *w = nullptr, int lrk = 0,
                      GBox box = GBox());
+// the scatter of launch_ET_image alone: the image must be zero on the box and the scales set
+// (the PCG direction kernel sets them; E of the previous product cleared the image)
+void launch_ET_scatter_only(const Dom &d, const RegDev &r, const double *fb, long sfb,
+                            double *img, long sq, int nbatch, const int *skip,
+                            const double *scale, hipStream_t s);
 // y = A x with A stored transposed (AT column-major: A's rows contiguous), one workgroup per row
 void launch_gemv_t(int n, const double *AT, const double *x, double *y, const int *skip,
                    hipStream_t s);
@@ -367,6 +376,7 @@ struct CGDev {
     const double *P;            // dense preconditioner n×n (PCG), stored transposed, or null
     double *sc;                 // [residual, prev_residual, tol, ρ]
     double *part;               // z_i·r_i of the PCG step (pcg_parts(n))
+    double *etscale;            // PCG: the Eᵀ fixed-point scale of u is set here (or null)
     int *it, *done;
     int n, maxiter;
     double reltol;
@@ -475,6 +485,19 @@ __host__ __device__ inline long grp_blocks(long elems) {
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
+}
+
+// one 256-thread workgroup: *out = V_j·f over n2 entries in a fixed order (the low-rank dots
+// w = (E·V)ᵀ z of a B-product; the Eᵀ preparation and rot's extra workgroups both use it)
+__device__ inline void lr_dot256(const double *__restrict__ Vj, const double *__restrict__ f,
+                                 int n2, double *out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n2; i += 256) s += Vj[i] * f[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 static constexpr int GRP_NS = 13;  // delta support per dimension
 
