@@ -722,37 +722,44 @@ void clear_box_work(ibh_ctx *c) {
 // `chained` (PCG iterations): the direction kernel has set z's Eᵀ scale and the previous
 // product's E cleared the image, so Eᵀ is the scatter alone and rot's extra workgroups form
 // w = EVᵀz.  Every product leaves the image zero on the box (E's extra workgroups).
+// Without `lowrank` this is the one-level product B₁ (zero boundary values on level 1: the
+// preconditioner's columns), in the transform precision `prec`.
 void B_times_lr(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc,
-                const int *skip, bool chained = false) {
+                const int *skip, bool chained = false, int prec = IBH_FP64, bool lowrank = true) {
     const Dom &d = c->d;
     const int L = c->L;
     const long ng = d.ng;
     const GBox box = body_gbox(c);
+    const int lrk = lowrank ? c->lr_k : 0;
+    const double *EV = lowrank ? c->lrEV : nullptr, *EQ = lowrank ? c->lrEQ : nullptr;
+    double *W = lowrank ? c->lrW : nullptr;
     if (chained) {
         launch_ET_scatter_only(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s);
-        launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box, c->lrEV, z,
-                         sz, 2 * c->nb, c->lrW, c->lr_k);
+        launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box, EV, z, sz,
+                         2 * c->nb, W, lrk);
     } else {
-        launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, c->lrEV,
-                        c->lrW, c->lr_k, box);  // w = EVᵀ z in the same launch
+        launch_ET_image(d, c->reg, z, sz, c->qB, d.nq, nc, skip, c->etscale, c->s, EV, W, lrk,
+                        box);  // w = EVᵀ z in the same launch
         launch_rot_image(d, c->qB, d.nq, c->etscale, c->g1B, ng, nc, skip, c->s, box);
     }
     LapInvPlan P;
     lapinv_plan(c, c->g1B, ng, c->psiB, L * ng, c->sAD, 0, nc, EPI_STORE, nullptr, skip, &P);
-    P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
-    P.g[0].khi = c->bxb / 2 + 1;
-    P.g[1].klo = c->bya / 2;
-    P.g[1].khi = c->byb / 2 + 1;
-    rows_for_pairs(P.g[0], P.g[1]);
-    pair_range(c->bxa, c->bxb, d.nx, &P.g[2].olo[0], &P.g[2].ohi[0]);  // ψ₁ on the box only
-    P.g[3].olo[0] = c->bxa;
-    P.g[3].ohi[0] = c->bxb + 1;
-    pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
-    for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
+    if (c->box_on) {
+        P.g[0].klo = c->bxa / 2;  // C'Eᵀz vanishes outside the body box
+        P.g[0].khi = c->bxb / 2 + 1;
+        P.g[1].klo = c->bya / 2;
+        P.g[1].khi = c->byb / 2 + 1;
+        rows_for_pairs(P.g[0], P.g[1]);
+        pair_range(c->bxa, c->bxb, d.nx, &P.g[2].olo[0], &P.g[2].ohi[0]);  // ψ₁ on the box only
+        P.g[3].olo[0] = c->bxa;
+        P.g[3].ohi[0] = c->bxb + 1;
+        pair_range(c->bya, c->byb, d.ny, &P.g[3].olo[1], &P.g[3].ohi[1]);
+    }
+    for (int k = 0; k < 4; ++k) gemm(c, P.g[k], P.epi[k], prec, P.kind[k] == 1, P.kind[k]);
     // E of curl ψ₁ (zero boundary values), the flux formed inside E's wave sums, + EQ·w; its
     // extra workgroups clear Eᵀ's image on the box for the next product
-    launch_E(d, c->reg, nullptr, 0, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, c->lrEQ,
-             c->lrW, c->lr_k, 0, -1, c->psiB, L * ng, c->zerobc, 0, c->qB, d.nq, box);
+    launch_E(d, c->reg, nullptr, 0, 0.0, 0.0, x, sx, nc, skip, c->s, nullptr, 1.0, EQ, W, lrk, 0,
+             -1, c->psiB, L * ng, c->zerobc, 0, c->qB, d.nq, box);
 }
 
 // the product of a column batch of nc goes through B_times_lr
@@ -767,6 +774,12 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
     const long ng = d.ng;
     if (lr_path(c, prec, nlev, nc)) {
         B_times_lr(c, z, sz, x, sx, nc, skip, chained);
+        return;
+    }
+    // one level (the preconditioner's columns): Δinv·Ainv₁ is one diagonal in the sine basis
+    // (S·S = (n/2)·I per dimension), so ψ₁ is one forward pair, the scale and one inverse pair
+    if (nlev == 1 && c->use_eo && c->sAD && c->zerobc) {
+        B_times_lr(c, z, sz, x, sx, nc, skip, false, prec, false);
         return;
     }
     const GBox box = body_gbox(c);
@@ -866,6 +879,8 @@ __global__ void k_unit_cols(double *Z, int nf, int j0, int nc) {
 }
 
 // Bm[:, j0:j0+cnt] = B·e_j (unit-vector products in column batches, cnab/ops.jl:88-94)
+int ensure_level1_tables(ibh_ctx *c);
+
 int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64, int nlev = -1) {
     const int nf = 2 * c->nb;
     // column batch sized to keep the B-product work under ~12 GB
@@ -878,6 +893,7 @@ int B_columns(ibh_ctx *c, double *Bm, int j0, int cnt, int prec = IBH_FP64, int 
     if (j0 < 0 || cnt < 0 || j0 + cnt > nf || (long)nf * nc >= (1L << 31))
         return fail(c, IBH_EINVAL, "B_columns: column range outside the force unknowns");
     int rc = ensure_B_work(c, nc);
+    if (!rc && nlev == 1) rc = ensure_level1_tables(c);
     if (rc) return rc;
     for (int j = j0; j < j0 + cnt; j += nc) {
         const int n = std::min(nc, j0 + cnt - j);
@@ -1103,6 +1119,20 @@ void lr_release(ibh_ctx *c) {
     c->lr_k = 0;
 }
 
+// level 1's combined scale κ·sAinv₁ ⊙ sD (Δinv·Ainv₁ in the sine basis, κ = nx·ny/4 from
+// S·S) and zero boundary values, for the one-level products
+int ensure_level1_tables(ibh_ctx *c) {
+    const Dom &d = c->d;
+    const long ng = d.ng;
+    if (!c->sAD) {
+        ALLOC(c->sAD, ng);
+        hipLaunchKernelGGL(k_scale_product, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
+                           c->s, c->sAD, c->sAinv, c->sD, 0.25 * d.nx * d.ny, ng);
+    }
+    if (!c->zerobc) ALLOC(c->zerobc, d.nbc);
+    return IBH_OK;
+}
+
 // the factorization for the current body box, with the regularization of the current points
 int lr_build(ibh_ctx *c) {
     const Dom &d = c->d;
@@ -1112,12 +1142,7 @@ int lr_build(ibh_ctx *c) {
     HIPCK(hipStreamSynchronize(c->s));
     drop_graphs(c);
     lr_release(c);
-    if (!c->sAD) {
-        ALLOC(c->sAD, ng);
-        hipLaunchKernelGGL(k_scale_product, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
-                           c->s, c->sAD, c->sAinv, c->sD, 0.25 * d.nx * d.ny, ng);
-    }
-    if (!c->zerobc) ALLOC(c->zerobc, d.nbc);
+    if (int rc0 = ensure_level1_tables(c)) return rc0;
     int rc = ensure_B_work(c, LR_NC);
     if (rc) return rc;
     // validation forces (host random, fixed seed) and their full products
