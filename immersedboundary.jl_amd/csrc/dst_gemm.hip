@@ -138,11 +138,16 @@ __global__ void k_sine_low(const double *S, float *f, unsigned *b, long n) {
            ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
 }
 
+// FP32: [Soe | the same with the two pairs of every 16-byte slot swapped] (n pairs each, n
+// even, rows of an even number of pairs): the LDS-direct tiles take rows r with r & 8 set from
+// the swapped copy, so that the 16 rows of a fragment read meet in different LDS banks
 __global__ void k_sine_interleave_f(const float *So, const float *Se, float *Soe, long n) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= n) return;
     Soe[2 * t] = So[t];
     Soe[2 * t + 1] = Se[t];
+    Soe[2 * n + 2 * t] = So[t ^ 1];
+    Soe[2 * n + 2 * t + 1] = Se[t ^ 1];
 }
 __global__ void k_sine_interleave(const double *So, const double *Se, double *Soe, long n) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -714,13 +719,18 @@ __device__ inline void lds_barrier() {
 
 // the interleaved sine parts of a precision: SB bytes per pair, (So, Se) of one element type
 template <class P> struct DmaSine;
+// SWAP: the array holds a second copy with the two pairs of each 16-byte slot swapped, which
+// the tile rows with bit 3 set are loaded from (their pair k then sits in the other half of the
+// slot, so rows r and r + 8 of a fragment read hit different banks)
 template <> struct DmaSine<PF64> {
     static constexpr int SB = 16;
+    static constexpr bool SWAP = false;
     typedef d2 T;
     __device__ static const void *ptr(const GemmArgs &g) { return g.Soe; }
 };
 template <> struct DmaSine<PF32> {
     static constexpr int SB = 8;
+    static constexpr bool SWAP = true;
     typedef float2 T;
     __device__ static const void *ptr(const GemmArgs &g) { return g.Soef; }
 };
@@ -779,7 +789,8 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
     __shared__ __attribute__((aligned(16))) char st0[Q::STAGE];
     __shared__ __attribute__((aligned(16))) char st1[Q::STAGE];
 
-    const __amdgpu_buffer_rsrc_t rs = wave_rsrc(DmaSine<P>::ptr(g), (long)h * h * SB);
+    constexpr bool SWAP = DmaSine<P>::SWAP;
+    const __amdgpu_buffer_rsrc_t rs = wave_rsrc(DmaSine<P>::ptr(g), (long)h * h * SB * (SWAP ? 2 : 1));
     const __amdgpu_buffer_rsrc_t rd = wave_rsrc(X, (ldx * (long)(nrd - 1) + Kd) * 8);
     // byte offsets of this lane's elements in K tile 0 (DMA_OOB for rows past the operand):
     // sine slot L of the tile holds pairs from k = PPS·((L % SPR) ^ (row & SWM)) of its row
@@ -789,7 +800,9 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
         const int L = (u * Q::NWT + wave) * 64 + lane;
         const int row = L / Q::SPR, k = PPS * ((L % Q::SPR) ^ (row & Q::SWM));
         sk[u] = k;
-        sbase[u] = r0s + row < h ? (k + h * (r0s + row)) * SB : (int)DMA_OOB;
+        sbase[u] = r0s + row < h
+                       ? (k + h * (r0s + row)) * SB + (SWAP && (row & 8) ? h * h * SB : 0)
+                       : (int)DMA_OOB;
     }
     int dbase[Q::NDI];
     const int de0 = (wave % Q::SEGS) * 32 + (lane >> 1);  // element of the tile row
@@ -840,7 +853,8 @@ __device__ __forceinline__ void eo_tile_dma(const GemmArgs &g, int bxt, int byt,
 #pragma unroll
         for (int i = 0; i < FP; ++i) {
             const int row = wp * Q::TP + i * 16 + l16;
-            const char *a = stage + (row * Q::SPR + ((k / PPS) ^ (row & Q::SWM))) * 16 + (k % PPS) * SB;
+            const int half = (k % PPS) ^ (SWAP ? (row >> 3) & 1 : 0);
+            const char *a = stage + (row * Q::SPR + ((k / PPS) ^ (row & Q::SWM))) * 16 + half * SB;
             if (WS == 2) sf[i].x = *(const E *)(a + part * (SB / 2));
             else sf[i] = *(const ST *)a;
         }

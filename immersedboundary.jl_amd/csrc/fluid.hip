@@ -375,25 +375,32 @@ __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q
     const int tx = threadIdx.x % NL_TX, ty = threadIdx.x / NL_TX;
     const int ntx = (d.m + NL_TX - 1) / NL_TX;
     const int i0 = 2 + imod(tile, ntx) * NL_TX, j0 = 2 + idiv(tile, ntx) * NL_TY;
-    // stage: QX, QY, Γ with bounds (1-based valid ranges of each block)
+    // stage: QX, QY, Γ with bounds (1-based valid ranges of each block).  With ψ, its tile
+    // (Γ's footprint, which holds every ψ value the flux of the tile's faces reads) is staged
+    // first and the flux formed from it (curl_u_g / curl_v_g: curl!'s arithmetic).
+    __shared__ double ps[NL_TY + 2][NL_TX + 2];
+    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 2); e += NL_TX * NL_TY) {
+        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
+        const int i = i0 - 2 + cidx, j = j0 - 2 + r;
+        const bool ok = i >= 1 && i <= d.m && j >= 1 && j <= d.p;
+        mem.g[r][cidx] = ok ? GI(g, i, j) : 0.0;
+        if (psi) ps[r][cidx] = ok ? GI(psi, i, j) : 0.0;
+    }
+    if (psi) __syncthreads();
+    const auto PT = [&](int a, int b) { return ps[b - j0 + 2][a - i0 + 2]; };
     for (int e = threadIdx.x; e < (NL_TY + 1) * (NL_TX + 2); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
         const int i = i0 - 1 + cidx, j = j0 - 1 + r;
         mem.qx[r][cidx] = (i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny)
-                              ? (psi ? curl_u(d, psi, pbc, i, j) : QX(q, i, j))
+                              ? (psi ? curl_u_g(d, PT, pbc, i, j) : QX(q, i, j))
                               : 0.0;
     }
     for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 1); e += NL_TX * NL_TY) {
         const int r = e / (NL_TX + 1), cidx = e % (NL_TX + 1);
         const int i = i0 - 1 + cidx, j = j0 - 1 + r;
         mem.qy[r][cidx] = (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1)
-                              ? (psi ? curl_v(d, psi, pbc, i, j) : QY(q, i, j))
+                              ? (psi ? curl_v_g(d, PT, pbc, i, j) : QY(q, i, j))
                               : 0.0;
-    }
-    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 2); e += NL_TX * NL_TY) {
-        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
-        const int i = i0 - 2 + cidx, j = j0 - 2 + r;
-        mem.g[r][cidx] = (i >= 1 && i <= d.m && j >= 1 && j <= d.p) ? GI(g, i, j) : 0.0;
     }
     __syncthreads();
     const NLTile c{d, &mem, i0, j0, gbc, U, V, gbc_scale};

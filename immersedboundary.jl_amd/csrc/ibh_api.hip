@@ -1873,7 +1873,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
         CK(dalloc(c, &c->Somb, hx * hx));
         CK(dalloc(c, &c->Semb, hx * hx));
         CK(dalloc(c, &c->Smoe, 2 * hx * hx));
-        CK(dalloc(c, &c->Smoef, 2 * hx * hx));
+        CK(dalloc(c, &c->Smoef, 4 * hx * hx));  // and the slot-swapped copy
         c->Spoe = c->Smoe;
         c->Spoef = c->Smoef;
         c->Sop = c->Som;
@@ -1890,7 +1890,7 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
             CK(dalloc(c, &c->Sopb, hy * hy));
             CK(dalloc(c, &c->Sepb, hy * hy));
             CK(dalloc(c, &c->Spoe, 2 * hy * hy));
-            CK(dalloc(c, &c->Spoef, 2 * hy * hy));
+            CK(dalloc(c, &c->Spoef, 4 * hy * hy));
         }
         if (const char *e = std::getenv("IBH_DST_SPLIT")) c->use_eo = std::atoi(e) != 0;
     }

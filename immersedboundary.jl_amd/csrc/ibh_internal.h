@@ -50,10 +50,11 @@ __device__ __forceinline__ int xcd_block() {
 // each q entry is written once by its caller.  (Inside a HIP source only: the curl kernels, E
 // and the nonlinear term evaluate it.)
 #ifdef __HIPCC__
-__device__ inline double curl_u(const Dom &d, const double *__restrict__ psi,
-                                const double *__restrict__ pbc, int i, int j) {
+// G(a, b): ψ at interior vertex (a, b) (1-based), from device memory or a staged copy
+template <class GA>
+__device__ inline double curl_u_g(const Dom &d, const GA &G, const double *__restrict__ pbc, int i,
+                                  int j) {
     const int nx = d.nx, ny = d.ny;
-    auto G = [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; };
     auto B = [&](int k) { return pbc[k - 1]; };
     if (i == 1) return B((d.Lo + 1) + j) - B(d.Lo + j);
     if (i == nx + 1) return B((d.Ro + 1) + j) - B(d.Ro + j);
@@ -61,16 +62,24 @@ __device__ inline double curl_u(const Dom &d, const double *__restrict__ psi,
     if (j == ny) return B(i + d.To) - G(i - 1, j - 1);
     return G(i - 1, j) - G(i - 1, j - 1);
 }
-__device__ inline double curl_v(const Dom &d, const double *__restrict__ psi,
-                                const double *__restrict__ pbc, int i, int j) {
+template <class GA>
+__device__ inline double curl_v_g(const Dom &d, const GA &G, const double *__restrict__ pbc, int i,
+                                  int j) {
     const int nx = d.nx, ny = d.ny;
-    auto G = [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; };
     auto B = [&](int k) { return pbc[k - 1]; };
     if (j == 1) return B(d.Bo + i) - B((d.Bo + 1) + i);
     if (j == ny + 1) return B(d.To + i) - B((d.To + 1) + i);
     if (i == 1) return B(d.Lo + j) - G(i, j - 1);
     if (i == nx) return G(i - 1, j - 1) - B(d.Ro + j);
     return G(i - 1, j - 1) - G(i, j - 1);
+}
+__device__ inline double curl_u(const Dom &d, const double *__restrict__ psi,
+                                const double *__restrict__ pbc, int i, int j) {
+    return curl_u_g(d, [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; }, pbc, i, j);
+}
+__device__ inline double curl_v(const Dom &d, const double *__restrict__ psi,
+                                const double *__restrict__ pbc, int i, int j) {
+    return curl_v_g(d, [&](int a, int b) { return psi[(a - 1) + (long)d.m * (b - 1)]; }, pbc, i, j);
 }
 __device__ inline double curl_at(const Dom &d, const double *__restrict__ psi,
                                  const double *__restrict__ pbc, long t) {
