@@ -758,54 +758,73 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
     }
 }
 
-// one 256-thread workgroup per output entry: wave w sums a quarter of the index range
-// lane-strided, the four wave sums are added in order (deterministic)
+// One 256-thread workgroup per mirrored pair of outputs: rows o and n−1−o of S share their
+// values up to the sign of the odd columns (S[n−1−o, k] = (−1)^k S[o, k]), so one read of row o
+// gives both lines' values at o and at its mirror from the even and odd sums, E ± O.  Wave w
+// sums a quarter of the index range lane-strided; the four wave sums are added in order
+// (deterministic).  Pairs [0, nx/2) are the column part ψ(o, J) = Σ_a S_m[o, a] v_J[a], then
+// [nx/2, nx/2 + ny/2) the row part ψ(I, c) = Σ_b w_I[b] S_p[b, c].
 __device__ inline void lines_out(const Dom &d, const double *__restrict__ Sm,
                                  const double *__restrict__ Sp, const double *__restrict__ Wb,
-                                 double *__restrict__ ps, int o) {
-    __shared__ double sh[2][4];
+                                 double *__restrict__ ps, int pr) {
+    __shared__ double sh[4][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool col = o < d.m;
+    const int hx = d.nx / 2;
+    if (pr >= hx + d.ny / 2) return;
+    const bool col = pr < hx;
+    const int o = col ? pr : pr - hx;
     const int n = col ? d.m : d.p;
+    const double *S = (col ? Sm : Sp) + (long)o * n;
+    const double *v1 = col ? Wb + 2L * LN_CH * d.m : Wb + (2L * LN_CH + 2) * d.m;
+    const double *v2 = col ? v1 + d.m : v1 + d.p;
     const int k0 = (int)((long)w * n / 4), k1 = (int)((long)(w + 1) * n / 4);
-    double s1 = 0.0, s2 = 0.0;
-    if (col) {  // ψ(o, J) = Σ_a S_m[o, a] v_J[a]
-        const double *v1 = Wb + 2L * LN_CH * d.m, *v2 = v1 + d.m;
-        for (int a = k0 + lane; a < k1; a += 64) {
-            const double sv = Sm[(long)o * d.m + a];
-            s1 += sv * v1[a];
-            s2 += sv * v2[a];
-        }
-    } else {    // ψ(I, c) = Σ_b w_I[b] S_p[b, c]
-        const int c = o - d.m;
-        const double *w1 = Wb + (2L * LN_CH + 2) * d.m, *w2 = w1 + d.p;
-        for (int b = k0 + lane; b < k1; b += 64) {
-            const double sv = Sp[(long)c * d.p + b];
-            s1 += w1[b] * sv;
-            s2 += w2[b] * sv;
+    double e1 = 0.0, o1 = 0.0, e2 = 0.0, o2 = 0.0;
+    for (int a = k0 + lane; a < k1; a += 64) {  // a's parity is fixed per lane
+        const double sv = S[a];
+        if (a & 1) {
+            o1 += sv * v1[a];
+            o2 += sv * v2[a];
+        } else {
+            e1 += sv * v1[a];
+            e2 += sv * v2[a];
         }
     }
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
+    e1 = wave_sum(e1);
+    o1 = wave_sum(o1);
+    e2 = wave_sum(e2);
+    o2 = wave_sum(o2);
     if (lane == 0) {
-        sh[0][w] = s1;
-        sh[1][w] = s2;
+        sh[0][w] = e1;
+        sh[1][w] = o1;
+        sh[2][w] = e2;
+        sh[3][w] = o2;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const double t1 = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
-    const double t2 = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    const double E1 = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+    const double O1 = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    const double E2 = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
+    const double O2 = ((sh[3][0] + sh[3][1]) + sh[3][2]) + sh[3][3];
+    const int mo = n - 1 - o;  // the mirror output
     if (col) {
-        ps[o + (long)d.m * (d.ny / 4 - 1)] = t1;
-        ps[o + (long)d.m * (3 * d.ny / 4 - 1)] = t2;
+        const long J1 = (long)d.m * (d.ny / 4 - 1), J2 = (long)d.m * (3 * d.ny / 4 - 1);
+        ps[o + J1] = E1 + O1;
+        ps[o + J2] = E2 + O2;
+        if (mo != o) {
+            ps[mo + J1] = E1 - O1;
+            ps[mo + J2] = E2 - O2;
+        }
     } else {
         // the four crossings with the columns belong to the column part (one writer each,
         // so the result does not depend on which workgroup finishes last)
-        const int c = o - d.m;
-        if (c != d.ny / 4 - 1 && c != 3 * d.ny / 4 - 1) {
-            ps[(d.nx / 4 - 1) + (long)d.m * c] = t1;
-            ps[(3 * d.nx / 4 - 1) + (long)d.m * c] = t2;
-        }
+        const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
+        auto put = [&](int c, double t1, double t2) {
+            if (c == d.ny / 4 - 1 || c == 3 * d.ny / 4 - 1) return;
+            ps[I1 + (long)d.m * c] = t1;
+            ps[I2 + (long)d.m * c] = t2;
+        };
+        put(o, E1 + O1, E2 + O2);
+        if (mo != o) put(mo, E1 - O1, E2 - O2);
     }
 }
 
@@ -853,7 +872,7 @@ void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double
                        F, sF, Sm, Sp, EW, D, W, sW, nca, skip);
     hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 4), nbatch), dim3(256), 0, s, d, W, sW,
                        skip);
-    hipLaunchKernelGGL(k_lines_out, dim3(d.m + d.p, nbatch), dim3(256), 0, s, d, Sm, Sp,
+    hipLaunchKernelGGL(k_lines_out, dim3(d.nx / 2 + d.ny / 2, nbatch), dim3(256), 0, s, d, Sm, Sp,
                        W, sW, psi, spsi, skip);
 }
 
@@ -1106,7 +1125,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, c
 long grp_lines_blocks(const Dom &d, int pass) {
     if (pass == 0) return nblk(d.m, 256) * (long)LN_CH + d.p;
     if (pass == 1) return nblk(d.m, 4);
-    return d.m + d.p;
+    return d.nx / 2 + d.ny / 2;  // mirrored output pairs
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, const int *off, int n) {
