@@ -309,8 +309,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
                 sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, nullptr, c->specT + lev * d.ng});
             } else {
                 if (md != SV_SPEC) {  // get_bc! fused with the copy into Γtmp + edges
-                    cg.push_back({d, rhs, c->gtmp, Psi[i] + (lev + 1) * d.ng, bcl});
-                    b2.push_back(nblk(d.ng + d.nbc, GRP_BS));
+                    cg.push_back({d, rhs, c->gtmp, Psi[i] + (lev + 1) * d.ng, bcl, 1});
+                    b2.push_back((int)grp_blocks(d.ng + d.nbc));
                 } else {
                     gb.push_back({d, Psi[i] + (lev + 1) * d.ng, 0, bcl, 0, 1});
                     b1.push_back(nblk(d.nbc, GRP_BS));

@@ -133,6 +133,7 @@ struct ibh_ctx {
     double *sAD = nullptr, *zerobc = nullptr;
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
+    double *pin_pts = nullptr;  // pinned host staging of uploaded points and velocities (2·2nb)
     // host copy of the points last given (nb×2, x then y) and whether every point's delta
     // support lies inside the finest grid (checked on the host before anything reads them)
     std::vector<double> hpts;

@@ -246,13 +246,10 @@ void launch_copy_apply_bc(const Dom &d, const double *in, long sin, double *out,
 // get_bc! and the copy with apply_bc! in one pass: rbc = get_bc!(r) (the coarser level's ψ) and
 // out = in + edges(rbc); the edge elements evaluate their boundary value from r themselves, the
 // same value the rbc entry receives.  Threads [0, nΓ) copy, [nΓ, nΓ + nbc) write rbc.
-__device__ inline void copy_getbc_at(const Dom &d, const double *__restrict__ in,
-                                     double *__restrict__ out, const double *__restrict__ r,
-                                     double *__restrict__ rbc, long t) {
-    if (t >= d.ng) {
-        if (t < d.ng + d.nbc) rbc[t - d.ng] = get_bc_at(d, r, (int)(t - d.ng));
-        return;
-    }
+// the value element t receives: out[t] for t < nΓ, rbc[t − nΓ] after
+__device__ inline double copy_getbc_val(const Dom &d, const double *__restrict__ in,
+                                        const double *__restrict__ r, long t) {
+    if (t >= d.ng) return get_bc_at(d, r, (int)(t - d.ng));
     const int i = imod(t, d.m) + 1, j = idiv(t, d.m) + 1;
     double v = in[t];
     // as copy_apply_bc_at (fac = 1): bottom, top, left, right
@@ -260,7 +257,17 @@ __device__ inline void copy_getbc_at(const Dom &d, const double *__restrict__ in
     if (j == d.ny - 1) v += get_bc_at(d, r, d.To + i);
     if (i == 1) v += get_bc_at(d, r, d.Lo + j);
     if (i == d.nx - 1) v += get_bc_at(d, r, d.Ro + j);
-    out[t] = v;
+    return v;
+}
+__device__ inline void copy_getbc_put(const Dom &d, double *__restrict__ out,
+                                      double *__restrict__ rbc, long t, double v) {
+    if (t < d.ng) out[t] = v;
+    else rbc[t - d.ng] = v;
+}
+__device__ inline void copy_getbc_at(const Dom &d, const double *__restrict__ in,
+                                     double *__restrict__ out, const double *__restrict__ r,
+                                     double *__restrict__ rbc, long t) {
+    if (t < d.ng + d.nbc) copy_getbc_put(d, out, rbc, t, copy_getbc_val(d, in, r, t));
 }
 
 __global__ void k_copy_getbc(Dom d, const double *__restrict__ in, long sin,
@@ -1033,9 +1040,19 @@ __global__ __launch_bounds__(GRP_BS) void kg_copy_apply_bc(const CopyBcA *args, 
 
 __global__ __launch_bounds__(GRP_BS) void kg_copy_getbc(const CopyGetBcA *args, const int *off,
                                                         int n) {
-    GRP_PROLOGUE(CopyGetBcA, A.d.ng + A.d.nbc)
-    (void)b;  // one batch entry per problem
-    copy_getbc_at(d, A.in, A.out, A.r, A.rbc, t);
+    GRP_PROLOGUE_N(CopyGetBcA, A.d.ng + A.d.nbc)
+    const long N = d.ng + d.nbc;
+    double v_[GRP_EPT];  // every value formed before any is stored
+#pragma unroll
+    for (int k = 0; k < GRP_EPT; ++k) {
+        const long t = t0 + (long)k * GRP_BS;
+        v_[k] = t < N ? copy_getbc_val(d, A.in, A.r, t) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < GRP_EPT; ++k) {
+        const long t = t0 + (long)k * GRP_BS;
+        if (t < N) copy_getbc_put(d, A.out, A.rbc, t, v_[k]);
+    }
 }
 
 // nl_tiles(d) workgroups per batch entry (level)

@@ -1402,8 +1402,18 @@ int upload_points(ibh_ctx *c, const double *r, const double *v) {
         rc = update_box(c, r);
         if (rc) return rc;
     }
-    if (r) HIPCK(hipMemcpyAsync(c->pts, r, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
-    if (v) HIPCK(hipMemcpyAsync(c->vel, v, sizeof(double) * 2 * c->nb, hipMemcpyHostToDevice, c->s));
+    // through pinned staging, so the copies are queued behind the running step without waiting
+    // for it (the staging is reused only after the next step's status read has passed them)
+    const size_t nf = 2 * (size_t)c->nb;
+    if (r) {
+        std::memcpy(c->pin_pts, r, sizeof(double) * nf);
+        HIPCK(hipMemcpyAsync(c->pts, c->pin_pts, sizeof(double) * nf, hipMemcpyHostToDevice, c->s));
+    }
+    if (v) {
+        std::memcpy(c->pin_pts + nf, v, sizeof(double) * nf);
+        HIPCK(hipMemcpyAsync(c->vel, c->pin_pts + nf, sizeof(double) * nf, hipMemcpyHostToDevice,
+                             c->s));
+    }
     return IBH_OK;
 }
 
@@ -1925,6 +1935,11 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     const int nf = 2 * nb;
     CK(dalloc(c, &c->pts, nf));
     CK(dalloc(c, &c->vel, nf));
+    if (nf && hipHostMalloc((void **)&c->pin_pts, sizeof(double) * 2 * nf, hipHostMallocDefault) !=
+                  hipSuccess) {
+        c->pin_pts = nullptr;
+        return fail(c, IBH_EDEVICE, "hipHostMalloc of the points' staging buffer failed");
+    }
     CK(dalloc(c, &c->ds, nb));
     CK(dalloc(c, &c->d_boff, D.nbodies + 1));
     CK(dalloc(c, &c->d_npts, D.nbodies + 1));
@@ -2014,6 +2029,7 @@ void ibh_destroy(ibh_ctx *c) {
     for (hipEvent_t e : c->prof.pool) hipEventDestroy(e);
     if (c->prof.ts) hipFree(c->prof.ts);
     if (c->rb) rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb));
+    if (c->pin_pts) hipHostFree(c->pin_pts);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }

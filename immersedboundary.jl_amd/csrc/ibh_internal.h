@@ -441,8 +441,8 @@ struct RotIA { Dom d; const double *img; const double *scale; double *g; int nba
 struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
 struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
 struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
-// get_bc! of r into rbc and out = in + edges(rbc) (one batch entry); blocks ceil((nΓ + nbc)/256)
-struct CopyGetBcA { Dom d; const double *in; double *out; const double *r; double *rbc; };
+// get_bc! of r into rbc and out = in + edges(rbc) (one batch entry); blocks grp_blocks(nΓ + nbc)
+struct CopyGetBcA { Dom d; const double *in; double *out; const double *r; double *rbc; int nbatch; };
 // the nonlinear term's output tiles per level (NL_TX × NL_TY outputs each)
 constexpr int NL_TX = 64, NL_TY = 4;
 __host__ __device__ inline long nl_tiles(const Dom &d) {
