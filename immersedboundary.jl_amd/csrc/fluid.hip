@@ -712,38 +712,53 @@ __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int
     lines_colsum(d, W + blockIdx.y * sW, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
 }
 
-// one 256-thread workgroup per column b: wave w sums a quarter of the rows lane-strided, the
-// four wave sums are added in order (deterministic)
+// The line passes give each output LN_W waves of a workgroup (LN_OUTS outputs per 256-thread
+// workgroup): wave h of an output sums the 64-aligned chunk h of the index range lane-strided,
+// and the chunks' wave sums are added in order (deterministic; the single-case and the grouped
+// kernels run the same arithmetic).
+constexpr int LN_W = 2, LN_OUTS = 4 / LN_W;
+__device__ inline int ln_chunk(int n, int h) {  // first index of chunk h (a multiple of 64)
+    const int c = ((n + 64 * LN_W - 1) / (64 * LN_W)) * 64;
+    return min(n, h * c);
+}
+
+// row sums w_I[b] = Σ_a S_m[I, a]·F[a, b] for output column b
 __device__ inline void lines_row(const Dom &d, const double *__restrict__ F, long sF,
                                  const double *__restrict__ Sm, const double *__restrict__ Sp,
                                  const double *__restrict__ EW, const double *__restrict__ D,
                                  double *__restrict__ W, long sW, int b, int bt) {
-    __shared__ double sh[2][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double *Fc = F + bt * sF + (long)d.m * b;
-    const double *Dc = D + (long)d.m * b;
-    const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
-    const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
-    const int a0 = (int)((long)w * d.m / 4), a1 = (int)((long)(w + 1) * d.m / 4);
+    __shared__ double sh[LN_OUTS][LN_W][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, slot = wv / LN_W, h = wv % LN_W;
     double s1 = 0.0, s2 = 0.0;
-    for (int a = a0 + lane; a < a1; a += 64) {
-        double f = Fc[a];
-        if (Eb) f += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
-        s1 += Sm[(long)I1 * d.m + a] * f;
-        s2 += Sm[(long)I2 * d.m + a] * f;
+    if (b < d.p) {
+        const double *Fc = F + bt * sF + (long)d.m * b;
+        const double *Dc = D + (long)d.m * b;
+        const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
+        const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
+        const int a1 = ln_chunk(d.m, h + 1);
+        for (int a = ln_chunk(d.m, h) + lane; a < a1; a += 64) {
+            double f = Fc[a];
+            if (Eb) f += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
+            s1 += Sm[(long)I1 * d.m + a] * f;
+            s2 += Sm[(long)I2 * d.m + a] * f;
+        }
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
     }
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
     if (lane == 0) {
-        sh[0][w] = s1;
-        sh[1][w] = s2;
+        sh[slot][h][0] = s1;
+        sh[slot][h][1] = s2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
-        Wr[b] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];        // w_I1
-        Wr[d.p + b] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];  // w_I2
+    if (b >= d.p || lane != 0 || h != 0) return;
+    double t1 = sh[slot][0][0], t2 = sh[slot][0][1];
+    for (int k = 1; k < LN_W; ++k) {
+        t1 += sh[slot][k][0];
+        t2 += sh[slot][k][1];
     }
+    double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
+    Wr[b] = t1;        // w_I1
+    Wr[d.p + b] = t2;  // w_I2
 }
 
 // the column chunks (blocks [0, nca·LN_CH)) and the row sums (the blocks after) in one launch
@@ -761,57 +776,58 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
         lines_colpart(d, F, sF, Sm, Sp, EW, D, W, sW, (blk % nca) * 256 + threadIdx.x, blk / nca,
                       bt);
     } else {
-        lines_row(d, F, sF, Sm, Sp, EW, D, W, sW, blk - nca * LN_CH, bt);
+        lines_row(d, F, sF, Sm, Sp, EW, D, W, sW,
+                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6) / LN_W, bt);
     }
 }
 
-// One 256-thread workgroup per mirrored pair of outputs: rows o and n−1−o of S share their
-// values up to the sign of the odd columns (S[n−1−o, k] = (−1)^k S[o, k]), so one read of row o
-// gives both lines' values at o and at its mirror from the even and odd sums, E ± O.  Wave w
-// sums a quarter of the index range lane-strided; the four wave sums are added in order
-// (deterministic).  Pairs [0, nx/2) are the column part ψ(o, J) = Σ_a S_m[o, a] v_J[a], then
-// [nx/2, nx/2 + ny/2) the row part ψ(I, c) = Σ_b w_I[b] S_p[b, c].
+// The mirrored pairs of outputs: rows o and n−1−o of S share their values up to the sign of the
+// odd columns (S[n−1−o, k] = (−1)^k S[o, k]), so one read of row o gives both lines' values at o
+// and at its mirror from the even and odd sums, E ± O (lane l of a wave sums the columns of
+// parity l, the chunks being 64-aligned).  Pairs [0, nx/2) are the column part
+// ψ(o, J) = Σ_a S_m[o, a] v_J[a], then [nx/2, nx/2 + ny/2) the row part ψ(I, c) = Σ_b w_I[b] S_p[b, c].
 __device__ inline void lines_out(const Dom &d, const double *__restrict__ Sm,
                                  const double *__restrict__ Sp, const double *__restrict__ Wb,
                                  double *__restrict__ ps, int pr) {
-    __shared__ double sh[4][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double sh[LN_OUTS][LN_W][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, slot = wv / LN_W, h = wv % LN_W;
     const int hx = d.nx / 2;
-    if (pr >= hx + d.ny / 2) return;
+    const bool live = pr < hx + d.ny / 2;
     const bool col = pr < hx;
     const int o = col ? pr : pr - hx;
     const int n = col ? d.m : d.p;
-    const double *S = (col ? Sm : Sp) + (long)o * n;
-    const double *v1 = col ? Wb + 2L * LN_CH * d.m : Wb + (2L * LN_CH + 2) * d.m;
-    const double *v2 = col ? v1 + d.m : v1 + d.p;
-    const int k0 = (int)((long)w * n / 4), k1 = (int)((long)(w + 1) * n / 4);
-    double e1 = 0.0, o1 = 0.0, e2 = 0.0, o2 = 0.0;
-    for (int a = k0 + lane; a < k1; a += 64) {  // a's parity is fixed per lane
-        const double sv = S[a];
-        if (a & 1) {
-            o1 += sv * v1[a];
-            o2 += sv * v2[a];
-        } else {
-            e1 += sv * v1[a];
-            e2 += sv * v2[a];
+    double s1 = 0.0, s2 = 0.0;
+    if (live) {
+        const double *S = (col ? Sm : Sp) + (long)o * n;
+        const double *v1 = col ? Wb + 2L * LN_CH * d.m : Wb + (2L * LN_CH + 2) * d.m;
+        const double *v2 = col ? v1 + d.m : v1 + d.p;
+        const int a1 = ln_chunk(n, h + 1);
+        for (int a = ln_chunk(n, h) + lane; a < a1; a += 64) {
+            const double sv = S[a];
+            s1 += sv * v1[a];
+            s2 += sv * v2[a];
         }
     }
-    e1 = wave_sum(e1);
-    o1 = wave_sum(o1);
-    e2 = wave_sum(e2);
-    o2 = wave_sum(o2);
+    for (int k = 32; k > 1; k >>= 1) {  // even lanes into lane 0, odd lanes into lane 1
+        s1 += __shfl_down(s1, k, 64);
+        s2 += __shfl_down(s2, k, 64);
+    }
+    const double o1 = __shfl_down(s1, 1, 64), o2 = __shfl_down(s2, 1, 64);
     if (lane == 0) {
-        sh[0][w] = e1;
-        sh[1][w] = o1;
-        sh[2][w] = e2;
-        sh[3][w] = o2;
+        sh[slot][h][0] = s1;
+        sh[slot][h][1] = o1;
+        sh[slot][h][2] = s2;
+        sh[slot][h][3] = o2;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    const double E1 = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
-    const double O1 = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
-    const double E2 = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
-    const double O2 = ((sh[3][0] + sh[3][1]) + sh[3][2]) + sh[3][3];
+    if (!live || lane != 0 || h != 0) return;
+    double E1 = sh[slot][0][0], O1 = sh[slot][0][1], E2 = sh[slot][0][2], O2 = sh[slot][0][3];
+    for (int k = 1; k < LN_W; ++k) {
+        E1 += sh[slot][k][0];
+        O1 += sh[slot][k][1];
+        E2 += sh[slot][k][2];
+        O2 += sh[slot][k][3];
+    }
     const int mo = n - 1 - o;  // the mirror output
     if (col) {
         const long J1 = (long)d.m * (d.ny / 4 - 1), J2 = (long)d.m * (3 * d.ny / 4 - 1);
@@ -841,7 +857,8 @@ __global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restri
                                                    double *__restrict__ psi, long spsi,
                                                    const int *skip) {
     SKIP_GUARD
-    lines_out(d, Sm, Sp, W + blockIdx.y * sW, psi + blockIdx.y * spsi, blockIdx.x);
+    lines_out(d, Sm, Sp, W + blockIdx.y * sW, psi + blockIdx.y * spsi,
+              blockIdx.x * LN_OUTS + (threadIdx.x >> 6) / LN_W);
 }
 
 long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d.p; }
@@ -875,11 +892,11 @@ void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double
                   const double *EW, const double *D) {
     const long sW = lines_work_size(d);
     const int nca = (int)nblk(d.m, 256);
-    hipLaunchKernelGGL(k_lines_part, dim3(nca * LN_CH + d.p, nbatch), dim3(256), 0, s, d,
+    hipLaunchKernelGGL(k_lines_part, dim3(nca * LN_CH + nblk(d.p, LN_OUTS), nbatch), dim3(256), 0, s, d,
                        F, sF, Sm, Sp, EW, D, W, sW, nca, skip);
     hipLaunchKernelGGL(k_lines_colsum, dim3(nblk(d.m, 4), nbatch), dim3(256), 0, s, d, W, sW,
                        skip);
-    hipLaunchKernelGGL(k_lines_out, dim3(d.nx / 2 + d.ny / 2, nbatch), dim3(256), 0, s, d, Sm, Sp,
+    hipLaunchKernelGGL(k_lines_out, dim3(nblk(d.nx / 2 + d.ny / 2, LN_OUTS), nbatch), dim3(256), 0, s, d, Sm, Sp,
                        W, sW, psi, spsi, skip);
 }
 
@@ -1140,9 +1157,9 @@ __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, c
 }
 
 long grp_lines_blocks(const Dom &d, int pass) {
-    if (pass == 0) return nblk(d.m, 256) * (long)LN_CH + d.p;
+    if (pass == 0) return nblk(d.m, 256) * (long)LN_CH + nblk(d.p, LN_OUTS);  // row sums
     if (pass == 1) return nblk(d.m, 4);
-    return d.nx / 2 + d.ny / 2;  // mirrored output pairs
+    return nblk(d.nx / 2 + d.ny / 2, LN_OUTS);  // mirrored output pairs
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, const int *off, int n) {
@@ -1153,7 +1170,8 @@ __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, cons
         lines_colpart(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0, (blk % nca) * 256 + threadIdx.x,
                       blk / nca, 0);
     else
-        lines_row(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0, blk - nca * LN_CH, 0);
+        lines_row(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0,
+                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6) / LN_W, 0);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, const int *off,
@@ -1166,7 +1184,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, co
 __global__ __launch_bounds__(GRP_BS) void kg_lines_out(const LinesA *args, const int *off, int n) {
     const int p = grp_find(off, n, XB);
     const LinesA A = args[p];
-    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, XB - off[p]);
+    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, (XB - off[p]) * LN_OUTS + (threadIdx.x >> 6) / LN_W);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_axpy_sub(const AxpyA *args, const int *off, int n) {
