@@ -162,6 +162,7 @@ struct SolveItem {
     double *psi;         // the level's ψ
     const double *bc;    // SPEC: the level's boundary values (null: zero)
     const double *spec;  // SPEC: the kept spectrum
+    const double *sub;   // SPEC: subtracted from it (the projection's spectrum), or null
 };
 
 void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem> &items,
@@ -182,7 +183,7 @@ void add_solves(ibh_batch *b, std::vector<Stage> &v, const std::vector<SolveItem
                 edb.push_back((int)nblk(d.nx / 2 + d.ny / 2, GRP_BS / 64));  // a wave per pair
             }
             sp.push_back({d, it.spec, c->sD, 0.25 * d.nx * d.ny, c->Sm, c->Sp,
-                          it.bc ? c->edgeS : nullptr, c->T2});
+                          it.bc ? c->edgeS : nullptr, c->T2, it.sub});
             spb.push_back(nblk(d.ng, GRP_BS));
             left1.push_back(P.g[2]);  // reads T2
             right2.push_back(P.g[3]);
@@ -264,6 +265,11 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
     auto rlevels = [&](const ibh_ctx *c) {
         return final_pass ? c->refine_final_levels : c->refine_levels;
     };
+    // the final pass's level 1 from the spectrum: Γ₁ = S(F_trial − F_proj)S for cases whose
+    // projection kept its spectrum
+    auto sub = [&](const ibh_ctx *c, int lev) -> const double * {
+        return final_pass && lev == 0 && keeps_spectrum(c) && c->nb > 0 ? c->specP : nullptr;
+    };
     auto mode = [&](const ibh_ctx *c, int lev) {
         if (!keeps_spectrum(c)) return (int)SV_FULL;
         if (lev == 0) return (int)SV_SPEC;
@@ -306,7 +312,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
             const int md = mode(c, lev);
             const double *rhs = G[i] + lev * d.ng;
             if (lev == c->L - 1) {  // the coarsest level: ψ_L = Δinv Γ_L
-                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, nullptr, c->specT + lev * d.ng});
+                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, nullptr, c->specT + lev * d.ng,
+                               sub(c, lev)});
             } else {
                 if (md != SV_SPEC) {  // get_bc! fused with the copy into Γtmp + edges
                     cg.push_back({d, rhs, c->gtmp, Psi[i] + (lev + 1) * d.ng, bcl, 1});
@@ -316,7 +323,8 @@ void add_vort2flux(ibh_batch *b, std::vector<Stage> &v, const std::vector<double
                     b1.push_back(nblk(d.nbc, GRP_BS));
                 }
                 rhs = c->gtmp;
-                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng});
+                sv.push_back({i, md, rhs, Psi[i] + lev * d.ng, bcl, c->specT + lev * d.ng,
+                              sub(c, lev)});
             }
             if (sweeps(c) && lev < rlevels(c))
                 rf.push_back({i, Psi[i] + lev * d.ng, rhs, -1.0, c->sD, 0, 1, sweeps(c)});
@@ -432,8 +440,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         std::vector<StressA> sa;
         std::vector<ETA> et;
         std::vector<RotIA> ro;
-        std::vector<AxpyA> ax;
-        std::vector<int> b1, b2, b3, b4, b5;
+        std::vector<int> b1, b2, b3, b4;
         std::vector<LapInvPlan> plans;
         for (int i = 0; i < n; ++i) {
             ibh_ctx *c = b->cs[i];
@@ -443,18 +450,25 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
                           c->CL, c->fbs});
             b1.push_back(c->nbodies);
             // Eᵀ into rot: zeroing and scale in one stage, rot reads the fixed-point image
-            et.push_back({d, c->reg, c->Fb, c->qtmp, c->etscale});
-            b2.push_back(nblk(d.nq, GRP_BS) + 1);
+            // the image cleared and rot formed on the body box (the case keeps qtmp and g1
+            // zero outside it, as its single-case projection does)
+            GBox bx;
+            if (c->box_on) {
+                bx.x0 = c->bxa;
+                bx.x1 = c->bxb;
+                bx.y0 = c->bya;
+                bx.y1 = c->byb;
+            }
+            et.push_back({d, c->reg, c->Fb, c->qtmp, c->etscale, bx});
+            b2.push_back(nblk(gbox_faces(d, bx), GRP_BS) + 1);
             b3.push_back(nblk(2L * c->nb * GRP_NS * GRP_NS, GRP_BS));
-            ro.push_back({d, c->qtmp, c->etscale, c->g1, 1});
-            b4.push_back(nblk(d.ng, GRP_BS));
+            ro.push_back({d, c->qtmp, c->etscale, c->g1, 1, bx});
+            b4.push_back(nblk(gbox_vertices(d, bx), GRP_BS));
             LapInvPlan P;
             lapinv_plan(c, c->g1, d.ng, gs[i], d.ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr, &P);
-            if (keeps_spectrum(c)) {  // Γ₁ = S(F_trial − F_proj)S: keep F_proj and subtract it
-                P.g[1].C = c->specP;
+            if (keeps_spectrum(c)) {  // Γ₁ = S(F_trial − F_proj)S: keep F_proj (the final
+                P.g[1].C = c->specP;   // pass's level-1 solve reads F_trial − F_proj)
                 P.g[2].B = c->specP;
-                ax.push_back({d.ng, c->specT, c->specP, 1.0});
-                b5.push_back(nblk(d.ng, GRP_BS));
             }
             if (c->box_on) {  // rot Eᵀ Fb vanishes outside the body box
                 P.g[0].klo = c->bxa / 2;
@@ -479,7 +493,7 @@ void build_step(ibh_batch *b, int P, std::vector<Stage> &v) {
         }
         add(v, OP_ROTI, ro, b4);
         add_lapinv(v, plans, IBH_FP64, false);  // the projection stays FP64
-        add(v, OP_AXPY, ax, b5);
+
     }
     add_vort2flux(b, v, gs, psi, q, true, true);
 }

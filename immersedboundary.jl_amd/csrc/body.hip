@@ -778,12 +778,13 @@ __global__ __launch_bounds__(GRP_BS) void kg_ET_zero_scale(const ETA *args, cons
     const int p = grp_find(off, n, blockIdx.x);
     const ETA A = args[p];
     const long lb = blockIdx.x - off[p];
-    if (lb == (long)nblk(A.d.nq, GRP_BS)) {
+    const long nfaces = gbox_faces(A.d, A.box);  // the image is kept zero outside the box
+    if (lb == (long)nblk(nfaces, GRP_BS)) {
         et_scale_body(A.fb, 2 * A.r.nb, A.scale);
         return;
     }
     const long t = lb * GRP_BS + threadIdx.x;
-    if (t < A.d.nq) A.q[t] = 0.0;
+    if (t < nfaces) A.q[gbox_face(A.d, A.box, t)] = 0.0;
 }
 
 // one workgroup per problem

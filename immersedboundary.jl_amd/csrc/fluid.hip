@@ -867,12 +867,13 @@ __global__ void k_spectral_scale(Dom d, const double *__restrict__ A, long sA,
                                  const double *__restrict__ D, double kappa,
                                  const double *__restrict__ Sm, const double *__restrict__ Sp,
                                  const double *__restrict__ EW, double *__restrict__ F, long sF,
-                                 const int *skip) {
+                                 const int *skip, const double *__restrict__ sub) {
     SKIP_GUARD
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= d.ng) return;
     const int b = blockIdx.y;
-    double v = kappa * A[b * sA + t] * D[t];
+    const double a = sub ? A[b * sA + t] - sub[b * sA + t] : A[b * sA + t];
+    double v = kappa * a * D[t];
     if (EW) {
         const int a = imod(t, d.m), c = idiv(t, d.m);
         v += D[t] * edge_term(d, Sm, Sp, EW + b * 2L * (d.m + d.p), a, c);
@@ -882,9 +883,10 @@ __global__ void k_spectral_scale(Dom d, const double *__restrict__ A, long sA,
 
 void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
                            double *F, long sF, int nbatch, const int *skip, hipStream_t s,
-                           const double *Sm, const double *Sp, const double *EW) {
+                           const double *Sm, const double *Sp, const double *EW,
+                           const double *sub) {
     hipLaunchKernelGGL(k_spectral_scale, dim3(nblk(d.ng, 256), nbatch), dim3(256), 0, s, d, A, sA,
-                       D, kappa, Sm, Sp, EW, F, sF, skip);
+                       D, kappa, Sm, Sp, EW, F, sF, skip, sub);
 }
 
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
@@ -1030,10 +1032,11 @@ __global__ __launch_bounds__(GRP_BS) void kg_rot(const RotA *args, const int *of
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_rot_image(const RotIA *args, const int *off, int n) {
-    GRP_PROLOGUE(RotIA, A.d.ng)
+    GRP_PROLOGUE(RotIA, gbox_vertices(A.d, A.box))
     (void)b;  // one batch entry per problem
-    if (t >= d.ng) return;
-    A.g[t] = rot_image_at(d, reinterpret_cast<const long long *>(A.img), 1.0 / *A.scale, t);
+    if (t >= gbox_vertices(d, A.box)) return;
+    const long v = gbox_vertex(d, A.box, t);  // Γ is kept zero outside the box
+    A.g[v] = rot_image_at(d, reinterpret_cast<const long long *>(A.img), 1.0 / *A.scale, v);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_coarsify(const CoarsA *args, const int *off, int n) {
@@ -1151,7 +1154,8 @@ __global__ __launch_bounds__(GRP_BS) void kg_spectral_scale(const SpecA *args, c
     GRP_PROLOGUE(SpecA, A.d.ng)
     (void)b;  // one batch entry per problem
     if (t >= d.ng) return;
-    double v = A.kappa * A.A[t] * A.D[t];
+    const double a = A.sub ? A.A[t] - A.sub[t] : A.A[t];
+    double v = A.kappa * a * A.D[t];
     if (A.EW) v += A.D[t] * edge_term(d, A.Sm, A.Sp, A.EW, imod(t, d.m), idiv(t, d.m));
     A.F[t] = v;
 }

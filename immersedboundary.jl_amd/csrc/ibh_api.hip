@@ -582,7 +582,8 @@ void pair_range(int lo, int hi, int n, int *plo, int *phi) {
 void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc, bool curl_all,
                const int *skip, int prec = -1, int refine = 0, int nlev = -1,
                bool sparse = false, const double *spec0 = nullptr, bool lines_ok = false,
-               int rlev = -1, int prec_coarse = -1, bool skip_q = false) {
+               int rlev = -1, int prec_coarse = -1, bool skip_q = false,
+               const double *spec0_sub = nullptr) {
     const Dom &d = c->d;
     if (rlev < 0) rlev = c->refine_levels;  // refined levels (the finest rlev)
     const int Ls = c->L, L = nlev > 0 ? std::min(nlev, Ls) : Ls;
@@ -610,7 +611,7 @@ void vort2flux(ibh_ctx *c, double *G, double *Psi, double *Q, double *bc, int nc
             lapinv_plan(c, nullptr, 0, Pl, sG, c->sD, 0, nc, EPI_STORE, nullptr, skip, &P);
             if (bcl) launch_edge_gemv(d, c->Sm, c->Sp, bcl, sB, edgeW, nc, skip, c->s);
             launch_spectral_scale(d, spec0, ng, c->sD, 0.25 * d.nx * d.ny, c->T2, ng, nc, skip,
-                                  c->s, c->Sm, c->Sp, bcl ? edgeW : nullptr);
+                                  c->s, c->Sm, c->Sp, bcl ? edgeW : nullptr, spec0_sub);
             if (sparse && c->box_on) {
                 // E reads level 1 only inside the body box: the x transform forms the pairs
                 // whose rows meet it, the y transform its rows and the pairs of its columns
@@ -1540,7 +1541,6 @@ void seg_post(ibh_ctx *c) {
             }
             for (int k = 0; k < 4; ++k)
                 gemm(c, P.g[k], P.epi[k], IBH_FP64, P.kind[k] == 1, P.kind[k]);
-            launch_axpy_sub((int)ng, c->specT, c->specP, 1.0, c->s);
         } else {
             lapinv(c, c->g1, ng, c->gam, ng, c->sAinv, 0, 1, EPI_SUBFROM, nullptr, nullptr,
                    IBH_FP64);
@@ -1548,9 +1548,10 @@ void seg_post(ibh_ctx *c) {
     }
     // ---- vort2flux!(qty) (fluid-ops.jl:438); the finest level is refined like the trial's (its
     // flux feeds the next step's nonlinear term next to the bodies)
+    // level 1's spectrum is F_trial − F_proj (formed where the spectral scale reads it)
     vort2flux(c, c->gam, c->psi, c->q, c->psibc, 1, true, nullptr, -1, c->refine_final, -1,
               false, fuse ? c->specT : nullptr, false, c->refine_final_levels,
-              c->mixed ? IBH_FP32 : -1, true);
+              c->mixed ? IBH_FP32 : -1, true, fuse && nf ? c->specP : nullptr);
 }
 
 // status word: [CG done, CG iterations, error flags]

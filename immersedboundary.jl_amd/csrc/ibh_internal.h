@@ -257,7 +257,7 @@ long lines_work_size(const Dom &d);
 void launch_spectral_scale(const Dom &d, const double *A, long sA, const double *D, double kappa,
                            double *F, long sF, int nbatch, const int *skip, hipStream_t s,
                            const double *Sm = nullptr, const double *Sp = nullptr,
-                           const double *EW = nullptr);
+                           const double *EW = nullptr, const double *sub = nullptr);
 // EW (optional, from launch_edge_gemv): read F + D ⊙ (rank-4 boundary term) instead of F.
 void launch_lines(const Dom &d, const double *Sm, const double *Sp, const double *F, long sF,
                   double *psi, long spsi, double *W, int nbatch, const int *skip, hipStream_t s,
@@ -437,7 +437,7 @@ __device__ inline int grp_find(const int *off, int n, int b) {
 struct CurlA { Dom d; const double *psi; long spsi; const double *pbc; long sbc; double *q; long sq; int nbatch; };
 struct RotA { Dom d; const double *q; long sq; double *g; long sg; int nbatch; };
 // rot of Eᵀ's fixed-point image (img, as written by the ET stages, scale one double)
-struct RotIA { Dom d; const double *img; const double *scale; double *g; int nbatch; };
+struct RotIA { Dom d; const double *img; const double *scale; double *g; int nbatch; GBox box; };
 struct CoarsA { Dom d; double *gc; const double *g; long sgc, sg; int nbatch; };
 struct GetBcA { Dom d; const double *r; long sr; double *rbc; long sbc; int nbatch; };
 struct CopyBcA { Dom d; const double *in; long sin; double *out; long sout; const double *rbc; long sbc; double fac; int nbatch; };
@@ -461,7 +461,7 @@ struct RhsA {
 struct ResidA { Dom d; const double *x, *rhs; double a, ar; double *r; int nbatch; };
 struct FillA { double *p; long n; double v; };
 struct EA { Dom d; RegDev r; const double *q; double *fb; const double *uv; double uvscale; };
-struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; };
+struct ETA { Dom d; RegDev r; const double *fb; double *q; double *scale; GBox box; };
 struct GemvA { int n; const double *A; const double *x; double *y; };  // A transposed
 struct StressA {
     int nbodies; const int *boff, *npts; int nb; const double *Fb, *ds; double h, dt;
@@ -472,7 +472,8 @@ struct StressA {
 // one batch entry per problem; W of EdgeA holds 2(m + p), of LinesA lines_work_size(d)
 // doubles), and y -= a·x.
 struct EdgeA { Dom d; const double *Sm, *Sp, *bc; double *W; };
-struct SpecA { Dom d; const double *A, *D; double kappa; const double *Sm, *Sp, *EW; double *F; };
+// sub (or null): the spectrum read is A − sub (the trial's minus the projection's)
+struct SpecA { Dom d; const double *A, *D; double kappa; const double *Sm, *Sp, *EW; double *F; const double *sub; };
 struct LinesA { Dom d; const double *F, *Sm, *Sp, *EW, *D; double *W, *psi; };
 struct AxpyA { long n; double *y; const double *x; double a; };
 // workgroups per problem of the three lines passes
