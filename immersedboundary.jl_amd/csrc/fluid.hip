@@ -385,29 +385,80 @@ __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q
     // stage: QX, QY, Γ with bounds (1-based valid ranges of each block).  With ψ, its tile
     // (Γ's footprint, which holds every ψ value the flux of the tile's faces reads) is staged
     // first and the flux formed from it (curl_u_g / curl_v_g: curl!'s arithmetic).
+    // Every global load of the staging is issued into registers before the first LDS store
+    // (fixed trip counts, unrolled), so a wave has all of them in flight at once instead of
+    // waiting out one load per loop iteration.
+    constexpr int NT = NL_TX * NL_TY;
+    constexpr int GW = NL_TX + 2, GE = (NL_TY + 2) * GW, GIT = (GE + NT - 1) / NT;
+    constexpr int XW = NL_TX + 2, XE = (NL_TY + 1) * XW, XIT = (XE + NT - 1) / NT;
+    constexpr int YW = NL_TX + 1, YE = (NL_TY + 2) * YW, YIT = (YE + NT - 1) / NT;
     __shared__ double ps[NL_TY + 2][NL_TX + 2];
-    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 2); e += NL_TX * NL_TY) {
-        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
+    double gv[GIT], pv[GIT], xv[XIT], yv[YIT];
+#pragma unroll
+    for (int k = 0; k < GIT; ++k) {
+        const int e = threadIdx.x + k * NT;
+        const int r = e / GW, cidx = e - r * GW;
         const int i = i0 - 2 + cidx, j = j0 - 2 + r;
-        const bool ok = i >= 1 && i <= d.m && j >= 1 && j <= d.p;
-        mem.g[r][cidx] = ok ? GI(g, i, j) : 0.0;
-        if (psi) ps[r][cidx] = ok ? GI(psi, i, j) : 0.0;
+        const bool ok = e < GE && i >= 1 && i <= d.m && j >= 1 && j <= d.p;
+        gv[k] = ok ? GI(g, i, j) : 0.0;
+        pv[k] = (ok && psi) ? GI(psi, i, j) : 0.0;
     }
-    if (psi) __syncthreads();
-    const auto PT = [&](int a, int b) { return ps[b - j0 + 2][a - i0 + 2]; };
-    for (int e = threadIdx.x; e < (NL_TY + 1) * (NL_TX + 2); e += NL_TX * NL_TY) {
-        const int r = e / (NL_TX + 2), cidx = e % (NL_TX + 2);
-        const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-        mem.qx[r][cidx] = (i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny)
-                              ? (psi ? curl_u_g(d, PT, pbc, i, j) : QX(q, i, j))
-                              : 0.0;
+    if (!psi) {
+#pragma unroll
+        for (int k = 0; k < XIT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            const int r = e / XW, cidx = e - r * XW;
+            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+            xv[k] = (e < XE && i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny) ? QX(q, i, j) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < YIT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            const int r = e / YW, cidx = e - r * YW;
+            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+            yv[k] = (e < YE && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1) ? QY(q, i, j) : 0.0;
+        }
     }
-    for (int e = threadIdx.x; e < (NL_TY + 2) * (NL_TX + 1); e += NL_TX * NL_TY) {
-        const int r = e / (NL_TX + 1), cidx = e % (NL_TX + 1);
-        const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-        mem.qy[r][cidx] = (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1)
-                              ? (psi ? curl_v_g(d, PT, pbc, i, j) : QY(q, i, j))
-                              : 0.0;
+#pragma unroll
+    for (int k = 0; k < GIT; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < GE) {
+            const int r = e / GW, cidx = e - r * GW;
+            mem.g[r][cidx] = gv[k];
+            if (psi) ps[r][cidx] = pv[k];
+        }
+    }
+    if (psi) {
+        __syncthreads();
+        const auto PT = [&](int a, int b) { return ps[b - j0 + 2][a - i0 + 2]; };
+#pragma unroll
+        for (int k = 0; k < XIT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            const int r = e / XW, cidx = e - r * XW;
+            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+            xv[k] = (e < XE && i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny)
+                        ? curl_u_g(d, PT, pbc, i, j)
+                        : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < YIT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            const int r = e / YW, cidx = e - r * YW;
+            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
+            yv[k] = (e < YE && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1)
+                        ? curl_v_g(d, PT, pbc, i, j)
+                        : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < XIT; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < XE) mem.qx[e / XW][e % XW] = xv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < YIT; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < YE) mem.qy[e / YW][e % YW] = yv[k];
     }
     __syncthreads();
     const NLTile c{d, &mem, i0, j0, gbc, U, V, gbc_scale};
@@ -679,12 +730,25 @@ __device__ inline void lines_colpart(const Dom &d, const double *__restrict__ F,
     const int J1 = d.ny / 4 - 1, J2 = 3 * d.ny / 4 - 1;
     const int b0 = (int)((long)ch * d.p / LN_CH), b1 = (int)((long)(ch + 1) * d.p / LN_CH);
     double s1 = 0.0, s2 = 0.0;
-    for (int b = b0; b < b1; ++b) {
-        const long t = a + (long)d.m * b;
-        double f = Fb[t];
-        if (Eb) f += D[t] * edge_term(d, Sm, Sp, Eb, a, b);
-        s1 += f * Sp[(long)J1 * d.p + b];
-        s2 += f * Sp[(long)J2 * d.p + b];
+    // four rows' loads issued together; the sums still run over b in order (padding adds 0)
+    for (int b = b0; b < b1; b += 4) {
+        double f[4], p1[4], p2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int bb = b + u;
+            const bool ok = bb < b1;
+            const long t = a + (long)d.m * bb;
+            double x = ok ? Fb[t] : 0.0;
+            if (Eb && ok) x += D[t] * edge_term(d, Sm, Sp, Eb, a, bb);
+            f[u] = x;
+            p1[u] = ok ? Sp[(long)J1 * d.p + bb] : 0.0;
+            p2[u] = ok ? Sp[(long)J2 * d.p + bb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s1 += f[u] * p1[u];
+            s2 += f[u] * p2[u];
+        }
     }
     double *Wb = W + bt * sW;
     Wb[((long)a * LN_CH + ch) * 2] = s1;  // chunks of one entry contiguous: [a][ch][J]
@@ -712,53 +776,50 @@ __global__ void k_lines_colsum(Dom d, double *__restrict__ W, long sW, const int
     lines_colsum(d, W + blockIdx.y * sW, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
 }
 
-// The line passes give each output LN_W waves of a workgroup (LN_OUTS outputs per 256-thread
-// workgroup): wave h of an output sums the 64-aligned chunk h of the index range lane-strided,
-// and the chunks' wave sums are added in order (deterministic; the single-case and the grouped
-// kernels run the same arithmetic).
-constexpr int LN_W = 2, LN_OUTS = 4 / LN_W;
-__device__ inline int ln_chunk(int n, int h) {  // first index of chunk h (a multiple of 64)
-    const int c = ((n + 64 * LN_W - 1) / (64 * LN_W)) * 64;
-    return min(n, h * c);
-}
+// The row sums and the line values give each output one wave of a 256-thread workgroup
+// (LN_OUTS outputs per workgroup).  Lane l sums the indices l, l + 64, ... in order; the loads
+// of LN_U of them are issued together before their products are added, and the lane partials
+// go through a fixed-order wave sum (deterministic; the single-case and the grouped kernels run
+// the same arithmetic).
+constexpr int LN_OUTS = 4, LN_U = 8;
 
 // row sums w_I[b] = Σ_a S_m[I, a]·F[a, b] for output column b
 __device__ inline void lines_row(const Dom &d, const double *__restrict__ F, long sF,
                                  const double *__restrict__ Sm, const double *__restrict__ Sp,
                                  const double *__restrict__ EW, const double *__restrict__ D,
                                  double *__restrict__ W, long sW, int b, int bt) {
-    __shared__ double sh[LN_OUTS][LN_W][2];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, slot = wv / LN_W, h = wv % LN_W;
+    if (b >= d.p) return;  // whole wave
+    const int lane = threadIdx.x & 63;
+    const double *Fc = F + bt * sF + (long)d.m * b;
+    const double *Dc = D + (long)d.m * b;
+    const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
+    const double *S1 = Sm + (long)(d.nx / 4 - 1) * d.m, *S2 = Sm + (long)(3 * d.nx / 4 - 1) * d.m;
     double s1 = 0.0, s2 = 0.0;
-    if (b < d.p) {
-        const double *Fc = F + bt * sF + (long)d.m * b;
-        const double *Dc = D + (long)d.m * b;
-        const double *Eb = EW ? EW + bt * 2L * (d.m + d.p) : nullptr;
-        const int I1 = d.nx / 4 - 1, I2 = 3 * d.nx / 4 - 1;
-        const int a1 = ln_chunk(d.m, h + 1);
-        for (int a = ln_chunk(d.m, h) + lane; a < a1; a += 64) {
-            double f = Fc[a];
-            if (Eb) f += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
-            s1 += Sm[(long)I1 * d.m + a] * f;
-            s2 += Sm[(long)I2 * d.m + a] * f;
+    for (int a0 = 0; a0 < d.m; a0 += 64 * LN_U) {
+        double f[LN_U], m1[LN_U], m2[LN_U];
+#pragma unroll
+        for (int u = 0; u < LN_U; ++u) {
+            const int a = a0 + 64 * u + lane;
+            const bool ok = a < d.m;
+            double x = ok ? Fc[a] : 0.0;
+            if (Eb && ok) x += Dc[a] * edge_term(d, Sm, Sp, Eb, a, b);
+            f[u] = x;
+            m1[u] = ok ? S1[a] : 0.0;
+            m2[u] = ok ? S2[a] : 0.0;
         }
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
+#pragma unroll
+        for (int u = 0; u < LN_U; ++u) {
+            s1 += m1[u] * f[u];
+            s2 += m2[u] * f[u];
+        }
     }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
     if (lane == 0) {
-        sh[slot][h][0] = s1;
-        sh[slot][h][1] = s2;
+        double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
+        Wr[b] = s1;        // w_I1
+        Wr[d.p + b] = s2;  // w_I2
     }
-    __syncthreads();
-    if (b >= d.p || lane != 0 || h != 0) return;
-    double t1 = sh[slot][0][0], t2 = sh[slot][0][1];
-    for (int k = 1; k < LN_W; ++k) {
-        t1 += sh[slot][k][0];
-        t2 += sh[slot][k][1];
-    }
-    double *Wr = W + bt * sW + (2L * LN_CH + 2) * d.m;
-    Wr[b] = t1;        // w_I1
-    Wr[d.p + b] = t2;  // w_I2
 }
 
 // the column chunks (blocks [0, nca·LN_CH)) and the row sums (the blocks after) in one launch
@@ -777,7 +838,7 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
                       bt);
     } else {
         lines_row(d, F, sF, Sm, Sp, EW, D, W, sW,
-                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6) / LN_W, bt);
+                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6), bt);
     }
 }
 
@@ -789,45 +850,39 @@ __global__ __launch_bounds__(256) void k_lines_part(Dom d, const double *__restr
 __device__ inline void lines_out(const Dom &d, const double *__restrict__ Sm,
                                  const double *__restrict__ Sp, const double *__restrict__ Wb,
                                  double *__restrict__ ps, int pr) {
-    __shared__ double sh[LN_OUTS][LN_W][4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, slot = wv / LN_W, h = wv % LN_W;
     const int hx = d.nx / 2;
-    const bool live = pr < hx + d.ny / 2;
+    if (pr >= hx + d.ny / 2) return;  // whole wave
+    const int lane = threadIdx.x & 63;
     const bool col = pr < hx;
     const int o = col ? pr : pr - hx;
     const int n = col ? d.m : d.p;
+    const double *S = (col ? Sm : Sp) + (long)o * n;
+    const double *v1 = col ? Wb + 2L * LN_CH * d.m : Wb + (2L * LN_CH + 2) * d.m;
+    const double *v2 = col ? v1 + d.m : v1 + d.p;
     double s1 = 0.0, s2 = 0.0;
-    if (live) {
-        const double *S = (col ? Sm : Sp) + (long)o * n;
-        const double *v1 = col ? Wb + 2L * LN_CH * d.m : Wb + (2L * LN_CH + 2) * d.m;
-        const double *v2 = col ? v1 + d.m : v1 + d.p;
-        const int a1 = ln_chunk(n, h + 1);
-        for (int a = ln_chunk(n, h) + lane; a < a1; a += 64) {
-            const double sv = S[a];
-            s1 += sv * v1[a];
-            s2 += sv * v2[a];
+    for (int a0 = 0; a0 < n; a0 += 64 * LN_U) {
+        double sv[LN_U], x1[LN_U], x2[LN_U];
+#pragma unroll
+        for (int u = 0; u < LN_U; ++u) {
+            const int a = a0 + 64 * u + lane;
+            const bool ok = a < n;
+            sv[u] = ok ? S[a] : 0.0;
+            x1[u] = ok ? v1[a] : 0.0;
+            x2[u] = ok ? v2[a] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LN_U; ++u) {
+            s1 += sv[u] * x1[u];
+            s2 += sv[u] * x2[u];
         }
     }
     for (int k = 32; k > 1; k >>= 1) {  // even lanes into lane 0, odd lanes into lane 1
         s1 += __shfl_down(s1, k, 64);
         s2 += __shfl_down(s2, k, 64);
     }
-    const double o1 = __shfl_down(s1, 1, 64), o2 = __shfl_down(s2, 1, 64);
-    if (lane == 0) {
-        sh[slot][h][0] = s1;
-        sh[slot][h][1] = o1;
-        sh[slot][h][2] = s2;
-        sh[slot][h][3] = o2;
-    }
-    __syncthreads();
-    if (!live || lane != 0 || h != 0) return;
-    double E1 = sh[slot][0][0], O1 = sh[slot][0][1], E2 = sh[slot][0][2], O2 = sh[slot][0][3];
-    for (int k = 1; k < LN_W; ++k) {
-        E1 += sh[slot][k][0];
-        O1 += sh[slot][k][1];
-        E2 += sh[slot][k][2];
-        O2 += sh[slot][k][3];
-    }
+    const double O1 = __shfl_down(s1, 1, 64), O2 = __shfl_down(s2, 1, 64);
+    if (lane != 0) return;
+    const double E1 = s1, E2 = s2;
     const int mo = n - 1 - o;  // the mirror output
     if (col) {
         const long J1 = (long)d.m * (d.ny / 4 - 1), J2 = (long)d.m * (3 * d.ny / 4 - 1);
@@ -858,7 +913,7 @@ __global__ __launch_bounds__(256) void k_lines_out(Dom d, const double *__restri
                                                    const int *skip) {
     SKIP_GUARD
     lines_out(d, Sm, Sp, W + blockIdx.y * sW, psi + blockIdx.y * spsi,
-              blockIdx.x * LN_OUTS + (threadIdx.x >> 6) / LN_W);
+              blockIdx.x * LN_OUTS + (threadIdx.x >> 6));
 }
 
 long lines_work_size(const Dom &d) { return 2L * LN_CH * d.m + 2L * d.m + 2L * d.p; }
@@ -1175,7 +1230,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_lines_part(const LinesA *args, cons
                       blk / nca, 0);
     else
         lines_row(A.d, A.F, 0, A.Sm, A.Sp, A.EW, A.D, A.W, 0,
-                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6) / LN_W, 0);
+                  (blk - nca * LN_CH) * LN_OUTS + (threadIdx.x >> 6), 0);
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, const int *off,
@@ -1188,7 +1243,7 @@ __global__ __launch_bounds__(GRP_BS) void kg_lines_colsum(const LinesA *args, co
 __global__ __launch_bounds__(GRP_BS) void kg_lines_out(const LinesA *args, const int *off, int n) {
     const int p = grp_find(off, n, XB);
     const LinesA A = args[p];
-    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, (XB - off[p]) * LN_OUTS + (threadIdx.x >> 6) / LN_W);
+    lines_out(A.d, A.Sm, A.Sp, A.W, A.psi, (XB - off[p]) * LN_OUTS + (threadIdx.x >> 6));
 }
 
 __global__ __launch_bounds__(GRP_BS) void kg_axpy_sub(const AxpyA *args, const int *off, int n) {
