@@ -293,175 +293,170 @@ void launch_apply_bc(const Dom &d, double *r, long sr, const double *rbc, long s
 
 // Nonlinear term at one level: avg_flux! + direct_product! + rot! + 1/hc² (cnab/ops.jl:151-168,
 // fluid-ops.jl:277-401).  q0 is the constant base flux (ux·hc, uy·hc) (fluid-ops.jl:403-417).
-// The face products are written once against an accessor of q, Γ and Γbc: NLGlobal reads
-// device memory, NLTile a workgroup's LDS copy; both give the same operands to the same
-// arithmetic.
-struct NLGlobal {
-    Dom d;
-    const double *q, *g, *gbc;
-    double U, V, gs;  // q0x, q0y values; scale of gbc
-    __device__ double qx(int i, int j) const { return QX(q, i, j); }
-    __device__ double qy(int i, int j) const { return QY(q, i, j); }
-    __device__ double gi(int i, int j) const { return GI(g, i, j); }
-};
-
-// one tile's operands in LDS: QX over i ∈ [i0-1, i0+TX], j ∈ [j0-1, j0+TY-1]; QY over
-// i ∈ [i0-1, i0+TX-1], j ∈ [j0-1, j0+TY]; Γ over i ∈ [i0-2, i0+TX-1], j ∈ [j0-2, j0+TY-1]
-// (1-based, zero outside the grid: the products never read those)
+//
+// One NL_TX×NL_TY tile of outputs per 256-thread workgroup.  Γ and ψ are staged in LDS together
+// with their boundary ring: position 0 and nx of the x index, 0 and ny of the y index hold the
+// boundary values (Γbc·scale, ψbc) that direct_product! and curl! read at the domain edges
+// (fluid-ops.jl:316-349, 177-208).  With the ring in place the boundary cases of the flux
+// (curl!) and of the face products are the interior formulas, so every thread runs the same
+// arithmetic whatever its position.  The one exception is the ψ ring's corners: the x-flux
+// reads the left and right columns' values there, the y-flux rows j = 1 and j = ny + 1 the
+// bottom and top rows' (they are separate entries of ψbc), so those two y-flux rows read ψbc
+// directly.  Γ's corners are never read.
+// Tile extents (1-based): Γ, ψ over i ∈ [i0-2, i0+TX-1], j ∈ [j0-2, j0+TY-1]; QX over
+// i ∈ [i0-1, i0+TX], j ∈ [j0-1, j0+TY-1]; QY over i ∈ [i0-1, i0+TX-1], j ∈ [j0-1, j0+TY]; zero
+// outside the grid and its ring (the products never read those).
 struct NLTileMem {
     double qx[NL_TY + 1][NL_TX + 2];
     double qy[NL_TY + 2][NL_TX + 1];
     double g[NL_TY + 2][NL_TX + 2];
 };
 struct NLTile {
-    Dom d;
     const NLTileMem *m;
     int i0, j0;
-    const double *gbc;
-    double U, V, gs;
+    double U, V;
     __device__ double qx(int i, int j) const { return m->qx[j - j0 + 1][i - i0 + 1]; }
     __device__ double qy(int i, int j) const { return m->qy[j - j0 + 1][i - i0 + 1]; }
     __device__ double gi(int i, int j) const { return m->g[j - j0 + 2][i - i0 + 2]; }
 };
 
-template <class C> __device__ inline double Qx_avg(const C &c, int i, int j) {  // j in 2..ny
+__device__ inline double Qx_avg(const NLTile &c, int i, int j) {  // j in 2..ny
     return (c.qx(i, j) + c.qx(i, j - 1) + c.U + c.U) / 2;
 }
-template <class C> __device__ inline double Qy_avg(const C &c, int i, int j) {  // i in 2..nx
+__device__ inline double Qy_avg(const NLTile &c, int i, int j) {  // i in 2..nx
     return (c.qy(i, j) + c.qy(i - 1, j) + c.V + c.V) / 2;
 }
-template <class C> __device__ inline double gbcv(const C &c, int k) { return c.gbc[k - 1] * c.gs; }
-
-// fqx[i, j], i in 1..nx+1, j in 1..ny (zero outside i = 2..nx)
-template <class C> __device__ inline double fqx(const C &c, int i, int j) {
-    const Dom &d = c.d;
-    if (i < 2 || i > d.nx) return 0.0;
-    if (j == 1) return (Qy_avg(c, i, j + 1) * c.gi(i - 1, j) + Qy_avg(c, i, j) * gbcv(c, d.Bo + i)) / 2;
-    if (j == d.ny)
-        return (Qy_avg(c, i, j) * c.gi(i - 1, j - 1) + Qy_avg(c, i, j + 1) * gbcv(c, d.To + i)) / 2;
+// fqx[i, j], i in 2..nx, j in 1..ny (Γ(i-1, 0) and Γ(i-1, ny) from the ring)
+__device__ inline double fqx(const NLTile &c, int i, int j) {
     return (Qy_avg(c, i, j + 1) * c.gi(i - 1, j) + Qy_avg(c, i, j) * c.gi(i - 1, j - 1)) / 2;
 }
-// fqy[i, j], i in 1..nx, j in 1..ny+1 (zero outside j = 2..ny)
-template <class C> __device__ inline double fqy(const C &c, int i, int j) {
-    const Dom &d = c.d;
-    if (j < 2 || j > d.ny) return 0.0;
-    if (i == 1)
-        return -(Qx_avg(c, i + 1, j) * c.gi(i, j - 1) + Qx_avg(c, i, j) * gbcv(c, d.Lo + j)) / 2;
-    if (i == d.nx)
-        return -(Qx_avg(c, i, j) * c.gi(i - 1, j - 1) + Qx_avg(c, i + 1, j) * gbcv(c, d.Ro + j)) / 2;
+// fqy[i, j], i in 1..nx, j in 2..ny (Γ(0, j-1) and Γ(nx, j-1) from the ring)
+__device__ inline double fqy(const NLTile &c, int i, int j) {
     return -(Qx_avg(c, i + 1, j) * c.gi(i, j - 1) + Qx_avg(c, i, j) * c.gi(i - 1, j - 1)) / 2;
 }
 
-__device__ inline double nonlinear_at(const Dom &d, const double *__restrict__ q,
-                                      const double *__restrict__ g,
-                                      const double *__restrict__ gbc, double gbc_scale,
-                                      double U, double V, double hc, long t) {
-    const NLGlobal c{d, q, g, gbc, U, V, gbc_scale};
-    const int i = imod(t, d.m) + 2, j = idiv(t, d.m) + 2;
-    double v = (fqx(c, i, j - 1) - fqx(c, i, j)) + (fqy(c, i, j) - fqy(c, i - 1, j));
-    return v * (1 / (hc * hc));
-}
-
-// The same values over an NL_TX×NL_TY tile of outputs per 256-thread workgroup.  The tile's
-// operands (with their one- or two-wide halo) are first copied to LDS in coalesced rows, so each
-// value is loaded once per workgroup instead of about twelve times per output through the
-// vector caches; then each thread forms the face products it owns, fqx(i, j) and fqy(i, j),
-// and the tile's first row and column also the ones just below and left of the tile, so every
-// product is formed once; the sum is nonlinear_at's, term for term.
-// With psi set, the staged flux values are curl!(ψ, ψbc)'s (curl_u, curl_v: the values the curl
-// kernels would have stored in q), and q is not read.
+// The staging maps threads onto tile rows and columns directly: the first NL_TX columns of a
+// block run one row per wave (rows ty, ty + NL_TY, ...), the remaining one or two columns go to
+// the first threads.  Every global load of a thread is issued into registers before the first
+// LDS store, so a wave has them all in flight together.
+// With psi set the flux is curl!(ψ, ψbc) formed from the staged ψ (curl!'s arithmetic: one
+// difference per face), and q is not read.
 __device__ inline void nonlinear_tile(const Dom &d, const double *__restrict__ q,
                                       const double *__restrict__ g,
                                       const double *__restrict__ gbc, double gbc_scale, double U,
                                       double V, double hc, long tile, double *__restrict__ out,
                                       const double *__restrict__ psi = nullptr,
                                       const double *__restrict__ pbc = nullptr) {
+    static_assert(NL_TX == 64 && NL_TX * NL_TY == 256, "a wave per tile row");
+    constexpr int NT = NL_TX * NL_TY;
     __shared__ NLTileMem mem;
+    __shared__ double ps[NL_TY + 2][NL_TX + 2];
     __shared__ double fx[NL_TY + 1][NL_TX];
     __shared__ double fy[NL_TY][NL_TX + 1];
-    const int tx = threadIdx.x % NL_TX, ty = threadIdx.x / NL_TX;
+    const int tid = threadIdx.x, tx = tid % NL_TX, ty = tid / NL_TX;
     const int ntx = (d.m + NL_TX - 1) / NL_TX;
     const int i0 = 2 + imod(tile, ntx) * NL_TX, j0 = 2 + idiv(tile, ntx) * NL_TY;
-    // stage: QX, QY, Γ with bounds (1-based valid ranges of each block).  With ψ, its tile
-    // (Γ's footprint, which holds every ψ value the flux of the tile's faces reads) is staged
-    // first and the flux formed from it (curl_u_g / curl_v_g: curl!'s arithmetic).
-    // Every global load of the staging is issued into registers before the first LDS store
-    // (fixed trip counts, unrolled), so a wave has all of them in flight at once instead of
-    // waiting out one load per loop iteration.
-    constexpr int NT = NL_TX * NL_TY;
-    constexpr int GW = NL_TX + 2, GE = (NL_TY + 2) * GW, GIT = (GE + NT - 1) / NT;
-    constexpr int XW = NL_TX + 2, XE = (NL_TY + 1) * XW, XIT = (XE + NT - 1) / NT;
-    constexpr int YW = NL_TX + 1, YE = (NL_TY + 2) * YW, YIT = (YE + NT - 1) / NT;
-    __shared__ double ps[NL_TY + 2][NL_TX + 2];
-    double gv[GIT], pv[GIT], xv[XIT], yv[YIT];
+    const int nx = d.nx, ny = d.ny;
+
+    // slots of a thread in a block of R rows and NL_TX + X columns
+    constexpr int GR = NL_TY + 2, GX = 2, GS = (GR + NL_TY - 1) / NL_TY + 1;
+    auto slot = [&](int k, int R, int X, int &r, int &cc) {
+        const int nm = (R + NL_TY - 1) / NL_TY;
+        if (k < nm) {
+            r = ty + k * NL_TY;
+            cc = tx;
+            return r < R;
+        }
+        r = tid / X;
+        cc = NL_TX + tid % X;
+        return tid < R * X;
+    };
+    // Γ and ψ with the ring
+    double gv[GS], pv[GS];
 #pragma unroll
-    for (int k = 0; k < GIT; ++k) {
-        const int e = threadIdx.x + k * NT;
-        const int r = e / GW, cidx = e - r * GW;
-        const int i = i0 - 2 + cidx, j = j0 - 2 + r;
-        const bool ok = e < GE && i >= 1 && i <= d.m && j >= 1 && j <= d.p;
-        gv[k] = ok ? GI(g, i, j) : 0.0;
-        pv[k] = (ok && psi) ? GI(psi, i, j) : 0.0;
+    for (int k = 0; k < GS; ++k) {
+        int r, cc;
+        const bool live = slot(k, GR, GX, r, cc);
+        const int a = i0 - 2 + cc, b = j0 - 2 + r;
+        double x = 0.0, y = 0.0;
+        if (live && a >= 1 && a <= d.m && b >= 1 && b <= d.p) {
+            x = GI(g, a, b);
+            if (psi) y = GI(psi, a, b);
+        } else if (live && a >= 0 && a <= nx && b >= 0 && b <= ny) {
+            const bool side = a == 0 || a == nx;
+            const int kb = a == 0 ? d.Lo + b : a == nx ? d.Ro + b : b == 0 ? d.Bo + a : d.To + a;
+            if (!(side && (b == 0 || b == ny))) x = gbc[kb] * gbc_scale;
+            if (psi) y = pbc[kb];
+        }
+        gv[k] = x;
+        pv[k] = y;
     }
+    constexpr int XR = NL_TY + 1, XS = (XR + NL_TY - 1) / NL_TY + 1;
+    constexpr int YR = NL_TY + 2, YS = (YR + NL_TY - 1) / NL_TY + 1;
+    double xv[XS], yv[YS];
     if (!psi) {
 #pragma unroll
-        for (int k = 0; k < XIT; ++k) {
-            const int e = threadIdx.x + k * NT;
-            const int r = e / XW, cidx = e - r * XW;
-            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-            xv[k] = (e < XE && i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny) ? QX(q, i, j) : 0.0;
+        for (int k = 0; k < XS; ++k) {
+            int r, cc;
+            const bool live = slot(k, XR, 2, r, cc);
+            const int i = i0 - 1 + cc, j = j0 - 1 + r;
+            xv[k] = (live && i >= 1 && i <= nx + 1 && j >= 1 && j <= ny) ? QX(q, i, j) : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < YIT; ++k) {
-            const int e = threadIdx.x + k * NT;
-            const int r = e / YW, cidx = e - r * YW;
-            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-            yv[k] = (e < YE && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1) ? QY(q, i, j) : 0.0;
+        for (int k = 0; k < YS; ++k) {
+            int r, cc;
+            const bool live = slot(k, YR, 1, r, cc);
+            const int i = i0 - 1 + cc, j = j0 - 1 + r;
+            yv[k] = (live && i >= 1 && i <= nx && j >= 1 && j <= ny + 1) ? QY(q, i, j) : 0.0;
         }
     }
 #pragma unroll
-    for (int k = 0; k < GIT; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < GE) {
-            const int r = e / GW, cidx = e - r * GW;
-            mem.g[r][cidx] = gv[k];
-            if (psi) ps[r][cidx] = pv[k];
+    for (int k = 0; k < GS; ++k) {
+        int r, cc;
+        if (slot(k, GR, GX, r, cc)) {
+            mem.g[r][cc] = gv[k];
+            if (psi) ps[r][cc] = pv[k];
         }
     }
     if (psi) {
         __syncthreads();
-        const auto PT = [&](int a, int b) { return ps[b - j0 + 2][a - i0 + 2]; };
 #pragma unroll
-        for (int k = 0; k < XIT; ++k) {
-            const int e = threadIdx.x + k * NT;
-            const int r = e / XW, cidx = e - r * XW;
-            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-            xv[k] = (e < XE && i >= 1 && i <= d.nx + 1 && j >= 1 && j <= d.ny)
-                        ? curl_u_g(d, PT, pbc, i, j)
-                        : 0.0;
+        for (int k = 0; k < XS; ++k) {  // QX(i, j) = ψ(i-1, j) − ψ(i-1, j-1)
+            int r, cc;
+            const bool live = slot(k, XR, 2, r, cc);
+            const int i = i0 - 1 + cc, j = j0 - 1 + r;
+            xv[k] = (live && i >= 1 && i <= nx + 1 && j >= 1 && j <= ny) ? ps[r + 1][cc] - ps[r][cc]
+                                                                       : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < YIT; ++k) {
-            const int e = threadIdx.x + k * NT;
-            const int r = e / YW, cidx = e - r * YW;
-            const int i = i0 - 1 + cidx, j = j0 - 1 + r;
-            yv[k] = (e < YE && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny + 1)
-                        ? curl_v_g(d, PT, pbc, i, j)
-                        : 0.0;
+        for (int k = 0; k < YS; ++k) {  // QY(i, j) = ψ(i-1, j-1) − ψ(i, j-1)
+            int r, cc;
+            const bool live = slot(k, YR, 1, r, cc);
+            const int i = i0 - 1 + cc, j = j0 - 1 + r;
+            double v = 0.0;
+            if (live && i >= 1 && i <= nx && j >= 1 && j <= ny + 1) {
+                if (j == 1)
+                    v = pbc[d.Bo + i - 1] - pbc[d.Bo + i];
+                else if (j == ny + 1)
+                    v = pbc[d.To + i - 1] - pbc[d.To + i];
+                else
+                    v = ps[r][cc] - ps[r][cc + 1];
+            }
+            yv[k] = v;
         }
     }
 #pragma unroll
-    for (int k = 0; k < XIT; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < XE) mem.qx[e / XW][e % XW] = xv[k];
+    for (int k = 0; k < XS; ++k) {
+        int r, cc;
+        if (slot(k, XR, 2, r, cc)) mem.qx[r][cc] = xv[k];
     }
 #pragma unroll
-    for (int k = 0; k < YIT; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < YE) mem.qy[e / YW][e % YW] = yv[k];
+    for (int k = 0; k < YS; ++k) {
+        int r, cc;
+        if (slot(k, YR, 1, r, cc)) mem.qy[r][cc] = yv[k];
     }
     __syncthreads();
-    const NLTile c{d, &mem, i0, j0, gbc, U, V, gbc_scale};
+    const NLTile c{&mem, i0, j0, U, V};
     const int i = i0 + tx, j = j0 + ty;
     const bool in = i <= d.m + 1 && j <= d.p + 1;
     if (in) {
