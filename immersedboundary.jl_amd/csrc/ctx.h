@@ -134,6 +134,7 @@ struct ibh_ctx {
     // bodies
     double *pts = nullptr, *vel = nullptr, *ds = nullptr;
     double *pin_pts = nullptr;  // pinned host staging of uploaded points and velocities (2·2nb)
+    int *pin_stat = nullptr;    // pinned host words: the status read [0, 3) and the flag read [3]
     // host copy of the points last given (nb×2, x then y) and whether every point's delta
     // support lies inside the finest grid (checked on the host before anything reads them)
     std::vector<double> hpts;

@@ -815,8 +815,10 @@ void B_times(ibh_ctx *c, const double *z, long sz, double *x, long sx, int nc, c
 
 int check_flags(ibh_ctx *c) {
     int f = 0;
-    HIPCK(hipMemcpyAsync(&f, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    int *h = c->pin_stat ? c->pin_stat + 3 : &f;
+    HIPCK(hipMemcpyAsync(h, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
     HIPCK(hipStreamSynchronize(c->s));
+    f = *h;
     if (f) {
         HIPCK(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
         if (f & 1) return fail(c, IBH_EBOUNDS, "a body point's delta support leaves the finest grid");
@@ -1555,9 +1557,13 @@ void seg_post(ibh_ctx *c) {
 }
 
 // status word: [CG done, CG iterations, error flags]
+// The status words (and the error flag among them) are read through pinned host memory: one
+// copy into it instead of a staged copy into pageable memory, on the path of every CG step.
 int read_status(ibh_ctx *c, int *st) {
-    HIPCK(hipMemcpyAsync(st, c->status, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s));
+    int *h = c->pin_stat ? c->pin_stat : st;
+    HIPCK(hipMemcpyAsync(h, c->status, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s));
     HIPCK(hipStreamSynchronize(c->s));
+    if (h != st) std::memcpy(st, h, 3 * sizeof(int));
     return IBH_OK;
 }
 
@@ -1946,6 +1952,10 @@ int ibh_create(const ibh_case_desc *desc, ibh_ctx **out) {
     CK(dalloc(c, &c->d_npts, D.nbodies + 1));
     CK(dalloc(c, &c->status, 4));
     c->derr = c->status + 2;
+    if (hipHostMalloc((void **)&c->pin_stat, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        c->pin_stat = nullptr;
+        return fail(c, IBH_EDEVICE, "hipHostMalloc of the status words failed");
+    }
     CK(dalloc(c, &c->uv, 2));
     if (const char *ge = std::getenv("IBH_GRAPH")) c->use_graphs = std::atoi(ge) != 0;
     CK(dalloc(c, &c->reg.bidx, nf));
@@ -2031,6 +2041,7 @@ void ibh_destroy(ibh_ctx *c) {
     if (c->prof.ts) hipFree(c->prof.ts);
     if (c->rb) rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb));
     if (c->pin_pts) hipHostFree(c->pin_pts);
+    if (c->pin_stat) hipHostFree(c->pin_stat);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
 }
