@@ -100,6 +100,30 @@ def test_nonlinear(ibhip):
         assert rel(case.nonlinear(lev, q, q0, g, gbc), orc.nonlinear(lev, q, q0, g, gbc)) < 1e-14
 
 
+def test_nonlinear_rectangular_tiles(ibhip):
+    # 148×68 cells: Γ is 147×67, so the 64×4 output tiles come three to a row (the last 19
+    # wide) and seventeen to a column (the last 3 tall); tiles meet the left, right, bottom and
+    # top boundaries alone, in corners, and not at all, each reading Γbc from the tile's ring
+    from ibhip.problem import BodyGroup, MultiLevel, Problem, SchemeCNAB, StreamFcnFluid, UniformGrid
+
+    h = 0.05
+    grid = UniformGrid(h, (0.0, 148 * h), (0.0, 68 * h))
+    prob = Problem(StreamFcnFluid(MultiLevel(grid, 2), (1.0, 0.0), 100.0), BodyGroup(),
+                   SchemeCNAB(0.01))
+    case = ibhip.DeviceCase(prob)
+    orc = ibref.OracleRun(prob, setup=False)
+    nx, ny = case.nx, case.ny
+    assert (nx, ny) == (148, 68)
+    rng = np.random.default_rng(11)
+    for lev in range(2):
+        hc = prob.fluid.domain.gridstep(lev + 1)
+        q = rng.standard_normal(case.nq)
+        q0 = np.concatenate([np.full((nx + 1) * ny, 1.0 * hc), np.full(nx * (ny + 1), -0.3 * hc)])
+        g = rng.standard_normal(case.ng)
+        gbc = rng.standard_normal(case.nbc)
+        assert rel(case.nonlinear(lev, q, q0, g, gbc), orc.nonlinear(lev, q, q0, g, gbc)) < 1e-14
+
+
 def test_vort2flux(ibhip):
     prob, case, orc = small_case(ibhip, n=64, nlev=3)
     g = np.random.default_rng(5).standard_normal((case.ng, 3))
